@@ -1,0 +1,106 @@
+/*
+ * lbm_mi355x.h -- C ABI of the MI355X sparse D3Q19 MRT library (liblbm_mi355x.so).
+ *
+ * Drop-in boundary for the hot path of the jianhui-lbm porous-media solvers.  The reference
+ * has no library interface: its path is a set of free C++ functions over global state inside
+ * two programs,
+ *   SP = Single_Phase_3D/SINGLE_PHASE_MPI_Software_Spars.cpp
+ *   CG = MP_3D/Multi_Component_CG_MPI_SPARSE_Software.cpp
+ * Each entry point below names the reference routine(s) it replaces (file:line).
+ *
+ * Conventions
+ *   - Every function returns LBM_OK (0) or a negative LBM_ERR_* code; nothing exits.
+ *     lbm_last_error() gives a message for the most recent failure on a handle.
+ *   - A handle owns all of its device memory.  Host pointers are borrowed for the duration
+ *     of the call only.  One host thread per handle at a time.
+ *   - All work is ordered on the handle's own HIP stream.  Calls that return data to the host
+ *     (get_state, diagnostics) synchronise that stream.
+ *   - Geometry: int32 solid flags indexed [i][j][k] (z fastest), 0 = pore, non-zero = solid,
+ *     exactly the reference's binary geometry file (SP:1058-1065).  The box is periodic in
+ *     x, y and z (the reference's x-slab ring closes on itself, SP:1954-1999).
+ *   - Fluid nodes keep the reference's numbering: i-major (x slowest, z fastest) over pore
+ *     voxels (SP:1172-1193), 0-based here (the reference is 1-based with a padding record).
+ *   - Host state arrays are FP64 in the reference's layouts: f[count][19] in the solver's own
+ *     velocity ordering (SP:157-159 for single phase, CG:203-205 for colour gradient),
+ *     u[count][3], scalars [count].
+ */
+#ifndef LBM_MI355X_H
+#define LBM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes */
+#define LBM_OK 0
+#define LBM_ERR_ARG (-22)       /* invalid argument */
+#define LBM_ERR_NOMEM (-12)     /* device or host allocation failed */
+#define LBM_ERR_HIP (-5)        /* HIP runtime error (message has the detail) */
+#define LBM_ERR_STATE (-71)     /* call not valid in the handle's current state */
+#define LBM_ERR_BUSY (-16)      /* serving: no room for the request right now */
+
+/* ------------------------------------------------------------------ precision */
+#define LBM_FP64 0   /* populations stored and collided in FP64 (parity build) */
+#define LBM_FP32 1   /* FP32 populations stored as f_i - w_i*rho0, FP32 collide */
+#define LBM_FP16 2   /* FP16 storage of (f_i - w_i*rho0)*scale, FP32 collide */
+
+typedef struct lbm_handle lbm_handle;
+
+/* ------------------------------------------------------------------ single phase */
+typedef struct lbm_sp_params {
+    int32_t nx, ny, nz;   /* box size in voxels (the counts of INPUT.dat line 3) */
+    double nu;            /* kinematic viscosity in_vis (SP:1301): tau = 3 nu + 1/2 */
+    double g[3];          /* body force gx, gy, gz (Guo forcing, SP:1519-1593) */
+    double rho0;          /* rest density of the shifted storage, 1.0 in the reference */
+    int32_t precision;    /* LBM_FP64 / LBM_FP32 / LBM_FP16 */
+    double fp16_scale;    /* LBM_FP16 only: storage scale (power of two); 0 = default 4096 */
+    int32_t device;       /* HIP device ordinal */
+} lbm_sp_params;
+
+typedef struct lbm_sp_diag {
+    double Q[3];        /* sum of u over fluid nodes (in the box if given) -- Comput_Perm, SP:4617-4645 */
+    double err;         /* sqrt(sum|u-u0|^2)/(sqrt(sum|u|^2)+1e-30), last node skipped -- Error, SP:3651-3664 */
+    double umax;        /* max |u| over the same nodes (SP:3660-3665) */
+    double sum_speed;   /* sum |u| over the same nodes (SP:3659), u_ave = sum_speed/(V*porosity) */
+    double nan_count;   /* number of nodes whose u is not finite (the reference's NaN stop, SP:838) */
+    double count_in_box;/* fluid nodes inside the box (all nodes if no box) */
+} lbm_sp_diag;
+
+/* Geometry, sparse numbering, neighbour lists and device allocation.
+ * Replaces Parallelize_Geometry + init_Sparse_read_rock_parallel + Suppliment + the
+ * allocations of main (SP:972-1145, SP:1147-1277, SP:486-534). */
+int lbm_sp_create(const lbm_sp_params *p, const int32_t *solid_ijk, lbm_handle **out);
+/* number of fluid nodes (Count) */
+int lbm_sp_count(const lbm_handle *h, int64_t *count);
+/* f = feq(rho, u0), u = u0, rho everywhere -- init (SP:1282-1383) */
+int lbm_sp_init(lbm_handle *h, double rho, const double u0[3]);
+/* load a post-streaming state, e.g. a reference checkpoint -- Backup_init (SP:4715-4828).
+ * u and rho are used by the first collision as given, like the reference does. */
+int lbm_sp_load_state(lbm_handle *h, const double *f, const double *u, const double *rho);
+/* nsteps x (collision + comput_macro_variables), the loop body of SP:662-684
+ * (collision SP:1406-2013, comput_macro_variables SP:2702-2746), fused into one
+ * pull-stream + collide kernel launch per step */
+int lbm_sp_step(lbm_handle *h, int32_t nsteps);
+/* same, and report the device time of the nsteps launches (HIP events on the handle stream) */
+int lbm_sp_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms);
+/* post-streaming f, u and rho in the reference's layout -- the content of Backup (SP:4507-4562)
+ * without the padding record.  Any pointer may be NULL. */
+int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho);
+/* FP64 device reductions for Error (SP:3625-3704) and Comput_Perm (SP:4565-4712).
+ * box = {xn, xp, yn, yp, zn, zp} inclusive voxel bounds (par_per_* / per_*) or NULL. */
+int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out);
+/* number of time steps taken since create/init/load */
+int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps);
+
+/* ------------------------------------------------------------------ common */
+const char *lbm_last_error(const lbm_handle *h);   /* NULL handle: last create() failure */
+void lbm_destroy(lbm_handle *h);
+/* library version string and the device architecture the kernels were built for */
+const char *lbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

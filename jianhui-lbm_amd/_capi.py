@@ -1,0 +1,83 @@
+"""ctypes binding of liblbm_mi355x.so (the C ABI in include/lbm_mi355x.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "liblbm_mi355x.so")
+HEADER = os.path.join(REPO_DIR, "include", "lbm_mi355x.h")
+
+LBM_OK = 0
+LBM_FP64, LBM_FP32, LBM_FP16 = 0, 1, 2
+PRECISIONS = {"fp64": LBM_FP64, "fp32": LBM_FP32, "fp16": LBM_FP16}
+
+
+class LBMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lbm error {code}: {msg}")
+        self.code = code
+
+
+class SpParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nu", ctypes.c_double), ("g", ctypes.c_double * 3), ("rho0", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("fp16_scale", ctypes.c_double), ("device", ctypes.c_int32)]
+
+
+class SpDiag(ctypes.Structure):
+    _fields_ = [("Q", ctypes.c_double * 3), ("err", ctypes.c_double), ("umax", ctypes.c_double),
+                ("sum_speed", ctypes.c_double), ("nan_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
+
+
+_lib = None
+
+
+def build(jobs: int = 8) -> str:
+    """Compile the library (hipcc, gfx950) in place."""
+    subprocess.run(["make", "-s", "-C", PKG_DIR, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+def declared_symbols():
+    """Function names declared in include/lbm_mi355x.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(lbm_[a-z0-9_]+)\s*\(", text, re.M)))
+
+
+def lib():
+    """Load the built library; fails loudly if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with jianhui-lbm_amd/Makefile "
+                               "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.lbm_version.restype = ctypes.c_char_p
+        L.lbm_last_error.restype = ctypes.c_char_p
+        L.lbm_last_error.argtypes = [vp]
+        L.lbm_destroy.argtypes = [vp]
+        L.lbm_destroy.restype = None
+        L.lbm_sp_create.argtypes = [ctypes.POINTER(SpParams), ip, ctypes.POINTER(vp)]
+        L.lbm_sp_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_sp_init.argtypes = [vp, ctypes.c_double, dp]
+        L.lbm_sp_load_state.argtypes = [vp, dp, dp, dp]
+        L.lbm_sp_step.argtypes = [vp, ctypes.c_int32]
+        L.lbm_sp_step_timed.argtypes = [vp, ctypes.c_int32, dp]
+        L.lbm_sp_get_state.argtypes = [vp, dp, dp, dp]
+        L.lbm_sp_diagnostics.argtypes = [vp, ip, ctypes.POINTER(SpDiag)]
+        L.lbm_sp_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        _lib = L
+    return _lib
+
+
+def check(rc: int, handle=None):
+    if rc != LBM_OK:
+        msg = lib().lbm_last_error(handle)
+        raise LBMError(rc, msg.decode() if msg else "")

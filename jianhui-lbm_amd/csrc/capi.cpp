@@ -1,0 +1,104 @@
+// capi.cpp -- the extern "C" entry points of liblbm_mi355x.so (declared in include/lbm_mi355x.h).
+#include <new>
+#include <string>
+
+#include "handle.h"
+
+namespace {
+thread_local std::string g_create_error;
+
+lbm::SpSolver *as_sp(lbm_handle *h) { return (h && h->kind == lbm_handle::SP) ? static_cast<lbm::SpSolver *>(h) : nullptr; }
+const lbm::SpSolver *as_sp(const lbm_handle *h)
+{
+    return (h && h->kind == lbm_handle::SP) ? static_cast<const lbm::SpSolver *>(h) : nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+const char *lbm_version(void) { return "lbm-mi355x 0.1 (gfx950, sparse D3Q19 MRT)"; }
+
+const char *lbm_last_error(const lbm_handle *h)
+{
+    if (!h) return g_create_error.c_str();
+    return h->last_error.c_str();
+}
+
+void lbm_destroy(lbm_handle *h) { delete h; }
+
+int lbm_sp_create(const lbm_sp_params *p, const int32_t *solid_ijk, lbm_handle **out)
+{
+    if (!p || !solid_ijk || !out) {
+        g_create_error = "lbm_sp_create: null argument";
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    auto *s = new (std::nothrow) lbm::SpSolver();
+    if (!s) {
+        g_create_error = "lbm_sp_create: out of host memory";
+        return LBM_ERR_NOMEM;
+    }
+    int rc = s->create(*p, solid_ijk);
+    if (rc != LBM_OK) {
+        g_create_error = s->last_error;
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_sp_count(const lbm_handle *h, int64_t *count)
+{
+    auto *s = as_sp(h);
+    if (!s || !count) return LBM_ERR_ARG;
+    *count = s->count();
+    return LBM_OK;
+}
+
+int lbm_sp_init(lbm_handle *h, double rho, const double u0[3])
+{
+    auto *s = as_sp(h);
+    return s ? s->init(rho, u0) : LBM_ERR_ARG;
+}
+
+int lbm_sp_load_state(lbm_handle *h, const double *f, const double *u, const double *rho)
+{
+    auto *s = as_sp(h);
+    return s ? s->load_state(f, u, rho) : LBM_ERR_ARG;
+}
+
+int lbm_sp_step(lbm_handle *h, int32_t nsteps)
+{
+    auto *s = as_sp(h);
+    return s ? s->step(nsteps, nullptr) : LBM_ERR_ARG;
+}
+
+int lbm_sp_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms)
+{
+    auto *s = as_sp(h);
+    if (!s || !elapsed_ms) return LBM_ERR_ARG;
+    return s->step(nsteps, elapsed_ms);
+}
+
+int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho)
+{
+    auto *s = as_sp(h);
+    return s ? s->get_state(f, u, rho) : LBM_ERR_ARG;
+}
+
+int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out)
+{
+    auto *s = as_sp(h);
+    return s ? s->diagnostics(box, out) : LBM_ERR_ARG;
+}
+
+int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps)
+{
+    auto *s = as_sp(h);
+    if (!s || !steps) return LBM_ERR_ARG;
+    *steps = s->steps_done();
+    return LBM_OK;
+}
+
+}  // extern "C"

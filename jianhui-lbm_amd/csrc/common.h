@@ -1,0 +1,86 @@
+// common.h -- error handling, launch geometry and deterministic FP64 reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "lbm_mi355x.h"
+
+namespace lbm {
+
+constexpr int BLOCK = 256;          // 4 wavefronts of 64
+constexpr int WAVE = 64;
+constexpr int NUM_XCD = 8;          // MI355X: 8 accelerator complex dies, one L2 each
+
+// Map the hardware workgroup id to a logical tile so that each XCD sweeps one contiguous
+// slab of nodes.  Workgroups are dispatched round-robin over the XCDs (id % 8), so without
+// this remap every XCD touches every region of the node list and neighbouring tiles, which
+// share cache lines through the pull gathers, land in different L2s.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t grid)
+{
+    const int64_t x = b % NUM_XCD, slot = b / NUM_XCD;
+    const int64_t per = grid / NUM_XCD, rem = grid % NUM_XCD;
+    return x * per + (x < rem ? x : rem) + slot;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block-level reduction of K FP64 values per thread into out[0..K-1] (one writer thread per
+// value).  IS_MAX[k] selects max instead of sum.  The tree is fixed, so the result does not
+// depend on scheduling.
+template <int K>
+__device__ __forceinline__ void block_reduce_store(double (&v)[K], const bool (&is_max)[K], double *out)
+{
+    __shared__ double red[BLOCK / WAVE][K];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double r = is_max[k] ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
+        double r = red[0][k];
+        for (int w = 1; w < BLOCK / WAVE; ++w) r = is_max[k] ? (red[w][k] > r ? red[w][k] : r) : r + red[w][k];
+        out[k] = r;
+    }
+}
+
+struct Status {
+    int code = LBM_OK;
+    std::string msg;
+};
+
+#define LBM_HIP_TRY(expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            char buf_[512];                                                                     \
+            std::snprintf(buf_, sizeof(buf_), "%s failed: %s (%s:%d)", #expr,                  \
+                          hipGetErrorString(e_), __FILE__, __LINE__);                           \
+            return fail(LBM_ERR_HIP, buf_);                                                     \
+        }                                                                                       \
+    } while (0)
+
+}  // namespace lbm

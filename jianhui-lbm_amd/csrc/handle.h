@@ -1,0 +1,88 @@
+// handle.h -- the opaque lbm_handle behind the C ABI, and the single-phase solver object.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lbm_mi355x.h"
+
+struct lbm_handle {
+    enum Kind { SP = 1, CG = 2, SERVER = 3 };
+    explicit lbm_handle(Kind k) : kind(k) {}
+    virtual ~lbm_handle() = default;
+    Kind kind;
+    std::string last_error;
+    int fail(int code, const std::string &msg)
+    {
+        last_error = msg;
+        return code;
+    }
+};
+
+namespace lbm {
+
+// Sparse fluid-node structure on the host: the reference's numbering (SP:1172-1193) and, per
+// node and moving direction q = 1..18, the upstream node x - e_q in pull form (BB_FLAG |
+// self when x - e_q is solid).  Stored direction-major with row stride P.
+struct SparseGeometry {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t N = 0;   // fluid nodes
+    int64_t P = 0;   // padded row stride (multiple of the block size)
+    std::vector<int32_t> supinv;   // node -> i*ny*nz + j*nz + k   (size N)
+    std::vector<uint32_t> nbr;     // 18 x P upstream indices
+};
+
+// Number fluid voxels and build the pull neighbour table.  Returns LBM_OK or LBM_ERR_ARG.
+int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid_ijk, SparseGeometry &g,
+                          std::string &err);
+
+// Round up to a multiple of the block size so every direction row starts 256-byte aligned.
+inline int64_t padded_stride(int64_t n) { return ((n + 255) / 256) * 256; }
+
+enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
+
+class SpSolver : public lbm_handle {
+public:
+    SpSolver() : lbm_handle(SP) {}
+    ~SpSolver() override;
+
+    int create(const lbm_sp_params &p, const int32_t *solid);
+    int init(double rho, const double u0[3]);
+    int load_state(const double *f, const double *u, const double *rho);
+    int step(int nsteps, double *elapsed_ms);
+    int get_state(double *f, double *u, double *rho);
+    int diagnostics(const int32_t *box, lbm_sp_diag *out);
+
+    int64_t count() const { return geo_.N; }
+    int64_t steps_done() const { return steps_; }
+
+private:
+    int launch_step(bool store_u);
+    int release();
+
+    lbm_sp_params prm_{};
+    SparseGeometry geo_;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double sv_ = 1.0, sq_ = 1.0;
+    float scale_ = 1.0f, inv_scale_ = 1.0f;
+    size_t st_bytes_ = 8, ct_bytes_ = 8;
+    void *pops_[2] = {nullptr, nullptr};
+    int cur_ = 0;
+    uint32_t *d_nbr_ = nullptr;
+    int32_t *d_supinv_ = nullptr;
+    void *d_u_ = nullptr;     // 3 x P compute-type velocity of the previous step
+    void *d_rho_ = nullptr;   // P compute-type density (loaded states only)
+    double *d_partial_ = nullptr;
+    double *d_result_ = nullptr;
+    double *h_result_ = nullptr;   // pinned
+    int64_t grid_ = 0;
+    StateMode mode_ = STATE_EMPTY;
+    double rho_i_ = 1.0, u_i_[3] = {0, 0, 0};
+    int64_t steps_ = 0;
+};
+
+}  // namespace lbm

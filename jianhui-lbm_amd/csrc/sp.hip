@@ -1,0 +1,567 @@
+// sp.hip -- single-phase sparse D3Q19 MRT: fused pull-stream + collide, diagnostics, state I/O.
+//
+// One launch per time step does what the reference's collision() + comput_macro_variables()
+// pair does per step (SP:1406-2013, SP:2702-2746): every thread owns one fluid node, pulls its
+// 19 post-streaming populations from the upstream nodes (halfway bounce-back folded into the
+// neighbour index), takes the macroscopic moments, collides in moment space and writes the 19
+// post-collision populations, coalesced, to the other buffer.  The stored state is therefore
+// post-collision; the reference's post-streaming state is the pull of it (get_state).
+//
+// Data layout in HBM (structure of arrays, row stride P = N rounded up to 256):
+//   pops[2]  19 x P  storage type (double / float / half), deviations f - w*rho0
+//   nbr      18 x P  uint32 upstream node of direction q = 1..18, bit 31 = bounce back
+//   u_aux     3 x P  compute type, u of the previous step (for the Error diagnostic)
+//   rho_aux   1 x P  compute type, density of a loaded checkpoint (first step only)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "collide.h"
+#include "common.h"
+#include "handle.h"
+#include "storage.h"
+
+namespace lbm {
+
+enum KMode { K_INIT = 0, K_DIRECT = 1, K_PULL = 2 };
+
+template <typename ST>
+struct SpArgs {
+    using C = typename Store<ST>::C;
+    const ST *__restrict__ src;
+    ST *__restrict__ dst;
+    const uint32_t *__restrict__ nbr;
+    C *__restrict__ u_aux;
+    const C *__restrict__ rho_aux;
+    int64_t N, P;
+    C gx, gy, gz, rho0, sv, sq;
+    C rho_i, ux_i, uy_i, uz_i;
+    float scale, inv_scale;
+};
+
+// Gather the 19 pre-collision deviations of node n.
+template <typename ST, int MODE>
+__device__ __forceinline__ void gather(const SpArgs<ST> &a, int64_t n, typename Store<ST>::C (&h)[Q])
+{
+    h[0] = Store<ST>::load(a.src[n], a.inv_scale);
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        if (MODE == K_DIRECT) {
+            h[q] = Store<ST>::load(a.src[(int64_t)q * a.P + n], a.inv_scale);
+        } else {
+            const uint32_t v = a.nbr[(int64_t)(q - 1) * a.P + n];
+            const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+            h[q] = Store<ST>::load(a.src[row * a.P + (int64_t)(v & ~BB_FLAG)], a.inv_scale);
+        }
+    }
+}
+
+template <typename ST, int MODE>
+__device__ __forceinline__ Macro<typename Store<ST>::C> node_macro(const SpArgs<ST> &a, int64_t n,
+                                                                   const typename Store<ST>::C (&m)[Q])
+{
+    using C = typename Store<ST>::C;
+    Macro<C> mc;
+    if (MODE == K_INIT) {
+        mc.rho = a.rho_i;
+        mc.drho = a.rho_i - a.rho0;
+        mc.ux = a.ux_i;
+        mc.uy = a.uy_i;
+        mc.uz = a.uz_i;
+    } else if (MODE == K_DIRECT) {   // a loaded checkpoint: rho and u as given (Backup_init)
+        mc.rho = a.rho_aux[n];
+        mc.drho = mc.rho - a.rho0;
+        mc.ux = a.u_aux[n];
+        mc.uy = a.u_aux[a.P + n];
+        mc.uz = a.u_aux[2 * a.P + n];
+    } else {
+        mc = macro_from_moments(m, a.rho0, a.gx, a.gy, a.gz);
+    }
+    return mc;
+}
+
+template <typename ST, int MODE, bool STORE_U>
+__global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    C h[Q], m[Q];
+    Macro<C> mc;
+    if (MODE == K_INIT) {
+        mc = node_macro<ST, MODE>(a, n, m);
+        sp_eq_moments(mc, m);
+    } else {
+        gather<ST, MODE>(a, n, h);
+        moments_fwd(h, m);
+        mc = node_macro<ST, MODE>(a, n, m);
+    }
+    if (STORE_U && MODE != K_DIRECT) {
+        a.u_aux[n] = mc.ux;
+        a.u_aux[a.P + n] = mc.uy;
+        a.u_aux[2 * a.P + n] = mc.uz;
+    }
+    sp_relax(m, mc, a.gx, a.gy, a.gz, a.sv, a.sq, h);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.dst[(int64_t)q * a.P + n] = Store<ST>::put(h[q], a.scale);
+}
+
+// ---------------------------------------------------------------- diagnostics
+constexpr int NDIAG = 8;   // Q0 Q1 Q2 err_num err_den speed_sum umax2 nan  (+ box count below)
+
+struct DiagBox {
+    int32_t on, xn, xp, yn, yp, zn, zp;
+    int32_t ny, nz;
+};
+
+template <typename ST, int MODE>
+__global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int32_t *__restrict__ supinv,
+                                                        DiagBox box, double *__restrict__ partial)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double v[NDIAG + 1];
+#pragma unroll
+    for (int k = 0; k < NDIAG + 1; ++k) v[k] = 0.0;
+    if (n < a.N) {
+        C h[Q], m[Q];
+        Macro<C> mc;
+        if (MODE == K_INIT) {
+            mc = node_macro<ST, MODE>(a, n, m);
+        } else {
+            gather<ST, MODE>(a, n, h);
+            moments_fwd(h, m);
+            mc = node_macro<ST, MODE>(a, n, m);
+        }
+        const double ux = (double)mc.ux, uy = (double)mc.uy, uz = (double)mc.uz;
+        bool inside = true;
+        if (box.on) {
+            const int32_t s = supinv[n];
+            const int32_t nyz = box.ny * box.nz;
+            const int32_t i = s / nyz, j = (s % nyz) / box.nz, k = s % box.nz;
+            inside = i >= box.xn && i <= box.xp && j >= box.yn && j <= box.yp && k >= box.zn && k <= box.zp;
+        }
+        if (inside) {
+            v[0] = ux;
+            v[1] = uy;
+            v[2] = uz;
+            v[8] = 1.0;
+        }
+        if (n < a.N - 1) {   // the reference's Error loop stops before the last node (SP:3651)
+            const double dx = ux - (double)a.u_aux[n], dy = uy - (double)a.u_aux[a.P + n],
+                         dz = uz - (double)a.u_aux[2 * a.P + n];
+            const double s2 = ux * ux + uy * uy + uz * uz;
+            v[3] = dx * dx + dy * dy + dz * dz;
+            v[4] = s2;
+            v[5] = sqrt(s2);
+            v[6] = s2;
+        }
+        v[7] = (isfinite(ux) && isfinite(uy) && isfinite(uz)) ? 0.0 : 1.0;
+    }
+    const bool is_max[NDIAG + 1] = {false, false, false, false, false, false, true, false, false};
+    block_reduce_store<NDIAG + 1>(v, is_max, partial + (int64_t)blockIdx.x * (NDIAG + 1));
+}
+
+// Second pass over the per-block partials: fixed assignment and fixed tree, so the sums are
+// reproducible run to run.
+template <int K>
+__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double *__restrict__ partial, int64_t nblk,
+                                                                uint32_t max_mask, double *__restrict__ out)
+{
+    double v[K];
+    bool is_max[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        is_max[k] = (max_mask >> k) & 1u;
+        v[k] = 0.0;
+    }
+    for (int64_t b = threadIdx.x; b < nblk; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double x = partial[b * K + k];
+            v[k] = is_max[k] ? (x > v[k] ? x : v[k]) : v[k] + x;
+        }
+    }
+    block_reduce_store<K>(v, is_max, out);
+}
+
+// ---------------------------------------------------------------- state I/O
+template <typename ST, int MODE>
+__global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, double *__restrict__ f_out,
+                                                             double *__restrict__ u_out, double *__restrict__ rho_out)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    C h[Q], m[Q];
+    Macro<C> mc;
+    if (MODE == K_INIT) {
+        mc = node_macro<ST, MODE>(a, n, m);
+        // f = feq(rho_i, u_i) written the way the reference's feq evaluates it (SP:1386-1396)
+        const double ux = (double)mc.ux, uy = (double)mc.uy, uz = (double)mc.uz;
+        const double uv = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double eu = EX[q] * ux + EY[q] * uy + EZ[q] * uz;
+            if (f_out) f_out[n * Q + q] = weight(q) * (double)mc.rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
+        }
+    } else {
+        gather<ST, MODE>(a, n, h);
+        moments_fwd(h, m);
+        mc = node_macro<ST, MODE>(a, n, m);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (f_out) f_out[n * Q + q] = (double)h[q] + weight(q) * (double)a.rho0;
+    }
+    if (u_out) {
+        u_out[n * 3 + 0] = (double)mc.ux;
+        u_out[n * 3 + 1] = (double)mc.uy;
+        u_out[n * 3 + 2] = (double)mc.uz;
+    }
+    if (rho_out) rho_out[n] = (double)mc.rho;
+}
+
+template <typename ST>
+__global__ void __launch_bounds__(BLOCK) sp_load_state_kernel(SpArgs<ST> a, const double *__restrict__ f,
+                                                              const double *__restrict__ u, const double *__restrict__ rho,
+                                                              typename Store<ST>::C *__restrict__ rho_aux)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((typename Store<ST>::C)(f[n * Q + q] - weight(q) * (double)a.rho0),
+                                                    a.scale);
+    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (typename Store<ST>::C)u[n * 3 + c];
+    rho_aux[n] = (typename Store<ST>::C)rho[n];
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+template <typename ST>
+SpArgs<ST> make_args(const void *src, void *dst, const uint32_t *nbr, void *u_aux, const void *rho_aux, int64_t N,
+                     int64_t P, const lbm_sp_params &p, double sv, double sq, double rho_i, const double *u_i,
+                     float scale, float inv_scale)
+{
+    using C = typename Store<ST>::C;
+    SpArgs<ST> a;
+    a.src = (const ST *)src;
+    a.dst = (ST *)dst;
+    a.nbr = nbr;
+    a.u_aux = (C *)u_aux;
+    a.rho_aux = (const C *)rho_aux;
+    a.N = N;
+    a.P = P;
+    a.gx = (C)p.g[0];
+    a.gy = (C)p.g[1];
+    a.gz = (C)p.g[2];
+    a.rho0 = (C)p.rho0;
+    a.sv = (C)sv;
+    a.sq = (C)sq;
+    a.rho_i = (C)rho_i;
+    a.ux_i = (C)u_i[0];
+    a.uy_i = (C)u_i[1];
+    a.uz_i = (C)u_i[2];
+    a.scale = scale;
+    a.inv_scale = inv_scale;
+    return a;
+}
+
+template <typename ST>
+hipError_t launch_step_t(const SpArgs<ST> &a, int mode, bool store_u, int64_t grid, hipStream_t s)
+{
+    dim3 g((unsigned)grid), b(BLOCK);
+    if (mode == K_INIT)
+        store_u ? sp_step_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : sp_step_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
+    else if (mode == K_DIRECT)
+        sp_step_kernel<ST, K_DIRECT, false><<<g, b, 0, s>>>(a);
+    else
+        store_u ? sp_step_kernel<ST, K_PULL, true><<<g, b, 0, s>>>(a) : sp_step_kernel<ST, K_PULL, false><<<g, b, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+template <typename ST>
+hipError_t launch_diag_t(const SpArgs<ST> &a, int mode, const int32_t *supinv, DiagBox box, double *partial,
+                         int64_t grid, hipStream_t s)
+{
+    dim3 g((unsigned)grid), b(BLOCK);
+    if (mode == K_INIT)
+        sp_diag_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, supinv, box, partial);
+    else if (mode == K_DIRECT)
+        sp_diag_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, supinv, box, partial);
+    else
+        sp_diag_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, supinv, box, partial);
+    return hipGetLastError();
+}
+
+template <typename ST>
+hipError_t launch_get_t(const SpArgs<ST> &a, int mode, double *f, double *u, double *rho, int64_t grid, hipStream_t s)
+{
+    dim3 g((unsigned)grid), b(BLOCK);
+    if (mode == K_INIT)
+        sp_get_state_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, f, u, rho);
+    else if (mode == K_DIRECT)
+        sp_get_state_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, f, u, rho);
+    else
+        sp_get_state_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, f, u, rho);
+    return hipGetLastError();
+}
+
+int kmode_of(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT ? K_DIRECT : K_PULL); }
+
+}  // namespace
+
+SpSolver::~SpSolver() { release(); }
+
+int SpSolver::release()
+{
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    for (void *&p : pops_)
+        if (p) (void)hipFree(p), p = nullptr;
+    if (d_nbr_) (void)hipFree(d_nbr_), d_nbr_ = nullptr;
+    if (d_supinv_) (void)hipFree(d_supinv_), d_supinv_ = nullptr;
+    if (d_u_) (void)hipFree(d_u_), d_u_ = nullptr;
+    if (d_rho_) (void)hipFree(d_rho_), d_rho_ = nullptr;
+    if (d_partial_) (void)hipFree(d_partial_), d_partial_ = nullptr;
+    if (d_result_) (void)hipFree(d_result_), d_result_ = nullptr;
+    if (h_result_) (void)hipHostFree(h_result_), h_result_ = nullptr;
+    if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
+    if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
+    if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
+    return LBM_OK;
+}
+
+int SpSolver::create(const lbm_sp_params &p, const int32_t *solid)
+{
+    prm_ = p;
+    if (p.precision < LBM_FP64 || p.precision > LBM_FP16) return fail(LBM_ERR_ARG, "unknown precision");
+    if (!(p.nu > 0.0)) return fail(LBM_ERR_ARG, "viscosity must be positive");
+    if (prm_.rho0 == 0.0) prm_.rho0 = 1.0;
+    std::string err;
+    int rc = build_sparse_geometry(p.nx, p.ny, p.nz, solid, geo_, err);
+    if (rc != LBM_OK) return fail(rc, err);
+
+    // relaxation rates as the reference derives them (SP:1291-1307)
+    const double c_s2 = 1.0 / 3.0;
+    const double tau = p.nu / (c_s2 * 1.0) + 0.5;
+    sv_ = 1.0 / tau;
+    sq_ = 8.0 * (2.0 - sv_) / (8.0 - sv_);
+    if (p.precision == LBM_FP16) {
+        const double s = p.fp16_scale > 0 ? p.fp16_scale : 4096.0;
+        scale_ = (float)s;
+        inv_scale_ = (float)(1.0 / s);
+    }
+    st_bytes_ = p.precision == LBM_FP64 ? 8 : (p.precision == LBM_FP32 ? 4 : 2);
+    ct_bytes_ = p.precision == LBM_FP64 ? 8 : 4;
+
+    device_ = p.device;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    LBM_HIP_TRY(hipEventCreate(&ev0_));
+    LBM_HIP_TRY(hipEventCreate(&ev1_));
+    const int64_t P = geo_.P;
+    grid_ = (geo_.N + BLOCK - 1) / BLOCK;
+    for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * P * st_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * P * sizeof(uint32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_supinv_, (size_t)P * sizeof(int32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_u_, (size_t)3 * P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_rho_, (size_t)P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)grid_ * (NDIAG + 1) * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&d_result_, 32 * sizeof(double)));
+    LBM_HIP_TRY(hipHostMalloc((void **)&h_result_, 32 * sizeof(double), hipHostMallocDefault));
+    LBM_HIP_TRY(hipMemcpyAsync(d_nbr_, geo_.nbr.data(), geo_.nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               stream_));
+    LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    // the host copy of the neighbour table is no longer needed
+    std::vector<uint32_t>().swap(geo_.nbr);
+    const double zero[3] = {0, 0, 0};
+    return init(1.0, zero);
+}
+
+int SpSolver::init(double rho, const double u0[3])
+{
+    if (!u0) return fail(LBM_ERR_ARG, "null u0");
+    rho_i_ = rho;
+    for (int c = 0; c < 3; ++c) u_i_[c] = u0[c];
+    mode_ = STATE_INIT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+#define SP_DISPATCH(EXPR_DOUBLE, EXPR_FLOAT, EXPR_HALF)                          \
+    (prm_.precision == LBM_FP64 ? (EXPR_DOUBLE) : prm_.precision == LBM_FP32 ? (EXPR_FLOAT) : (EXPR_HALF))
+
+int SpSolver::load_state(const double *f, const double *u, const double *rho)
+{
+    if (!f || !u || !rho) return fail(LBM_ERR_ARG, "null state array");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    double *df = nullptr, *du = nullptr, *dr = nullptr;
+    LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&du, (size_t)N * 3 * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&dr, (size_t)N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(df, f, (size_t)N * Q * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(du, u, (size_t)N * 3 * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rho, (size_t)N * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) {
+        const int nxt = cur_;   // DIRECT state lives in the current buffer
+        const double zero[3] = {0, 0, 0};
+        dim3 g((unsigned)grid_), b(BLOCK);
+        if (prm_.precision == LBM_FP64) {
+            auto a = make_args<double>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+                                       scale_, inv_scale_);
+            sp_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, (double *)d_rho_);
+        } else if (prm_.precision == LBM_FP32) {
+            auto a = make_args<float>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+                                      scale_, inv_scale_);
+            sp_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
+        } else {
+            auto a = make_args<__half>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+                                       scale_, inv_scale_);
+            sp_load_state_kernel<__half><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(df);
+    (void)hipFree(du);
+    (void)hipFree(dr);
+    LBM_HIP_TRY(e);
+    mode_ = STATE_DIRECT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+int SpSolver::launch_step(bool store_u)
+{
+    const int mode = kmode_of(mode_);
+    const int nxt = cur_ ^ 1;
+    hipError_t e;
+    if (prm_.precision == LBM_FP64)
+        e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
+                          mode, store_u, grid_, stream_);
+    else if (prm_.precision == LBM_FP32)
+        e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+                                           sq_, rho_i_, u_i_, scale_, inv_scale_),
+                          mode, store_u, grid_, stream_);
+    else
+        e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
+                          mode, store_u, grid_, stream_);
+    LBM_HIP_TRY(e);
+    cur_ = nxt;
+    mode_ = STATE_PULL;
+    ++steps_;
+    return LBM_OK;
+}
+
+int SpSolver::step(int nsteps, double *elapsed_ms)
+{
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "negative step count");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    if (elapsed_ms) LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
+    for (int s = 0; s < nsteps; ++s) {
+        // the last launch of a call keeps u of the step before it, which the Error
+        // diagnostic compares against (u0 in SP:3625-3704)
+        int rc = launch_step(s == nsteps - 1);
+        if (rc != LBM_OK) return rc;
+    }
+    if (elapsed_ms) {
+        LBM_HIP_TRY(hipEventRecord(ev1_, stream_));
+        LBM_HIP_TRY(hipEventSynchronize(ev1_));
+        float ms = 0.f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+        *elapsed_ms = ms;
+    }
+    return LBM_OK;
+}
+
+int SpSolver::get_state(double *f, double *u, double *rho)
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    double *df = nullptr, *du = nullptr, *dr = nullptr;
+    if (f) LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
+    if (u) LBM_HIP_TRY(hipMalloc(&du, (size_t)N * 3 * sizeof(double)));
+    if (rho) LBM_HIP_TRY(hipMalloc(&dr, (size_t)N * sizeof(double)));
+    const int mode = kmode_of(mode_);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64)
+        e = launch_get_t(make_args<double>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+                                           u_i_, scale_, inv_scale_),
+                         mode, df, du, dr, grid_, stream_);
+    else if (prm_.precision == LBM_FP32)
+        e = launch_get_t(make_args<float>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+                                          u_i_, scale_, inv_scale_),
+                         mode, df, du, dr, grid_, stream_);
+    else
+        e = launch_get_t(make_args<__half>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+                                           u_i_, scale_, inv_scale_),
+                         mode, df, du, dr, grid_, stream_);
+    if (e == hipSuccess && f) e = hipMemcpyAsync(f, df, (size_t)N * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (df) (void)hipFree(df);
+    if (du) (void)hipFree(du);
+    if (dr) (void)hipFree(dr);
+    LBM_HIP_TRY(e);
+    return LBM_OK;
+}
+
+int SpSolver::diagnostics(const int32_t *box, lbm_sp_diag *out)
+{
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    DiagBox db{};
+    db.ny = geo_.ny;
+    db.nz = geo_.nz;
+    if (box) {
+        db.on = 1;
+        db.xn = box[0];
+        db.xp = box[1];
+        db.yn = box[2];
+        db.yp = box[3];
+        db.zn = box[4];
+        db.zp = box[5];
+    }
+    const int mode = kmode_of(mode_);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64)
+        e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+    else if (prm_.precision == LBM_FP32)
+        e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                           rho_i_, u_i_, scale_, inv_scale_),
+                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+    else
+        e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+    LBM_HIP_TRY(e);
+    reduce_partials_kernel<NDIAG + 1><<<1, BLOCK, 0, stream_>>>(d_partial_, grid_, 1u << 6, d_result_);
+    LBM_HIP_TRY(hipGetLastError());
+    LBM_HIP_TRY(hipMemcpyAsync(h_result_, d_result_, (NDIAG + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    const double *r = h_result_;
+    out->Q[0] = r[0];
+    out->Q[1] = r[1];
+    out->Q[2] = r[2];
+    out->err = std::sqrt(r[3]) / (std::sqrt(r[4]) + 1e-30);
+    out->umax = std::sqrt(r[6]);
+    out->sum_speed = r[5];
+    out->nan_count = r[7];
+    out->count_in_box = r[8];
+    return LBM_OK;
+}
+
+}  // namespace lbm

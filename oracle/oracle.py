@@ -1,0 +1,219 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/liblbm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, by ``__graft_entry__.smoke()`` as the checker and
+by ``bench.py``'s cpu_baseline leg.  The product path (the HIP library) never imports it.
+
+Arrays follow the reference's conventions: per-node arrays have ``count + 1`` rows and row 0
+is padding; populations are ``[count + 1, 19]`` in the reference's own velocity ordering
+(SP ordering for ``SPOracle``, CG ordering for ``CGOracle``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc is enough; no GPU toolchain)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "liblbm_oracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_number_fluid.restype = ctypes.c_int
+        L.orc_sp_error.restype = ctypes.c_double
+        L.orc_cg_wet_count.restype = ctypes.c_double
+        L.orc_invert19.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def lattice(which: int):
+    """(e[19,3], w[19], opp[19]) in the SP (0) or CG (1) ordering."""
+    e = np.zeros((19, 3), np.int32)
+    w = np.zeros(19)
+    opp = np.zeros(19, np.int32)
+    lib().orc_lattice(ctypes.c_int(which), _i(e), _d(w), _i(opp))
+    return e, w, opp
+
+
+def moment_matrix(which: int) -> np.ndarray:
+    M = np.zeros((19, 19))
+    lib().orc_moment_matrix(ctypes.c_int(which), _d(M))
+    return M
+
+
+def invert19(A: np.ndarray) -> np.ndarray:
+    A = np.ascontiguousarray(A, np.float64)
+    out = np.zeros((19, 19))
+    if lib().orc_invert19(_d(A), _d(out)) != 0:
+        raise ValueError("singular matrix")
+    return out
+
+
+def number_fluid(solid: np.ndarray):
+    """(count, idx[nx,ny,nz] 0=solid else 1-based, supinv[count+1]) for an [i][j][k] solid map."""
+    solid = np.ascontiguousarray(solid, np.int32)
+    nx, ny, nz = solid.shape
+    idx = np.zeros(solid.shape, np.int32)
+    count = lib().orc_number_fluid(nx, ny, nz, _i(solid), _i(idx), None)
+    supinv = np.zeros(count + 1, np.int32)
+    lib().orc_number_fluid(nx, ny, nz, _i(solid), _i(idx), _i(supinv))
+    return count, idx, supinv
+
+
+def sp_relaxation(nu: float):
+    S = np.zeros(19)
+    sv = ctypes.c_double()
+    lib().orc_sp_relaxation(ctypes.c_double(nu), _d(S), ctypes.byref(sv))
+    return S, sv.value
+
+
+class SPOracle:
+    """Single-phase state on the CPU, stepped exactly like the reference time loop."""
+
+    def __init__(self, solid, nu, g, u0=(0.0, 0.0, 0.0), rho0=1.0, nthreads=1):
+        self.solid = np.ascontiguousarray(solid, np.int32)
+        self.nx, self.ny, self.nz = self.solid.shape
+        self.count, self.idx, self.supinv = number_fluid(self.solid)
+        self.nu = float(nu)
+        self.g = np.ascontiguousarray(g, np.float64)
+        self.S, self.s_v = sp_relaxation(self.nu)
+        n1 = self.count + 1
+        self.rho = np.zeros(n1)
+        self.u = np.zeros((n1, 3))
+        self.u0 = np.zeros((n1, 3))
+        self.f = np.zeros((n1, 19))
+        self.F = np.zeros((n1, 19))
+        self.nthreads = int(nthreads)
+        u0 = np.ascontiguousarray(u0, np.float64)
+        lib().orc_sp_init(self.count, ctypes.c_double(rho0), _d(u0), _d(self.rho), _d(self.u), _d(self.f))
+        self.steps = 0
+
+    def load(self, f, u, rho):
+        """Replace the state (per-node arrays without the padding row)."""
+        self.f[1:] = f
+        self.u[1:] = u
+        self.rho[1:] = rho
+
+    def run(self, nsteps: int):
+        L = lib()
+        L.orc_sp_run(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv), _d(self.S),
+                     _d(self.g), _d(self.rho), _d(self.u), _d(self.u0), _d(self.f), _d(self.F),
+                     int(nsteps), self.nthreads)
+        self.steps += nsteps
+
+    def error(self):
+        umax = ctypes.c_double()
+        ssum = ctypes.c_double()
+        err = lib().orc_sp_error(self.count, _d(self.u), _d(self.u0), ctypes.byref(umax), ctypes.byref(ssum))
+        return err, umax.value, ssum.value
+
+    def flux(self, box=None):
+        Q = np.zeros(3)
+        b = None if box is None else (ctypes.c_int * 6)(*box)
+        lib().orc_flux(self.ny, self.nz, self.count, _i(self.supinv), _d(self.u), b, _d(Q))
+        return Q
+
+
+class CGOracle:
+    """Colour-gradient state on the CPU, stepped like the reference CG time loop."""
+
+    def __init__(self, solid, psi_field, nu_l, nu_g, g, cap_a, cos_theta, u0=(0.0, 0.0, 0.0),
+                 bodyforce_apply=0, rel_perm_psi=0.99, rel_perm_psi2=0.92):
+        self.solid = np.ascontiguousarray(solid, np.int32)
+        self.nx, self.ny, self.nz = self.solid.shape
+        self.count, self.idx, self.supinv = number_fluid(self.solid)
+        n1 = self.count + 1
+        psi_node = np.zeros(n1)
+        flat = np.clip(np.asarray(psi_field, np.float64).reshape(-1), -1.0, 1.0)
+        psi_node[1:] = flat[self.supinv[1:]]
+        self.visc = np.zeros(8)
+        lib().orc_cg_visc_consts(ctypes.c_double(nu_l), ctypes.c_double(nu_g), _d(self.visc))
+        self.nu_l, self.nu_g = float(nu_l), float(nu_g)
+        self.params = _CGParams()
+        for a in range(3):
+            self.params.g[a] = float(g[a])
+        self.params.cap_a = float(cap_a)
+        self.params.psi_solid = float(cos_theta)
+        for a in range(8):
+            self.params.visc[a] = self.visc[a]
+        self.params.bodyforce_apply = int(bodyforce_apply)
+        self.params.rel_perm_psi = float(rel_perm_psi)
+        self.params.rel_perm_psi2 = float(rel_perm_psi2)
+        self.g = np.ascontiguousarray(g, np.float64)
+        self.rho = np.zeros(n1)
+        self.u = np.zeros((n1, 3))
+        self.u0 = np.zeros((n1, 3))
+        self.f = np.zeros((n1, 19))
+        self.F = np.zeros((n1, 19))
+        self.psi = np.zeros(n1)
+        self.rho_r = np.zeros(n1)
+        self.rho_b = np.zeros(n1)
+        self.rhor = np.zeros(n1)
+        self.rhob = np.zeros(n1)
+        u0 = np.ascontiguousarray(u0, np.float64)
+        lib().orc_cg_init(self.count, _d(u0), _d(psi_node), _d(self.rho), _d(self.u), _d(self.psi),
+                          _d(self.rho_r), _d(self.rho_b), _d(self.rhor), _d(self.rhob), _d(self.f))
+        self.steps = 0
+
+    def run(self, nsteps: int):
+        L = lib()
+        for _ in range(int(nsteps)):
+            L.orc_cg_collide(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv),
+                             ctypes.byref(self.params), _d(self.rho), _d(self.u), _d(self.f), _d(self.F),
+                             _d(self.psi), _d(self.rho_r), _d(self.rho_b), _d(self.rhor), _d(self.rhob))
+            L.orc_cg_macro(self.count, _d(self.g), _d(self.F), _d(self.f), _d(self.rho), _d(self.u),
+                           _d(self.u0), _d(self.rho_r), _d(self.rho_b), _d(self.rhor), _d(self.rhob),
+                           _d(self.psi))
+            self.steps += 1
+
+    def error(self):
+        umax = ctypes.c_double()
+        ssum = ctypes.c_double()
+        err = lib().orc_sp_error(self.count, _d(self.u), _d(self.u0), ctypes.byref(umax), ctypes.byref(ssum))
+        return err, umax.value, ssum.value
+
+    def flux(self, box=None):
+        Ql = np.zeros(3)
+        Qg = np.zeros(3)
+        b = None if box is None else (ctypes.c_int * 6)(*box)
+        lib().orc_cg_flux(self.ny, self.nz, self.count, _i(self.supinv), _d(self.u), _d(self.psi), b,
+                          ctypes.c_double(self.params.rel_perm_psi), ctypes.c_double(self.params.rel_perm_psi2),
+                          _d(Ql), _d(Qg))
+        return Ql, Qg
+
+    def wet_count(self, box=None):
+        b = None if box is None else (ctypes.c_int * 6)(*box)
+        return lib().orc_cg_wet_count(self.ny, self.nz, self.count, _i(self.supinv), _d(self.psi), b)
+
+
+class _CGParams(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_double * 3), ("cap_a", ctypes.c_double), ("psi_solid", ctypes.c_double),
+                ("visc", ctypes.c_double * 8), ("bodyforce_apply", ctypes.c_int),
+                ("rel_perm_psi", ctypes.c_double), ("rel_perm_psi2", ctypes.c_double)]

@@ -1,0 +1,160 @@
+"""Generate the golden fixtures from the reference solvers themselves.
+
+Runs the reference's own single-phase and colour-gradient programs (built from their sources
+by ``make -C oracle ref``) on small seeded porous boxes and stores the full-precision FP64
+state they write through their checkpoint routine (SP:4507-4562, CG:7020-7090), plus the
+text result files, as ``tests/golden/*.npz``.  The fixtures are data only: inputs (geometry,
+phase field, parameters) and the reference's outputs.
+
+The reference needs MPI and cannot run with one rank (SP:1226 sends to rank+1), so it runs
+with ``mpirun -np 2``.  Its per-node FP64 state at two ranks equals the single-rank semantics
+(periodic x) that the oracle and the HIP library implement.  Checkpoint label n holds the
+state after n+1 time steps (the loop runs n = 0..n_max, SP:662), and record 0 of each rank
+file is padding; ranks are concatenated in order to get the global i-major node order.
+
+Usage (in the build container, where /root/reference exists):
+    python tests/golden/make_ref_fixtures.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "jianhui-lbm_amd"))
+import geometry  # noqa: E402
+import inputs  # noqa: E402
+
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+MPI_BIN = "/opt/conda/bin"
+
+
+def _build_ref():
+    subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+
+
+def _run(binary, workdir, input_name="INPUT.dat"):
+    env = dict(os.environ)
+    env["PATH"] = MPI_BIN + ":" + env.get("PATH", "")
+    proc = subprocess.run(["mpirun", "-np", "2", binary, input_name], cwd=workdir, env=env,
+                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
+    if proc.returncode != 0:
+        raise RuntimeError(f"reference run failed in {workdir}:\n{proc.stdout[-3000:]}")
+    return proc.stdout
+
+
+def _read_ckpt(workdir, name, label, width):
+    parts = []
+    for rank in range(2):
+        path = os.path.join(workdir, f"LBM_checkpoint_{name}_{label}.{rank}.bin_input")
+        a = np.fromfile(path, dtype="<f8")
+        a = a.reshape(-1, width) if width > 1 else a
+        parts.append(a[1:])       # record 0 is padding
+    return np.concatenate(parts, axis=0)
+
+
+def _text(workdir, fname):
+    p = os.path.join(workdir, fname)
+    return open(p).read() if os.path.exists(p) else ""
+
+
+def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None):
+    solid = geometry.sphere_pack(shape, radius, por, seed)
+    out = {"solid": solid.astype(np.int8)}
+    params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
+                  steps=list(steps_list), radius=radius, porosity_target=por, seed=seed)
+    for steps in steps_list:
+        wd = tempfile.mkdtemp(prefix="sp_fix_")
+        try:
+            geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+            inp = inputs.SPInput(filename="geometry.bin", n_xyz=tuple(shape), n_max=steps - 1, reso=1.0,
+                                 g=tuple(g), in_vis=nu, ini_v=tuple(u0), wr_per=1, PerDir=1, freRe=freRe,
+                                 Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
+            if box is not None:
+                inp.par_per = (1, 1, 1)
+                inp.per_x = (box[1], box[0])
+                inp.per_y = (box[3], box[2])
+                inp.per_z = (box[5], box[4])
+            inputs.write_sp_input(inp, os.path.join(wd, "INPUT.dat"))
+            _run(os.path.join(REF_DIR, "sp_ref"), wd)
+            label = steps - 1
+            out[f"f_{steps}"] = _read_ckpt(wd, "f", label, 19)
+            out[f"u_{steps}"] = _read_ckpt(wd, "velocity", label, 3)
+            out[f"rho_{steps}"] = _read_ckpt(wd, "rho", label, 1)
+            out[f"perm_txt_{steps}"] = np.array(_text(wd, "Permeability_error_local_Perm.txt"))
+            out[f"vel_txt_{steps}"] = np.array(_text(wd, "Velocity_ave_max.txt"))
+            out[f"general_txt_{steps}"] = np.array(_text(wd, "General_Results.txt"))
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+    out["params"] = np.array(json.dumps(params))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "count =", out[f"rho_{steps_list[0]}"].shape[0])
+
+
+def cg_case(name, shape, radius, por, seed, psi_mode, psi_seed, nu_l, nu_g, cos_theta, cap_a, g,
+            steps_list, freRe=10, bodyforce_apply=0, rel_perm=(0.99, 0.92)):
+    solid = geometry.sphere_pack(shape, radius, por, seed)
+    rng = np.random.default_rng(psi_seed)
+    if psi_mode == "binary":
+        psi = np.where(rng.uniform(size=shape) < 0.5, 1.0, -1.0)
+    else:   # values on a 1/64 grid so that the reference's float read is exact
+        psi = np.round(rng.uniform(-1.0, 1.0, size=shape) * 64.0) / 64.0
+    out = {"solid": solid.astype(np.int8), "psi_field": psi}
+    params = dict(shape=list(shape), nu_l=nu_l, nu_g=nu_g, cos_theta=cos_theta, cap_a=cap_a, g=list(g),
+                  freRe=freRe, steps=list(steps_list), bodyforce_apply=bodyforce_apply,
+                  rel_perm_psi=list(rel_perm), radius=radius, porosity_target=por, seed=seed,
+                  psi_mode=psi_mode, psi_seed=psi_seed)
+    for steps in steps_list:
+        wd = tempfile.mkdtemp(prefix="cg_fix_")
+        try:
+            geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+            geometry.write_phase_ascii(os.path.join(wd, "phase.dat"), psi)
+            inp = inputs.CGInput(filename="geometry.bin", filenamepsi="phase.dat", n_xyz=tuple(shape),
+                                 n_max=steps - 1, reso=1.0, g=tuple(g), niu_l=nu_l, niu_g=nu_g,
+                                 cos_theta=cos_theta, CapA=cap_a, Permeability=1.0, wr_per=1, PerDir=1,
+                                 freRe=freRe, outputfile="./", stab=(0, 0), ini_Sat=-1.0,
+                                 rel_perm_psi=tuple(rel_perm), fre_backup=0, decbin=1,
+                                 bodyforce_apply=bodyforce_apply)
+            inputs.write_cg_input(inp, os.path.join(wd, "INPUT.dat"))
+            _run(os.path.join(REF_DIR, "cg_ref"), wd)
+            label = steps - 1
+            out[f"f_{steps}"] = _read_ckpt(wd, "f", label, 19)
+            out[f"u_{steps}"] = _read_ckpt(wd, "velocity", label, 3)
+            out[f"rho_{steps}"] = _read_ckpt(wd, "rho", label, 1)
+            out[f"psi_{steps}"] = _read_ckpt(wd, "psi", label, 1)
+            out[f"rhor_{steps}"] = _read_ckpt(wd, "rhor", label, 1)
+            out[f"rhob_{steps}"] = _read_ckpt(wd, "rhob", label, 1)
+            out[f"relperm1_txt_{steps}"] = np.array(_text(wd, "Relative_Permeability_Component1.txt"))
+            out[f"relperm2_txt_{steps}"] = np.array(_text(wd, "Relative_Permeability_Component2.txt"))
+            out[f"bodyforce_txt_{steps}"] = np.array(_text(wd, "bodyforce.txt"))
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+    out["params"] = np.array(json.dumps(params))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "count =", out[f"rho_{steps_list[0]}"].shape[0])
+
+
+def main():
+    _build_ref()
+    sp_case("sp_mrt_16x12x8", (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
+            u0=(1e-3, -2e-3, 5e-4), steps_list=[1, 101])
+    sp_case("sp_tau1_box_20x10x12", (20, 10, 12), 2.5, 0.5, 11, nu=1.0 / 6.0, g=(1e-6, 0.0, 0.0),
+            u0=(0.0, 0.0, 0.0), steps_list=[1, 51], box=[4, 15, 1, 8, 2, 10])
+    cg_case("cg_binary_16x12x8", (16, 12, 8), 3.0, 0.6, 5, "binary", 6, nu_l=0.1, nu_g=0.05,
+            cos_theta=0.3, cap_a=0.01, g=(1e-5, 0.0, 0.0), steps_list=[1, 51])
+    cg_case("cg_mixed_gated_14x10x8", (14, 10, 8), 2.5, 0.55, 9, "uniform", 10, nu_l=1.0 / 6.0,
+            nu_g=1.0 / 6.0, cos_theta=0.0, cap_a=0.005, g=(2e-5, 1e-5, 0.0), steps_list=[1, 21],
+            bodyforce_apply=1)
+
+
+if __name__ == "__main__":
+    main()

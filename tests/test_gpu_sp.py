@@ -1,0 +1,188 @@
+"""Single-phase HIP kernels against the reference's checkpoints and the FP64 CPU oracle.
+
+Every test here runs the HIP library on the GPU through the C ABI (ctypes) and checks it with
+the oracle or the reference's own fixture states, at sizes the oracle finishes in seconds.
+Tolerances: FP64 within 1e-12 (norm-wise relative) of the oracle and the reference; FP32
+permeability within 1e-4 relative of FP64 at steady state.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_fixture, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SP_FIXTURES = ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12"]
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _fixture_solver(pkg, name, precision="fp64"):
+    d, p = load_fixture(name)
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision=precision)
+    s.init(1.0, p["u0"])
+    return s, d, p
+
+
+@pytest.mark.parametrize("name", SP_FIXTURES)
+def test_fp64_matches_reference_checkpoints(pkg, name):
+    _log(f"[gpu] fp64 vs reference checkpoints {name}")
+    s, d, p = _fixture_solver(pkg, name)
+    assert s.count == d[f"rho_{p['steps'][0]}"].shape[0]
+    done = 0
+    for steps in p["steps"]:
+        s.step(steps - done)
+        done = steps
+        f, u, rho = s.state()
+        assert rel_err(f, d[f"f_{steps}"]) < 1e-13
+        assert rel_err(rho, d[f"rho_{steps}"]) < 1e-13
+        assert np.abs(u - d[f"u_{steps}"]).max() < 1e-14
+        assert rel_err(u, d[f"u_{steps}"]) < 5e-10
+
+
+@pytest.mark.parametrize("name", SP_FIXTURES)
+def test_fp64_matches_oracle_step_by_step(pkg, oracle_mod, name):
+    _log(f"[gpu] fp64 vs oracle {name}")
+    s, d, p = _fixture_solver(pkg, name)
+    o = oracle_mod.SPOracle(d["solid"].astype(np.int32), p["nu"], p["g"], p["u0"])
+    for steps in (1, 2, 5, 30):
+        s.step(steps - s.steps_done)
+        o.run(steps - o.steps)
+        f, u, rho = s.state()
+        assert rel_err(f, o.f[1:]) < 1e-13
+        assert np.abs(u - o.u[1:]).max() < 1e-14
+        # diagnostics: Error() and the permeability flux, FP64 reductions on the device
+        box = p["box"]
+        diag = s.diagnostics(box)
+        err, umax, ssum = o.error()
+        Q = o.flux(box)
+        assert abs(diag["err"] - err) <= 1e-9 * err
+        assert abs(diag["umax"] - umax) <= 1e-12 * umax
+        assert abs(diag["sum_speed"] - ssum) <= 1e-12 * ssum
+        np.testing.assert_allclose(diag["Q"], Q, rtol=1e-11, atol=1e-18)
+        assert diag["nan_count"] == 0
+
+
+def test_restart_from_reference_checkpoint(pkg):
+    """Backup_init path: load the reference's 1-step state, continue, compare at 101 steps."""
+    _log("[gpu] restart from reference checkpoint")
+    d, p = load_fixture("sp_mrt_16x12x8")
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    s.load_state(d["f_1"], d["u_1"], d["rho_1"])
+    f, u, rho = s.state()
+    np.testing.assert_array_equal(f, d["f_1"])
+    s.step(100)
+    f, u, rho = s.state()
+    assert rel_err(f, d["f_101"]) < 1e-13
+    assert np.abs(u - d["u_101"]).max() < 1e-14
+
+
+def test_init_state_is_equilibrium(pkg, oracle_mod):
+    _log("[gpu] init state")
+    d, p = load_fixture("sp_mrt_16x12x8")
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    s.init(1.02, p["u0"])
+    f, u, rho = s.state()
+    o = oracle_mod.SPOracle(d["solid"].astype(np.int32), p["nu"], p["g"], p["u0"], rho0=1.02)
+    np.testing.assert_array_equal(f, o.f[1:])
+    np.testing.assert_array_equal(u, o.u[1:])
+    np.testing.assert_array_equal(rho, o.rho[1:])
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+def test_uniform_acceleration(pkg, precision, tol):
+    _log(f"[gpu] uniform acceleration {precision}")
+    solid = np.zeros((8, 6, 10), np.int32)
+    g = np.array([1e-5, -2e-6, 3e-6])
+    s = pkg.SinglePhase(solid, 0.1, g, precision=precision)
+    for steps in (1, 10, 40):
+        s.step(steps - s.steps_done)
+        u = s.velocity()
+        np.testing.assert_allclose(u, np.tile((steps + 0.5) * g, (s.count, 1)), rtol=tol, atol=1e-16)
+
+
+def test_plane_poiseuille(pkg):
+    _log("[gpu] plane Poiseuille")
+    H, nu, g = 10, 0.1, 1e-6
+    solid = np.zeros((4, H + 1, 3), np.int32)
+    solid[:, 0, :] = 1
+    y = np.arange(1, H + 1)
+    exact = g / (2 * nu) * (y - 0.5) * (H + 0.5 - y)
+    for precision, tol in (("fp64", 1e-9), ("fp32", 2e-5)):
+        s = pkg.SinglePhase(solid, nu, (g, 0, 0), precision=precision)
+        s.step(int(50 * H * H / nu))
+        u = s.velocity()[:, 0].reshape(4, H, 3)
+        np.testing.assert_allclose(u, np.broadcast_to(exact[None, :, None], u.shape), rtol=tol)
+
+
+@pytest.mark.parametrize("shape,por", [((1, 1, 1), 1.0), ((5, 7, 9), 1.0), ((7, 9, 11), 0.7), ((16, 16, 17), 0.5)])
+def test_edge_sizes_against_oracle(pkg, oracle_mod, shape, por):
+    """Node counts below, at and across the block size; a single node; an all-fluid box."""
+    _log(f"[gpu] edge size {shape} porosity {por}")
+    if por >= 1.0:
+        solid = np.zeros(shape, np.int32)
+    else:
+        solid = (np.random.default_rng(sum(shape)).uniform(size=shape) > por).astype(np.int32)
+        solid.reshape(-1)[0] = 0
+    g = (2e-5, 1e-5, -1e-5)
+    s = pkg.SinglePhase(solid, 0.12, g, precision="fp64")
+    s.init(1.0, (1e-3, 0, -2e-3))
+    o = oracle_mod.SPOracle(solid, 0.12, g, (1e-3, 0, -2e-3))
+    s.step(13)
+    o.run(13)
+    f, u, rho = s.state()
+    assert f.shape[0] == o.count
+    assert rel_err(f, o.f[1:]) < 1e-13
+    assert np.abs(u - o.u[1:]).max() < 1e-14
+
+
+def test_runs_are_bitwise_reproducible(pkg):
+    _log("[gpu] reproducibility")
+    d, p = load_fixture("sp_tau1_box_20x10x12")
+    outs = []
+    for _ in range(2):
+        s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp32")
+        s.step(37)
+        outs.append((s.state(), s.diagnostics(p["box"])))
+    for a, b in zip(outs[0][0], outs[1][0]):
+        np.testing.assert_array_equal(a, b)
+    for k in outs[0][1]:
+        assert np.array_equal(np.asarray(outs[0][1][k]), np.asarray(outs[1][1][k])), k
+
+
+def test_nan_is_reported(pkg):
+    """A non-finite population is caught by the device check (the reference's NaN stop)."""
+    _log("[gpu] nan detection")
+    d, p = load_fixture("sp_mrt_16x12x8")
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    f = d["f_1"].copy()
+    f[17, 3] = np.nan
+    s.load_state(f, d["u_1"], d["rho_1"])
+    s.step(2)
+    assert s.diagnostics()["nan_count"] >= 1
+
+
+@pytest.mark.parametrize("precision,k_tol", [("fp32", 1e-4), ("fp16", 2e-2)])
+def test_reduced_precision_permeability_vs_fp64(pkg, precision, k_tol):
+    """FP32 shifted populations (and FP16 storage) against the FP64 build on a sphere pack run
+    towards steady state; the velocity L2 error and the permeability error are printed."""
+    _log(f"[gpu] {precision} vs fp64 permeability")
+    solid = pkg.geometry.sphere_pack((40, 40, 40), 5.0, 0.45, seed=21)
+    runs = {}
+    for prec in ("fp64", precision):
+        s = pkg.SinglePhase(solid, 1.0 / 6.0, (1e-6, 0, 0), precision=prec)
+        s.step(3000)
+        k, _, _ = s.Comput_Perm(1)
+        runs[prec] = (k[0], s.velocity())
+    k64, u64 = runs["fp64"]
+    kp, up = runs[precision]
+    k_err = abs(kp - k64) / abs(k64)
+    u_err = rel_err(up, u64)
+    _log(f"[gpu] {precision}: permeability rel err {k_err:.3e}, velocity L2 rel err {u_err:.3e}")
+    assert k_err < k_tol
+    assert u_err < 10 * k_tol
