@@ -1,0 +1,56 @@
+// copy_probe.hip -- a plain device-to-device copy kernel of known size.
+//
+// Used two ways: (1) its rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate the counters
+// (known bytes per dispatch / counter reading), so the step kernel's counter readings can be
+// turned into HBM bytes; (2) its bandwidth is the practical copy ceiling of the device.
+//   copy_probe [MiB per buffer, default 1024] [repetitions, default 20]
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+__global__ void __launch_bounds__(256) copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+#define CHECK(x)                                                                        \
+    do {                                                                                \
+        hipError_t e = (x);                                                             \
+        if (e != hipSuccess) {                                                          \
+            std::fprintf(stderr, "%s: %s\n", #x, hipGetErrorString(e));               \
+            return 1;                                                                   \
+        }                                                                               \
+    } while (0)
+
+int main(int argc, char **argv)
+{
+    const size_t mib = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 1024;
+    const int reps = argc > 2 ? std::atoi(argv[2]) : 20;
+    const size_t bytes = mib << 20, n = bytes / sizeof(float4);
+    float4 *a = nullptr, *b = nullptr;
+    CHECK(hipMalloc(&a, bytes));
+    CHECK(hipMalloc(&b, bytes));
+    CHECK(hipMemset(a, 1, bytes));
+    CHECK(hipMemset(b, 0, bytes));
+    const int grid = 256 * 8 * 4;   // 8 workgroups of 256 per CU, 4 passes per thread group
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    copy_kernel<<<grid, 256>>>(a, b, n);
+    CHECK(hipDeviceSynchronize());
+    CHECK(hipEventRecord(e0));
+    for (int r = 0; r < reps; ++r) copy_kernel<<<grid, 256>>>(a, b, n);
+    CHECK(hipEventRecord(e1));
+    CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHECK(hipEventElapsedTime(&ms, e0, e1));
+    const double per = ms / 1e3 / reps;
+    std::printf("copy_probe: %zu MiB read + %zu MiB written per dispatch, %.2f us, %.1f GB/s\n", mib, mib,
+                per * 1e6, 2.0 * bytes / per / 1e9);
+    std::printf("bytes_read_per_dispatch: %zu\nbytes_written_per_dispatch: %zu\n", bytes, bytes);
+    CHECK(hipFree(a));
+    CHECK(hipFree(b));
+    return 0;
+}
