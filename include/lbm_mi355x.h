@@ -94,6 +94,52 @@ int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out);
 /* number of time steps taken since create/init/load */
 int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps);
 
+/* ------------------------------------------------------------------ colour gradient */
+typedef struct lbm_cg_params {
+    int32_t nx, ny, nz;       /* box size in voxels (INPUT_CG_SPARSE.dat line 4) */
+    double nu_l, nu_g;        /* viscosities of component A (psi = 1) and B (psi = -1), lines 19-20 */
+    double cos_theta;         /* ContactAngle_parameter: psi at solid sites (CG:2512) */
+    double cap_a;             /* CapA, surface-tension parameter (CG:3137-3143) */
+    double g[3];              /* body force gxs, gys, gzs */
+    int32_t bodyforce_apply;  /* 0 both phases; 1 / -1 force only psi >= rel_perm_psi / psi <= -rel_perm_psi2 (CG:3040) */
+    double rel_perm_psi;      /* phase-1 flux threshold, also the force gate (CG:6366-6404) */
+    double rel_perm_psi2;     /* phase-2 flux threshold */
+    double rho0;              /* rest density of the shifted storage, 1.0 */
+    int32_t precision;        /* LBM_FP64 / LBM_FP32 (FP16 is not offered for the colour-gradient path) */
+    int32_t device;
+} lbm_cg_params;
+
+typedef struct lbm_cg_diag {
+    double Ql[3];        /* sum u over psi >= rel_perm_psi (in the box) -- Comput_Perm, CG:6366-6404 */
+    double Qg[3];        /* sum u over psi <= -rel_perm_psi2 */
+    double err, umax, sum_speed, nan_count;   /* as lbm_sp_diag (Error, CG:4979-5040) */
+    double wet_count;    /* nodes with psi >= 0 in the box -- Comput_Saturation, CG:6640-6678 */
+    double count_in_box; /* fluid nodes in the box */
+} lbm_cg_diag;
+
+/* geometry, numbering, neighbour lists, allocation (CG:1887-2308, CG:598-660) */
+int lbm_cg_create(const lbm_cg_params *p, const int32_t *solid_ijk, lbm_handle **out);
+int lbm_cg_count(const lbm_handle *h, int64_t *count);
+/* init from a phase field given per voxel, double [i][j][k], clamped to [-1, 1] as the
+ * reference clamps phase.dat (CG:2150-2155); rho = 1, rho_r = (psi+1)/2, u = u0 -- the
+ * ini_Sat < 0 path of init (CG:2514-2547) */
+int lbm_cg_init(lbm_handle *h, const double *psi_ijk, const double u0[3]);
+/* load a post-streaming state in the CG velocity ordering -- Backup_init (CG:6765-7018) */
+int lbm_cg_load_state(lbm_handle *h, const double *f, const double *u, const double *rho, const double *psi,
+                      const double *rho_r, const double *rho_b);
+/* nsteps x (collision + comput_macro_variables) (CG:938, CG:966; collision CG:2687-3660,
+ * comput_macro_variables CG:3698-4099): per step one colour-gradient collide launch and
+ * one colour-streaming launch */
+int lbm_cg_step(lbm_handle *h, int32_t nsteps);
+int lbm_cg_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms);
+/* change the body force between steps, e.g. the stabiliser switch at stab_time (CG:934-935) */
+int lbm_cg_set_body_force(lbm_handle *h, const double g[3]);
+/* state in the reference's layouts (CG ordering for f); any pointer may be NULL -- Backup (CG:7020-7090) */
+int lbm_cg_get_state(lbm_handle *h, double *f, double *u, double *rho, double *psi, double *rho_r, double *rho_b);
+/* FP64 device reductions: phase fluxes, Error, saturation count (CG:4979, 6298, 6604) */
+int lbm_cg_diagnostics(lbm_handle *h, const int32_t *box, lbm_cg_diag *out);
+int lbm_cg_steps_done(const lbm_handle *h, int64_t *steps);
+
 /* ------------------------------------------------------------------ common */
 const char *lbm_last_error(const lbm_handle *h);   /* NULL handle: last create() failure */
 void lbm_destroy(lbm_handle *h);

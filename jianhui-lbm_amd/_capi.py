@@ -33,6 +33,20 @@ class SpDiag(ctypes.Structure):
                 ("sum_speed", ctypes.c_double), ("nan_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
 
 
+class CgParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nu_l", ctypes.c_double), ("nu_g", ctypes.c_double), ("cos_theta", ctypes.c_double),
+                ("cap_a", ctypes.c_double), ("g", ctypes.c_double * 3), ("bodyforce_apply", ctypes.c_int32),
+                ("rel_perm_psi", ctypes.c_double), ("rel_perm_psi2", ctypes.c_double), ("rho0", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class CgDiag(ctypes.Structure):
+    _fields_ = [("Ql", ctypes.c_double * 3), ("Qg", ctypes.c_double * 3), ("err", ctypes.c_double),
+                ("umax", ctypes.c_double), ("sum_speed", ctypes.c_double), ("nan_count", ctypes.c_double),
+                ("wet_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -73,6 +87,16 @@ def lib():
         L.lbm_sp_get_state.argtypes = [vp, dp, dp, dp]
         L.lbm_sp_diagnostics.argtypes = [vp, ip, ctypes.POINTER(SpDiag)]
         L.lbm_sp_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_cg_create.argtypes = [ctypes.POINTER(CgParams), ip, ctypes.POINTER(vp)]
+        L.lbm_cg_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_cg_init.argtypes = [vp, dp, dp]
+        L.lbm_cg_load_state.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.lbm_cg_step.argtypes = [vp, ctypes.c_int32]
+        L.lbm_cg_step_timed.argtypes = [vp, ctypes.c_int32, dp]
+        L.lbm_cg_set_body_force.argtypes = [vp, dp]
+        L.lbm_cg_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.lbm_cg_diagnostics.argtypes = [vp, ip, ctypes.POINTER(CgDiag)]
+        L.lbm_cg_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 

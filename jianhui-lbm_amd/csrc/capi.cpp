@@ -12,6 +12,11 @@ const lbm::SpSolver *as_sp(const lbm_handle *h)
 {
     return (h && h->kind == lbm_handle::SP) ? static_cast<const lbm::SpSolver *>(h) : nullptr;
 }
+lbm::CgSolver *as_cg(lbm_handle *h) { return (h && h->kind == lbm_handle::CG) ? static_cast<lbm::CgSolver *>(h) : nullptr; }
+const lbm::CgSolver *as_cg(const lbm_handle *h)
+{
+    return (h && h->kind == lbm_handle::CG) ? static_cast<const lbm::CgSolver *>(h) : nullptr;
+}
 }  // namespace
 
 extern "C" {
@@ -96,6 +101,88 @@ int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out)
 int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps)
 {
     auto *s = as_sp(h);
+    if (!s || !steps) return LBM_ERR_ARG;
+    *steps = s->steps_done();
+    return LBM_OK;
+}
+
+int lbm_cg_create(const lbm_cg_params *p, const int32_t *solid_ijk, lbm_handle **out)
+{
+    if (!p || !solid_ijk || !out) {
+        g_create_error = "lbm_cg_create: null argument";
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    auto *s = new (std::nothrow) lbm::CgSolver();
+    if (!s) {
+        g_create_error = "lbm_cg_create: out of host memory";
+        return LBM_ERR_NOMEM;
+    }
+    int rc = s->create(*p, solid_ijk);
+    if (rc != LBM_OK) {
+        g_create_error = s->last_error;
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_cg_count(const lbm_handle *h, int64_t *count)
+{
+    auto *s = as_cg(h);
+    if (!s || !count) return LBM_ERR_ARG;
+    *count = s->count();
+    return LBM_OK;
+}
+
+int lbm_cg_init(lbm_handle *h, const double *psi_ijk, const double u0[3])
+{
+    auto *s = as_cg(h);
+    return s ? s->init(psi_ijk, u0) : LBM_ERR_ARG;
+}
+
+int lbm_cg_load_state(lbm_handle *h, const double *f, const double *u, const double *rho, const double *psi,
+                      const double *rho_r, const double *rho_b)
+{
+    auto *s = as_cg(h);
+    return s ? s->load_state(f, u, rho, psi, rho_r, rho_b) : LBM_ERR_ARG;
+}
+
+int lbm_cg_step(lbm_handle *h, int32_t nsteps)
+{
+    auto *s = as_cg(h);
+    return s ? s->step(nsteps, nullptr) : LBM_ERR_ARG;
+}
+
+int lbm_cg_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms)
+{
+    auto *s = as_cg(h);
+    if (!s || !elapsed_ms) return LBM_ERR_ARG;
+    return s->step(nsteps, elapsed_ms);
+}
+
+int lbm_cg_set_body_force(lbm_handle *h, const double g[3])
+{
+    auto *s = as_cg(h);
+    return s ? s->set_body_force(g) : LBM_ERR_ARG;
+}
+
+int lbm_cg_get_state(lbm_handle *h, double *f, double *u, double *rho, double *psi, double *rho_r, double *rho_b)
+{
+    auto *s = as_cg(h);
+    return s ? s->get_state(f, u, rho, psi, rho_r, rho_b) : LBM_ERR_ARG;
+}
+
+int lbm_cg_diagnostics(lbm_handle *h, const int32_t *box, lbm_cg_diag *out)
+{
+    auto *s = as_cg(h);
+    return s ? s->diagnostics(box, out) : LBM_ERR_ARG;
+}
+
+int lbm_cg_steps_done(const lbm_handle *h, int64_t *steps)
+{
+    auto *s = as_cg(h);
     if (!s || !steps) return LBM_ERR_ARG;
     *steps = s->steps_done();
     return LBM_OK;

@@ -1,0 +1,748 @@
+// cg.hip -- colour-gradient two-component sparse D3Q19 MRT (the reference's CG solver).
+//
+// One reference time step (collision CG:2687-3660 + comput_macro_variables CG:3698-4099)
+// becomes two launches:
+//
+//  cg_collide_kernel  pull f from the upstream nodes, colour gradient C from psi at the 18
+//                     neighbour sites (psi_solid at solid sites), MRT collision in the CG
+//                     basis with the surface-tension equilibrium and the psi-interpolated
+//                     viscosity, post-collision f written coalesced; plus an 8-value colour
+//                     state per node (rho_r, rho_b, u, C/|C|) for the colour streaming.
+//  cg_colour_kernel   colour streaming in pull form: rho_r(x) = sum_q g_r,q(x - e_q), or
+//                     g_r,opp(q)(x) when x - e_q is solid, with each upstream g recomputed
+//                     (equilibrium split + recolouring) from that node's colour state.  The
+//                     reference pushes and accumulates (rhor[nbr] += g_r, CG:3513-3565); the
+//                     pull form sums the same terms without atomics, in a fixed order.
+//
+// Data layout (row stride P):
+//   pops[2] 19 x P  deviations f - w rho0, canonical (SP) velocity order
+//   nbr     18 x P  upstream node per direction, BB_FLAG = solid upstream site
+//   col      2 x P  rho_r, rho_b after colour streaming (the reference's rho_r/rho_b)
+//   psi      1 x P  (rho_r - rho_b)/(rho_r + rho_b)
+//   cs       8 x P  per node, contiguous: rho_r, rho_b, ux, uy, uz, nx, ny, nz
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "collide.h"
+#include "common.h"
+#include "handle.h"
+#include "pull.h"
+#include "storage.h"
+
+namespace lbm {
+
+template <typename ST>
+struct CgArgs {
+    using C = typename Store<ST>::C;
+    const ST *__restrict__ src;
+    ST *__restrict__ dst;
+    const uint32_t *__restrict__ nbr;
+    C *__restrict__ col;
+    C *__restrict__ psi;
+    C *__restrict__ cs;
+    C *__restrict__ u_aux;
+    const C *__restrict__ rho_aux;
+    int64_t N, P;
+    C gx, gy, gz, rho0, cap_a, psi_solid, rel1, rel2;
+    C wl, wg, lg0, l1, l2, g1, g2, delta;
+    C ux_i, uy_i, uz_i;
+    int32_t bfa;
+    float scale, inv_scale;
+};
+
+template <typename C>
+__device__ __forceinline__ void store_cs(C *cs, int64_t n, const C (&v)[8]);
+template <>
+__device__ __forceinline__ void store_cs<float>(float *cs, int64_t n, const float (&v)[8])
+{
+    float4 *p = reinterpret_cast<float4 *>(cs + 8 * n);
+    p[0] = make_float4(v[0], v[1], v[2], v[3]);
+    p[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+template <>
+__device__ __forceinline__ void store_cs<double>(double *cs, int64_t n, const double (&v)[8])
+{
+    double2 *p = reinterpret_cast<double2 *>(cs + 8 * n);
+    p[0] = make_double2(v[0], v[1]);
+    p[1] = make_double2(v[2], v[3]);
+    p[2] = make_double2(v[4], v[5]);
+    p[3] = make_double2(v[6], v[7]);
+}
+template <typename C>
+__device__ __forceinline__ void load_cs(const C *cs, int64_t n, C (&v)[8]);
+template <>
+__device__ __forceinline__ void load_cs<float>(const float *cs, int64_t n, float (&v)[8])
+{
+    const float4 *p = reinterpret_cast<const float4 *>(cs + 8 * n);
+    const float4 a = p[0], b = p[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+template <>
+__device__ __forceinline__ void load_cs<double>(const double *cs, int64_t n, double (&v)[8])
+{
+    const double2 *p = reinterpret_cast<const double2 *>(cs + 8 * n);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double2 a = p[k];
+        v[2 * k] = a.x;
+        v[2 * k + 1] = a.y;
+    }
+}
+
+template <typename ST, int MODE>
+__device__ __forceinline__ void cg_node_state(const CgArgs<ST> &a, int64_t n, typename Store<ST>::C (&h)[Q],
+                                              typename Store<ST>::C (&m)[Q], uint32_t (&nb)[Q - 1],
+                                              Macro<typename Store<ST>::C> &mc)
+{
+    using C = typename Store<ST>::C;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) nb[q - 1] = a.nbr[(int64_t)(q - 1) * a.P + n];
+    if (MODE == K_INIT) {
+        mc.rho = C(1);
+        mc.drho = C(1) - a.rho0;
+        mc.ux = a.ux_i;
+        mc.uy = a.uy_i;
+        mc.uz = a.uz_i;
+        sp_eq_moments(mc, m);   // f = M^-1 meq(1, u0) = feq(1, u0) (CG:2644-2681)
+        return;
+    }
+    h[0] = Store<ST>::load(a.src[n], a.inv_scale);
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        if (MODE == K_DIRECT) {
+            h[q] = Store<ST>::load(a.src[(int64_t)q * a.P + n], a.inv_scale);
+        } else {
+            const uint32_t v = nb[q - 1];
+            const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+            h[q] = Store<ST>::load(a.src[row * a.P + (int64_t)(v & ~BB_FLAG)], a.inv_scale);
+        }
+    }
+    moments_fwd(h, m);
+    if (MODE == K_DIRECT) {
+        mc.rho = a.rho_aux[n];
+        mc.drho = mc.rho - a.rho0;
+        mc.ux = a.u_aux[n];
+        mc.uy = a.u_aux[a.P + n];
+        mc.uz = a.u_aux[2 * a.P + n];
+    } else {
+        mc = macro_from_moments(m, a.rho0, a.gx, a.gy, a.gz);
+    }
+}
+
+template <typename ST, int MODE, bool STORE_U>
+__global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    C h[Q], m[Q];
+    uint32_t nb[Q - 1];
+    Macro<C> mc;
+    cg_node_state<ST, MODE>(a, n, h, m, nb, mc);
+    const C rr = a.col[n], rb = a.col[a.P + n], ps = a.psi[n];
+
+    // colour gradient C = sum_q 3 w_q e_q psi(x + e_q) (CG:2903-3012); x + e_q is the upstream
+    // site of the opposite direction
+    C cx = C(0), cy = C(0), cz = C(0);
+    bool wall = false;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        const uint32_t v = nb[opp(q) - 1];
+        const bool solid = (v & BB_FLAG) != 0;
+        wall |= solid;
+        const C pq = solid ? a.psi_solid : a.psi[v & ~BB_FLAG];
+        const C wq = C(3.0 * weight(q));
+        if (EX[q] != 0) cx += C(EX[q]) * wq * pq;
+        if (EY[q] != 0) cy += C(EY[q]) * wq * pq;
+        if (EZ[q] != 0) cz += C(EZ[q]) * wq * pq;
+    }
+    // no colour gradient next to a wall inside a nearly pure phase (CG:3014-3015)
+    const C dr = rr - rb;
+    if (wall && (dr >= C(0.9) || dr <= C(-0.9))) cx = cy = cz = C(0);
+    const C cc = sqrt(cx * cx + cy * cy + cz * cz);
+    C nx = C(0), ny = C(0), nz = C(0);
+    if (cc > C(0)) {
+        const C inv = C(1) / cc;
+        nx = cx * inv;
+        ny = cy * inv;
+        nz = cz * inv;
+    }
+    // body force gated by phase (CG:3040) -- the velocity keeps the full force (CG:3737-3739)
+    const bool gate = (a.bfa == 1 && ps < a.rel1) || (a.bfa == -1 && ps > -a.rel2);
+    const C gx = gate ? C(0) : a.gx, gy = gate ? C(0) : a.gy, gz = gate ? C(0) : a.gz;
+    // viscosity interpolated in psi (CG:3163-3176)
+    C sv;
+    if (ps > C(0))
+        sv = ps > a.delta ? a.wl : a.lg0 + a.l1 * ps + a.l2 * ps * ps;
+    else
+        sv = ps < -a.delta ? a.wg : a.lg0 + a.g1 * ps + a.g2 * ps * ps;
+
+    if (STORE_U && MODE != K_DIRECT) {
+        a.u_aux[n] = mc.ux;
+        a.u_aux[a.P + n] = mc.uy;
+        a.u_aux[2 * a.P + n] = mc.uz;
+    }
+    cg_relax(m, mc, gx, gy, gz, sv, a.cap_a * cc, nx, ny, nz, h);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a.dst[(int64_t)q * a.P + n] = Store<ST>::put(h[q], a.scale);
+    const C st[8] = {rr, rb, mc.ux, mc.uy, mc.uz, nx, ny, nz};
+    store_cs(a.cs, n, st);
+}
+
+// g_r, g_b leaving a node with colour state v in direction sgn*e_q (sgn = -1: the opposite
+// direction, used for the bounce-back term of the pull)
+template <typename C>
+__device__ __forceinline__ void colour_pull_term(const int q, const C (&v)[8], C sgn, C &gr, C &gb)
+{
+    const C rr = v[0], rb = v[1], ux = v[2], uy = v[3], uz = v[4];
+    const C uu = ux * ux + uy * uy + uz * uz;
+    const C eu = sgn * (C(EX[q]) * ux + C(EY[q]) * uy + C(EZ[q]) * uz);
+    const C base = C(1) - C(1.5) * uu + C(4.5) * eu * eu;
+    const C phq = base + C(3) * eu, php = base - C(3) * eu;
+    const C w = C(weight(q));
+    gr = w * rr * phq;
+    gb = w * rb * phq;
+    C mn = gr;
+    const C t1 = w * rr * php, t2 = gb, t3 = w * rb * php;
+    mn = t1 < mn ? t1 : mn;
+    mn = t2 < mn ? t2 : mn;
+    mn = t3 < mn ? t3 : mn;
+    const C en = sgn * (C(EX[q]) * v[5] + C(EY[q]) * v[6] + C(EZ[q]) * v[7]);
+    gr += mn * en;
+    gb -= mn * en;
+}
+
+// The colour sums are formed in FP64 whatever the storage type: each node's rho_r is split
+// over 19 directions and re-summed at 19 receivers, and in FP32 the rounding of those products
+// is biased enough to drift the total colour mass (about 3e-6 of it per 200 steps); in FP64 the
+// sums are exact to ~1e-16 and only the unbiased rounding of the stored totals remains.
+template <typename C>
+__global__ void __launch_bounds__(BLOCK) cg_colour_kernel(const uint32_t *__restrict__ nbr, const C *__restrict__ cs,
+                                                          C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P)
+{
+    const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    C own_s[8];
+    load_cs(cs, n, own_s);
+    double own[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) own[k] = (double)own_s[k];
+    // rest population stays on the node (e_0 = 0: no recolouring term)
+    const double uu = own[2] * own[2] + own[3] * own[3] + own[4] * own[4];
+    const double w0 = weight(0) * (1.0 - 1.5 * uu);
+    double rr = w0 * own[0], rb = w0 * own[1];
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        const uint32_t v = nbr[(int64_t)(q - 1) * P + n];
+        double gr, gb;
+        if (v & BB_FLAG) {
+            colour_pull_term<double>(q, own, -1.0, gr, gb);
+        } else {
+            C up_s[8];
+            load_cs(cs, (int64_t)v, up_s);
+            double up[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) up[k] = (double)up_s[k];
+            colour_pull_term<double>(q, up, 1.0, gr, gb);
+        }
+        rr += gr;
+        rb += gb;
+    }
+    col[n] = (C)rr;
+    col[P + n] = (C)rb;
+    psi[n] = (C)((rr - rb) / (rr + rb));
+}
+
+// ---------------------------------------------------------------- diagnostics
+constexpr int CG_NDIAG = 13;   // Ql[3] Qg[3] err_num err_den speed umax2 nan wet box
+constexpr uint32_t CG_MAX_MASK = 1u << 9;
+
+struct CgBox {
+    int32_t on, xn, xp, yn, yp, zn, zp, ny, nz;
+};
+
+template <typename ST, int MODE>
+__global__ void __launch_bounds__(BLOCK) cg_diag_kernel(CgArgs<ST> a, const int32_t *__restrict__ supinv, CgBox box,
+                                                        double *__restrict__ partial)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double v[CG_NDIAG];
+#pragma unroll
+    for (int k = 0; k < CG_NDIAG; ++k) v[k] = 0.0;
+    if (n < a.N) {
+        C h[Q], m[Q];
+        uint32_t nb[Q - 1];
+        Macro<C> mc;
+        cg_node_state<ST, MODE>(a, n, h, m, nb, mc);
+        const double ux = (double)mc.ux, uy = (double)mc.uy, uz = (double)mc.uz, ps = (double)a.psi[n];
+        bool inside = true;
+        if (box.on) {
+            const int32_t s = supinv[n];
+            const int32_t nyz = box.ny * box.nz;
+            const int32_t i = s / nyz, j = (s % nyz) / box.nz, k = s % box.nz;
+            inside = i >= box.xn && i <= box.xp && j >= box.yn && j <= box.yp && k >= box.zn && k <= box.zp;
+        }
+        if (inside) {
+            if (ps >= (double)a.rel1) {
+                v[0] = ux;
+                v[1] = uy;
+                v[2] = uz;
+            }
+            if (ps <= -(double)a.rel2) {
+                v[3] = ux;
+                v[4] = uy;
+                v[5] = uz;
+            }
+            v[11] = ps >= 0.0 ? 1.0 : 0.0;
+            v[12] = 1.0;
+        }
+        if (n < a.N - 1) {   // Error skips the last node (CG:5001)
+            const double dx = ux - (double)a.u_aux[n], dy = uy - (double)a.u_aux[a.P + n],
+                         dz = uz - (double)a.u_aux[2 * a.P + n];
+            const double s2 = ux * ux + uy * uy + uz * uz;
+            v[6] = dx * dx + dy * dy + dz * dz;
+            v[7] = s2;
+            v[8] = sqrt(s2);
+            v[9] = s2;
+        }
+        v[10] = (isfinite(ux) && isfinite(uy) && isfinite(uz) && isfinite(ps)) ? 0.0 : 1.0;
+    }
+    bool is_max[CG_NDIAG];
+#pragma unroll
+    for (int k = 0; k < CG_NDIAG; ++k) is_max[k] = (CG_MAX_MASK >> k) & 1u;
+    block_reduce_store<CG_NDIAG>(v, is_max, partial + (int64_t)blockIdx.x * CG_NDIAG);
+}
+
+// ---------------------------------------------------------------- state I/O
+template <typename ST, int MODE>
+__global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, double *__restrict__ f_out,
+                                                             double *__restrict__ u_out, double *__restrict__ rho_out,
+                                                             double *__restrict__ psi_out, double *__restrict__ rr_out,
+                                                             double *__restrict__ rb_out)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    C h[Q], m[Q];
+    uint32_t nb[Q - 1];
+    Macro<C> mc;
+    cg_node_state<ST, MODE>(a, n, h, m, nb, mc);
+    if (f_out) {
+        double f[Q];
+        if (MODE == K_INIT) {
+            const double ux = (double)mc.ux, uy = (double)mc.uy, uz = (double)mc.uz;
+            const double uv = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const double eu = EX[q] * ux + EY[q] * uy + EZ[q] * uz;
+                f[q] = weight(q) * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = (double)h[q] + weight(q) * (double)a.rho0;
+        }
+#pragma unroll
+        for (int qc = 0; qc < Q; ++qc) f_out[n * Q + qc] = f[cg_to_canon(qc)];
+    }
+    if (u_out) {
+        u_out[n * 3 + 0] = (double)mc.ux;
+        u_out[n * 3 + 1] = (double)mc.uy;
+        u_out[n * 3 + 2] = (double)mc.uz;
+    }
+    if (rho_out) rho_out[n] = (double)mc.rho;
+    if (psi_out) psi_out[n] = (double)a.psi[n];
+    if (rr_out) rr_out[n] = (double)a.col[n];
+    if (rb_out) rb_out[n] = (double)a.col[a.P + n];
+}
+
+template <typename ST>
+__global__ void __launch_bounds__(BLOCK) cg_load_state_kernel(CgArgs<ST> a, const double *__restrict__ f,
+                                                              const double *__restrict__ u, const double *__restrict__ rho,
+                                                              const double *__restrict__ psi, const double *__restrict__ rr,
+                                                              const double *__restrict__ rb, typename Store<ST>::C *rho_aux)
+{
+    using C = typename Store<ST>::C;
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+#pragma unroll
+    for (int qc = 0; qc < Q; ++qc) {
+        const int q = cg_to_canon(qc);
+        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((C)(f[n * Q + qc] - weight(q) * (double)a.rho0), a.scale);
+    }
+    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (C)u[n * 3 + c];
+    rho_aux[n] = (C)rho[n];
+    a.psi[n] = (C)psi[n];
+    a.col[n] = (C)rr[n];
+    a.col[a.P + n] = (C)rb[n];
+}
+
+template <typename C>
+__global__ void __launch_bounds__(BLOCK) cg_init_colour_kernel(const double *__restrict__ psi_node, C *col, C *psi,
+                                                               int64_t N, int64_t P)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    double p = psi_node[n];
+    p = p > 1.0 ? 1.0 : (p < -1.0 ? -1.0 : p);
+    const double rr = (p * 1.0 + 1.0) / 2;   // rho_r = (psi rho + rho)/2 with rho = 1 (CG:2531-2532)
+    psi[n] = (C)p;
+    col[n] = (C)rr;
+    col[P + n] = (C)(1.0 - rr);
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+template <typename ST>
+CgArgs<ST> make_cg_args(const lbm_cg_params &p, const double *visc, const double *u_i, const void *src, void *dst,
+                        const uint32_t *nbr, void *col, void *psi, void *cs, void *u_aux, const void *rho_aux,
+                        int64_t N, int64_t P)
+{
+    using C = typename Store<ST>::C;
+    CgArgs<ST> a;
+    a.src = (const ST *)src;
+    a.dst = (ST *)dst;
+    a.nbr = nbr;
+    a.col = (C *)col;
+    a.psi = (C *)psi;
+    a.cs = (C *)cs;
+    a.u_aux = (C *)u_aux;
+    a.rho_aux = (const C *)rho_aux;
+    a.N = N;
+    a.P = P;
+    a.gx = (C)p.g[0];
+    a.gy = (C)p.g[1];
+    a.gz = (C)p.g[2];
+    a.rho0 = (C)p.rho0;
+    a.cap_a = (C)p.cap_a;
+    a.psi_solid = (C)p.cos_theta;
+    a.rel1 = (C)p.rel_perm_psi;
+    a.rel2 = (C)p.rel_perm_psi2;
+    a.wl = (C)visc[0];
+    a.wg = (C)visc[1];
+    a.lg0 = (C)visc[2];
+    a.l1 = (C)visc[3];
+    a.l2 = (C)visc[4];
+    a.g1 = (C)visc[5];
+    a.g2 = (C)visc[6];
+    a.delta = (C)visc[7];
+    a.ux_i = (C)u_i[0];
+    a.uy_i = (C)u_i[1];
+    a.uz_i = (C)u_i[2];
+    a.bfa = p.bodyforce_apply;
+    a.scale = 1.0f;
+    a.inv_scale = 1.0f;
+    return a;
+}
+
+int cg_kmode(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT ? K_DIRECT : K_PULL); }
+
+}  // namespace
+
+CgSolver::~CgSolver() { release(); }
+
+void CgSolver::release()
+{
+    (void)hipSetDevice(device_);
+    for (void *&p : pops_)
+        if (p) (void)hipFree(p), p = nullptr;
+    void **bufs[] = {(void **)&d_nbr_, (void **)&d_supinv_, &d_col_, &d_psi_, &d_cs_, &d_u_, &d_rho_,
+                     (void **)&d_partial_, (void **)&d_result_};
+    for (void **b : bufs)
+        if (*b) (void)hipFree(*b), *b = nullptr;
+    if (h_result_) (void)hipHostFree(h_result_), h_result_ = nullptr;
+    if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
+    if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
+    if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
+}
+
+int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
+{
+    prm_ = p;
+    if (p.precision != LBM_FP64 && p.precision != LBM_FP32)
+        return fail(LBM_ERR_ARG, "colour gradient supports LBM_FP64 and LBM_FP32");
+    if (!(p.nu_l > 0.0) || !(p.nu_g > 0.0)) return fail(LBM_ERR_ARG, "viscosities must be positive");
+    if (prm_.rho0 == 0.0) prm_.rho0 = 1.0;
+    std::string err;
+    int rc = build_sparse_geometry(p.nx, p.ny, p.nz, solid, geo_, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    // viscosity interpolation constants, delta = 0.1 (CG:2368-2377)
+    const double c_s2 = 1.0 / 3.0, delta = 0.1;
+    const double wl = 1.0 / (p.nu_l / (c_s2 * 1.0) + 0.5), wg = 1.0 / (p.nu_g / (c_s2 * 1.0) + 0.5);
+    const double lg0 = 2 * wl * wg / (wl + wg), l1 = 2 * (wl - lg0) / delta, l2 = -l1 / (2 * delta);
+    const double g1 = 2 * (lg0 - wg) / delta, g2 = g1 / (2 * delta);
+    const double v[8] = {wl, wg, lg0, l1, l2, g1, g2, delta};
+    std::memcpy(visc_, v, sizeof(v));
+    ct_bytes_ = p.precision == LBM_FP64 ? 8 : 4;
+    const size_t st = ct_bytes_;
+
+    device_ = p.device;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    LBM_HIP_TRY(hipEventCreate(&ev0_));
+    LBM_HIP_TRY(hipEventCreate(&ev1_));
+    const int64_t P = geo_.P;
+    grid_ = (geo_.N + BLOCK - 1) / BLOCK;
+    for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * P * st));
+    LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * P * sizeof(uint32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_supinv_, (size_t)P * sizeof(int32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_col_, (size_t)2 * P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_psi_, (size_t)P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_cs_, (size_t)8 * P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_u_, (size_t)3 * P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_rho_, (size_t)P * ct_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)grid_ * CG_NDIAG * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&d_result_, 32 * sizeof(double)));
+    LBM_HIP_TRY(hipHostMalloc((void **)&h_result_, 32 * sizeof(double), hipHostMallocDefault));
+    LBM_HIP_TRY(hipMemcpyAsync(d_nbr_, geo_.nbr.data(), geo_.nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               stream_));
+    LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    std::vector<uint32_t>().swap(geo_.nbr);
+    return LBM_OK;   // the state is set by init() or load_state()
+}
+
+int CgSolver::init(const double *psi_ijk, const double u0[3])
+{
+    if (!psi_ijk || !u0) return fail(LBM_ERR_ARG, "null phase field or velocity");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    std::vector<double> pn((size_t)N);
+    for (int64_t n = 0; n < N; ++n) pn[(size_t)n] = psi_ijk[geo_.supinv[(size_t)n]];
+    double *d = nullptr;
+    LBM_HIP_TRY(hipMalloc(&d, (size_t)N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(d, pn.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) {
+        dim3 g((unsigned)grid_), b(BLOCK);
+        if (prm_.precision == LBM_FP64)
+            cg_init_colour_kernel<double><<<g, b, 0, stream_>>>(d, (double *)d_col_, (double *)d_psi_, N, geo_.P);
+        else
+            cg_init_colour_kernel<float><<<g, b, 0, stream_>>>(d, (float *)d_col_, (float *)d_psi_, N, geo_.P);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(d);
+    LBM_HIP_TRY(e);
+    for (int c = 0; c < 3; ++c) u_i_[c] = u0[c];
+    mode_ = STATE_INIT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+int CgSolver::load_state(const double *f, const double *u, const double *rho, const double *psi, const double *rr,
+                         const double *rb)
+{
+    if (!f || !u || !rho || !psi || !rr || !rb) return fail(LBM_ERR_ARG, "null state array");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    double *df = nullptr, *dx = nullptr;
+    LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&dx, (size_t)N * 7 * sizeof(double)));   // u(3), rho, psi, rr, rb
+    double *du = dx, *dr = dx + 3 * N, *dp = dx + 4 * N, *drr = dx + 5 * N, *drb = dx + 6 * N;
+    hipError_t e = hipMemcpyAsync(df, f, (size_t)N * Q * sizeof(double), hipMemcpyHostToDevice, stream_);
+    const double *srcs[5] = {u, rho, psi, rr, rb};
+    double *dsts[5] = {du, dr, dp, drr, drb};
+    const int64_t lens[5] = {3 * N, N, N, N, N};
+    for (int k = 0; k < 5 && e == hipSuccess; ++k)
+        e = hipMemcpyAsync(dsts[k], srcs[k], (size_t)lens[k] * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) {
+        dim3 g((unsigned)grid_), b(BLOCK);
+        if (prm_.precision == LBM_FP64) {
+            auto a = make_cg_args<double>(prm_, visc_, u_i_, nullptr, pops_[cur_], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                          d_rho_, N, geo_.P);
+            cg_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (double *)d_rho_);
+        } else {
+            auto a = make_cg_args<float>(prm_, visc_, u_i_, nullptr, pops_[cur_], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                         d_rho_, N, geo_.P);
+            cg_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (float *)d_rho_);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(df);
+    (void)hipFree(dx);
+    LBM_HIP_TRY(e);
+    mode_ = STATE_DIRECT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+int CgSolver::set_body_force(const double g[3])
+{
+    if (!g) return fail(LBM_ERR_ARG, "null force");
+    for (int c = 0; c < 3; ++c) prm_.g[c] = g[c];
+    return LBM_OK;
+}
+
+template <typename ST>
+static hipError_t cg_launch_collide(const CgArgs<ST> &a, int mode, bool store_u, int64_t grid, hipStream_t s)
+{
+    dim3 g((unsigned)grid), b(BLOCK);
+    if (mode == K_INIT)
+        store_u ? cg_collide_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
+    else if (mode == K_DIRECT)
+        cg_collide_kernel<ST, K_DIRECT, false><<<g, b, 0, s>>>(a);
+    else
+        store_u ? cg_collide_kernel<ST, K_PULL, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_PULL, false><<<g, b, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+int CgSolver::launch_step(bool store_u)
+{
+    const int mode = cg_kmode(mode_);
+    const int nxt = cur_ ^ 1;
+    const int64_t N = geo_.N, P = geo_.P;
+    dim3 g((unsigned)grid_), b(BLOCK);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64) {
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                      d_rho_, N, P);
+        e = cg_launch_collide(a, mode, store_u, grid_, stream_);
+        if (e == hipSuccess) {
+            cg_colour_kernel<double><<<g, b, 0, stream_>>>(d_nbr_, (const double *)d_cs_, (double *)d_col_,
+                                                            (double *)d_psi_, N, P);
+            e = hipGetLastError();
+        }
+    } else {
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                     d_rho_, N, P);
+        e = cg_launch_collide(a, mode, store_u, grid_, stream_);
+        if (e == hipSuccess) {
+            cg_colour_kernel<float><<<g, b, 0, stream_>>>(d_nbr_, (const float *)d_cs_, (float *)d_col_,
+                                                           (float *)d_psi_, N, P);
+            e = hipGetLastError();
+        }
+    }
+    LBM_HIP_TRY(e);
+    cur_ = nxt;
+    mode_ = STATE_PULL;
+    ++steps_;
+    return LBM_OK;
+}
+
+int CgSolver::step(int nsteps, double *elapsed_ms)
+{
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "negative step count");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised: call lbm_cg_init or lbm_cg_load_state");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    if (elapsed_ms) LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = launch_step(s == nsteps - 1);
+        if (rc != LBM_OK) return rc;
+    }
+    if (elapsed_ms) {
+        LBM_HIP_TRY(hipEventRecord(ev1_, stream_));
+        LBM_HIP_TRY(hipEventSynchronize(ev1_));
+        float ms = 0.f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+        *elapsed_ms = ms;
+    }
+    return LBM_OK;
+}
+
+int CgSolver::get_state(double *f, double *u, double *rho, double *psi, double *rr, double *rb)
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    double *bufs[6] = {nullptr};
+    double *outs[6] = {f, u, rho, psi, rr, rb};
+    const int64_t lens[6] = {N * Q, 3 * N, N, N, N, N};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k]) LBM_HIP_TRY(hipMalloc(&bufs[k], (size_t)lens[k] * sizeof(double)));
+    const int mode = cg_kmode(mode_);
+    dim3 g((unsigned)grid_), b(BLOCK);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64) {
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                      d_rho_, N, geo_.P);
+        if (mode == K_INIT)
+            cg_get_state_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        else if (mode == K_DIRECT)
+            cg_get_state_kernel<double, K_DIRECT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        else
+            cg_get_state_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+    } else {
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                     d_rho_, N, geo_.P);
+        if (mode == K_INIT)
+            cg_get_state_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        else if (mode == K_DIRECT)
+            cg_get_state_kernel<float, K_DIRECT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        else
+            cg_get_state_kernel<float, K_PULL><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+    }
+    e = hipGetLastError();
+    for (int k = 0; k < 6 && e == hipSuccess; ++k)
+        if (outs[k]) e = hipMemcpyAsync(outs[k], bufs[k], (size_t)lens[k] * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    for (double *p : bufs)
+        if (p) (void)hipFree(p);
+    LBM_HIP_TRY(e);
+    return LBM_OK;
+}
+
+int CgSolver::diagnostics(const int32_t *box, lbm_cg_diag *out)
+{
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    CgBox cb{};
+    cb.ny = geo_.ny;
+    cb.nz = geo_.nz;
+    if (box) {
+        cb.on = 1;
+        cb.xn = box[0];
+        cb.xp = box[1];
+        cb.yn = box[2];
+        cb.yp = box[3];
+        cb.zn = box[4];
+        cb.zp = box[5];
+    }
+    const int mode = cg_kmode(mode_);
+    dim3 g((unsigned)grid_), b(BLOCK);
+    const int64_t N = geo_.N;
+    if (prm_.precision == LBM_FP64) {
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                      d_rho_, N, geo_.P);
+        if (mode == K_INIT)
+            cg_diag_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+        else if (mode == K_DIRECT)
+            cg_diag_kernel<double, K_DIRECT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+        else
+            cg_diag_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+    } else {
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+                                     d_rho_, N, geo_.P);
+        if (mode == K_INIT)
+            cg_diag_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+        else if (mode == K_DIRECT)
+            cg_diag_kernel<float, K_DIRECT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+        else
+            cg_diag_kernel<float, K_PULL><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
+    }
+    LBM_HIP_TRY(hipGetLastError());
+    reduce_partials_kernel<CG_NDIAG><<<1, BLOCK, 0, stream_>>>(d_partial_, grid_, CG_MAX_MASK, d_result_);
+    LBM_HIP_TRY(hipGetLastError());
+    LBM_HIP_TRY(hipMemcpyAsync(h_result_, d_result_, CG_NDIAG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    const double *r = h_result_;
+    for (int c = 0; c < 3; ++c) {
+        out->Ql[c] = r[c];
+        out->Qg[c] = r[3 + c];
+    }
+    out->err = std::sqrt(r[6]) / (std::sqrt(r[7]) + 1e-30);
+    out->sum_speed = r[8];
+    out->umax = std::sqrt(r[9]);
+    out->nan_count = r[10];
+    out->wet_count = r[11];
+    out->count_in_box = r[12];
+    return LBM_OK;
+}
+
+}  // namespace lbm

@@ -192,4 +192,95 @@ __device__ __forceinline__ void sp_eq_moments(const Macro<T> &mc, T (&m)[Q])
     m[16] = m[17] = m[18] = T(0);
 }
 
+// ----------------------------------------------------------------------------- colour gradient
+// The colour-gradient solver collides in its own moment basis (CG:80-98), which spans the same
+// polynomial spaces as the SP basis: c = T s with s = M_SP h.  T is block diagonal with blocks
+// on (0,1,2), (3,4), (5,6), (7,8), (9,10), (11,12) and the identity elsewhere.
+template <typename T>
+__device__ __forceinline__ void cg_from_sp(const T (&s)[Q], T (&c)[Q])
+{
+#pragma unroll
+    for (int k = 0; k < Q; ++k) c[k] = s[k];
+    c[1] = (T(11) * s[0] + s[1]) * T(1.0 / 19.0);
+    c[2] = s[0] * T(1.0 / 19.0) + s[1] * T(11.0 / 133.0) + s[2] * T(2.0 / 7.0);
+    c[4] = (T(2) * s[3] + T(3) * s[4]) * T(0.2);
+    c[6] = (T(2) * s[5] + T(3) * s[6]) * T(0.2);
+    c[8] = (T(2) * s[7] + T(3) * s[8]) * T(0.2);
+    c[10] = (s[9] + T(2) * s[10]) * T(1.0 / 3.0);
+    c[12] = (s[11] + T(2) * s[12]) * T(1.0 / 3.0);
+}
+
+template <typename T>
+__device__ __forceinline__ void sp_from_cg(const T (&c)[Q], T (&s)[Q])
+{
+#pragma unroll
+    for (int k = 0; k < Q; ++k) s[k] = c[k];
+    s[1] = T(19) * c[1] - T(11) * c[0];
+    s[2] = T(3.5) * (c[2] - c[0] * T(1.0 / 19.0) - s[1] * T(11.0 / 133.0));
+    s[4] = (T(5) * c[4] - T(2) * c[3]) * T(1.0 / 3.0);
+    s[6] = (T(5) * c[6] - T(2) * c[5]) * T(1.0 / 3.0);
+    s[8] = (T(5) * c[8] - T(2) * c[7]) * T(1.0 / 3.0);
+    s[10] = (T(3) * c[10] - c[9]) * T(0.5);
+    s[12] = (T(3) * c[12] - c[11]) * T(0.5);
+}
+
+// Colour-gradient MRT collision (CG:3137-3400).  m: SP-basis deviation moments of the
+// pre-collision populations; mc: rho and u; g: the Guo force after the per-phase gate
+// (CG:3040); sv: the psi-interpolated shear rate (CG:3163-3176); capa_cc = CapA*|C| and n =
+// C/|C| (zero when C = 0) give the surface-tension terms of the equilibrium (CG:3137-3143).
+template <typename T>
+__device__ __forceinline__ void cg_relax(const T (&m)[Q], const Macro<T> &mc, T gx, T gy, T gz, T sv, T capa_cc,
+                                         T nx, T ny, T nz, T (&h)[Q])
+{
+    const T ux = mc.ux, uy = mc.uy, uz = mc.uz;
+    const T sq = T(8) * (T(2) - sv) / (T(8) - sv);
+    T c[Q];
+    cg_from_sp(m, c);
+    // equilibrium in the CG basis with rho_0 = 1 in the velocity terms
+    const T hc = T(0.5) * capa_cc;
+    T eq[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) eq[k] = T(0);
+    eq[0] = mc.drho;
+    eq[1] = ux * ux + uy * uy + uz * uz + capa_cc;
+    eq[3] = ux;
+    eq[5] = uy;
+    eq[7] = uz;
+    eq[9] = T(2) * ux * ux - uy * uy - uz * uz + hc * (T(2) * nx * nx - ny * ny - nz * nz);
+    eq[11] = uy * uy - uz * uz + hc * (ny * ny - nz * nz);
+    eq[13] = ux * uy + hc * (nx * ny);
+    eq[14] = uy * uz + hc * (ny * nz);
+    eq[15] = ux * uz + hc * (nx * nz);
+    // Guo forcing in the CG basis (T applied to the SP closed forms; rows 2,4,6,8,10,12 vanish)
+    T F[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) F[k] = T(0);
+    F[1] = T(2) * (ux * gx + uy * gy + uz * gz);
+    F[3] = gx;
+    F[5] = gy;
+    F[7] = gz;
+    F[9] = T(2) * (T(2) * ux * gx - uy * gy - uz * gz);
+    F[11] = T(2) * (uy * gy - uz * gz);
+    F[13] = ux * gy + uy * gx;
+    F[14] = uy * gz + uz * gy;
+    F[15] = ux * gz + uz * gx;
+    const T av = T(1) - sv, cv = T(1) - T(0.5) * sv;
+    const T aq = T(1) - sq, cq = T(1) - T(0.5) * sq;
+    T cs[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        if (RCLASS[k] == 0)
+            cs[k] = c[k] + F[k];
+        else if (RCLASS[k] == 1)
+            cs[k] = av * c[k] + sv * eq[k] + cv * F[k];
+        else
+            cs[k] = aq * c[k] + sq * eq[k] + cq * F[k];
+    }
+    T s[Q], mt[Q];
+    sp_from_cg(cs, s);
+#pragma unroll
+    for (int k = 0; k < Q; ++k) mt[k] = s[k] * T(1.0 / norm2(k));
+    moments_inv(mt, h);
+}
+
 }  // namespace lbm

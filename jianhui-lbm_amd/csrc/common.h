@@ -67,6 +67,29 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[K], const bool (&
     }
 }
 
+// Second pass over the per-block partials: fixed assignment and fixed tree, so the sums are
+// reproducible run to run.
+template <int K>
+__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double *__restrict__ partial, int64_t nblk,
+                                                                uint32_t max_mask, double *__restrict__ out)
+{
+    double v[K];
+    bool is_max[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        is_max[k] = (max_mask >> k) & 1u;
+        v[k] = 0.0;
+    }
+    for (int64_t b = threadIdx.x; b < nblk; b += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double x = partial[b * K + k];
+            v[k] = is_max[k] ? (x > v[k] ? x : v[k]) : v[k] + x;
+        }
+    }
+    block_reduce_store<K>(v, is_max, out);
+}
+
 struct Status {
     int code = LBM_OK;
     std::string msg;

@@ -85,4 +85,50 @@ private:
     int64_t steps_ = 0;
 };
 
+class CgSolver : public lbm_handle {
+public:
+    CgSolver() : lbm_handle(CG) {}
+    ~CgSolver() override;
+
+    int create(const lbm_cg_params &p, const int32_t *solid);
+    int init(const double *psi_ijk, const double u0[3]);
+    int load_state(const double *f, const double *u, const double *rho, const double *psi, const double *rho_r,
+                   const double *rho_b);
+    int step(int nsteps, double *elapsed_ms);
+    int set_body_force(const double g[3]);
+    int get_state(double *f, double *u, double *rho, double *psi, double *rho_r, double *rho_b);
+    int diagnostics(const int32_t *box, lbm_cg_diag *out);
+
+    int64_t count() const { return geo_.N; }
+    int64_t steps_done() const { return steps_; }
+
+private:
+    int launch_step(bool store_u);
+    void release();
+
+    lbm_cg_params prm_{};
+    SparseGeometry geo_;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double visc_[8] = {0};   // wl, wg, lg0, l1, l2, g1, g2, delta (CG:2368-2377)
+    size_t ct_bytes_ = 8;
+    void *pops_[2] = {nullptr, nullptr};
+    int cur_ = 0;
+    uint32_t *d_nbr_ = nullptr;
+    int32_t *d_supinv_ = nullptr;
+    void *d_col_ = nullptr;   // 2 x P: rho_r, rho_b after colour streaming
+    void *d_psi_ = nullptr;   // P: psi = (rho_r - rho_b)/(rho_r + rho_b)
+    void *d_cs_ = nullptr;    // 8 x P (AoS per node): rho_r, rho_b, u, n of the last collision
+    void *d_u_ = nullptr;     // 3 x P: u of the previous step
+    void *d_rho_ = nullptr;   // P: rho of a loaded checkpoint
+    double *d_partial_ = nullptr;
+    double *d_result_ = nullptr;
+    double *h_result_ = nullptr;
+    int64_t grid_ = 0;
+    StateMode mode_ = STATE_EMPTY;
+    double u_i_[3] = {0, 0, 0};
+    int64_t steps_ = 0;
+};
+
 }  // namespace lbm

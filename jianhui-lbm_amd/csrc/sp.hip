@@ -19,11 +19,10 @@
 #include "collide.h"
 #include "common.h"
 #include "handle.h"
+#include "pull.h"
 #include "storage.h"
 
 namespace lbm {
-
-enum KMode { K_INIT = 0, K_DIRECT = 1, K_PULL = 2 };
 
 template <typename ST>
 struct SpArgs {
@@ -39,21 +38,11 @@ struct SpArgs {
     float scale, inv_scale;
 };
 
-// Gather the 19 pre-collision deviations of node n.
 template <typename ST, int MODE>
 __device__ __forceinline__ void gather(const SpArgs<ST> &a, int64_t n, typename Store<ST>::C (&h)[Q])
 {
-    h[0] = Store<ST>::load(a.src[n], a.inv_scale);
-#pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        if (MODE == K_DIRECT) {
-            h[q] = Store<ST>::load(a.src[(int64_t)q * a.P + n], a.inv_scale);
-        } else {
-            const uint32_t v = a.nbr[(int64_t)(q - 1) * a.P + n];
-            const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
-            h[q] = Store<ST>::load(a.src[row * a.P + (int64_t)(v & ~BB_FLAG)], a.inv_scale);
-        }
-    }
+    uint32_t nb[Q - 1];
+    gather_pops<ST, MODE>(a.src, a.nbr, a.P, n, a.inv_scale, h, nb);
 }
 
 template <typename ST, int MODE>
@@ -160,29 +149,6 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
     }
     const bool is_max[NDIAG + 1] = {false, false, false, false, false, false, true, false, false};
     block_reduce_store<NDIAG + 1>(v, is_max, partial + (int64_t)blockIdx.x * (NDIAG + 1));
-}
-
-// Second pass over the per-block partials: fixed assignment and fixed tree, so the sums are
-// reproducible run to run.
-template <int K>
-__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double *__restrict__ partial, int64_t nblk,
-                                                                uint32_t max_mask, double *__restrict__ out)
-{
-    double v[K];
-    bool is_max[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        is_max[k] = (max_mask >> k) & 1u;
-        v[k] = 0.0;
-    }
-    for (int64_t b = threadIdx.x; b < nblk; b += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double x = partial[b * K + k];
-            v[k] = is_max[k] ? (x > v[k] ? x : v[k]) : v[k] + x;
-        }
-    }
-    block_reduce_store<K>(v, is_max, out);
 }
 
 // ---------------------------------------------------------------- state I/O

@@ -152,3 +152,125 @@ class SinglePhase:
         por = d["count_in_box"] / V
         avg = d["Q"] / (V * por)
         return k, rel, float(np.sqrt(avg @ avg))
+
+
+class ColorGradient:
+    """Sparse D3Q19 MRT colour-gradient two-component solver on one GPU (the reference's CG
+    program, MP_3D/Multi_Component_CG_MPI_SPARSE_Software.cpp).
+
+    ``psi_field`` for ``init`` is the per-voxel phase field ``[i][j][k]`` (the content of the
+    reference's phase.dat); ``state()`` returns f in the CG program's velocity ordering.
+    """
+
+    def __init__(self, solid: np.ndarray, nu_l: float, nu_g: float, cos_theta: float, cap_a: float,
+                 g: Sequence[float], precision: str = "fp32", bodyforce_apply: int = 0,
+                 rel_perm_psi: float = 0.99, rel_perm_psi2: float = 0.92, rho0: float = 1.0, device: int = 0):
+        L = _capi.lib()
+        solid = np.ascontiguousarray(solid, np.int32)
+        self.shape = solid.shape
+        self.nu_l, self.nu_g = float(nu_l), float(nu_g)
+        self.g = tuple(float(x) for x in g)
+        self.precision = precision
+        self.rel_perm_psi, self.rel_perm_psi2 = float(rel_perm_psi), float(rel_perm_psi2)
+        p = _capi.CgParams()
+        p.nx, p.ny, p.nz = solid.shape
+        p.nu_l, p.nu_g = self.nu_l, self.nu_g
+        p.cos_theta = float(cos_theta)
+        p.cap_a = float(cap_a)
+        for a in range(3):
+            p.g[a] = self.g[a]
+        p.bodyforce_apply = int(bodyforce_apply)
+        p.rel_perm_psi, p.rel_perm_psi2 = self.rel_perm_psi, self.rel_perm_psi2
+        p.rho0 = float(rho0)
+        p.precision = _capi.PRECISIONS[precision]
+        p.device = int(device)
+        h = ctypes.c_void_p()
+        _capi.check(L.lbm_cg_create(ctypes.byref(p), solid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    ctypes.byref(h)), None)
+        self._h = h
+        n = ctypes.c_int64()
+        _capi.check(L.lbm_cg_count(self._h, ctypes.byref(n)), self._h)
+        self.count = int(n.value)
+        self.porosity = self.count / float(np.prod(self.shape))
+        self._perm_prev = np.full(3, np.nan)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _capi.lib().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, psi_field: np.ndarray, u0: Sequence[float] = (0.0, 0.0, 0.0)):
+        psi = np.ascontiguousarray(psi_field, np.float64)
+        if psi.shape != self.shape:
+            raise ValueError("phase field must have the box shape")
+        u = np.ascontiguousarray(u0, np.float64)
+        _capi.check(_capi.lib().lbm_cg_init(self._h, _dp(psi), _dp(u)), self._h)
+
+    def load_state(self, f, u, rho, psi, rho_r, rho_b):
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (f, u, rho, psi, rho_r, rho_b)]
+        _capi.check(_capi.lib().lbm_cg_load_state(self._h, *[_dp(a) for a in arrs]), self._h)
+
+    def step(self, nsteps: int = 1):
+        _capi.check(_capi.lib().lbm_cg_step(self._h, int(nsteps)), self._h)
+
+    def step_timed(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _capi.check(_capi.lib().lbm_cg_step_timed(self._h, int(nsteps), ctypes.byref(ms)), self._h)
+        return ms.value
+
+    def set_body_force(self, g: Sequence[float]):
+        gg = np.ascontiguousarray(g, np.float64)
+        _capi.check(_capi.lib().lbm_cg_set_body_force(self._h, _dp(gg)), self._h)
+        self.g = tuple(float(x) for x in gg)
+
+    @property
+    def steps_done(self) -> int:
+        s = ctypes.c_int64()
+        _capi.check(_capi.lib().lbm_cg_steps_done(self._h, ctypes.byref(s)), self._h)
+        return int(s.value)
+
+    def state(self):
+        """(f[count,19] CG ordering, u[count,3], rho, psi, rho_r, rho_b)."""
+        f = np.empty((self.count, 19))
+        u = np.empty((self.count, 3))
+        rho, psi, rr, rb = (np.empty(self.count) for _ in range(4))
+        _capi.check(_capi.lib().lbm_cg_get_state(self._h, _dp(f), _dp(u), _dp(rho), _dp(psi), _dp(rr), _dp(rb)),
+                    self._h)
+        return f, u, rho, psi, rr, rb
+
+    def diagnostics(self, box: Optional[Sequence[int]] = None) -> dict:
+        d = _capi.CgDiag()
+        b = None if box is None else (ctypes.c_int32 * 6)(*[int(x) for x in box])
+        _capi.check(_capi.lib().lbm_cg_diagnostics(self._h, b, ctypes.byref(d)), self._h)
+        return {"Ql": np.array(d.Ql[:]), "Qg": np.array(d.Qg[:]), "err": d.err, "umax": d.umax,
+                "sum_speed": d.sum_speed, "nan_count": d.nan_count, "wet_count": d.wet_count,
+                "count_in_box": d.count_in_box}
+
+    def box_volume(self, box=None) -> int:
+        if box is None:
+            return int(np.prod(self.shape))
+        return (box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1)
+
+    def Comput_Perm(self, PerDir: int = 1, box=None, dp: float = 0.0):
+        """(k_l[3], k_g[3], relative change of k_l[PerDir-1]) as CG's Comput_Perm (CG:6298-6480)."""
+        d = self.diagnostics(box)
+        V = self.box_volume(box)
+        g = np.asarray(self.g)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            kl = d["Ql"] / V * self.nu_l / (g + dp)
+            kg = d["Qg"] / V * self.nu_g / (g + dp)
+            i = {1: 0, 2: 1, 3: 2}.get(PerDir, 0)
+            rel = (kl[i] - self._perm_prev[i]) / self._perm_prev[i]
+        self._perm_prev = kl.copy()
+        return kl, kg, rel
+
+    def Comput_Saturation(self, box=None) -> float:
+        """S_l = #(psi >= 0) / (V * porosity) over the box (CG:6604-6763)."""
+        d = self.diagnostics(box)
+        return d["wet_count"] / d["count_in_box"]
