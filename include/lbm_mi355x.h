@@ -140,6 +140,67 @@ int lbm_cg_get_state(lbm_handle *h, double *f, double *u, double *rho, double *p
 int lbm_cg_diagnostics(lbm_handle *h, const int32_t *box, lbm_cg_diag *out);
 int lbm_cg_steps_done(const lbm_handle *h, int64_t *steps);
 
+/* ------------------------------------------------------------------ permeability serving
+ * A persistent request loop for many independent single-phase permeability solves -- the
+ * GPU counterpart of the reference's practice of cutting a rock into sub-samples and running
+ * one solver per sub-sample INPUT file (CPP_SCRIPT/HPC_SUBDOMAIN_INPUT*.cpp).  Requests of
+ * uneven sizes are admitted into one device arena by HBM footprint, stepped together (one
+ * launch per time step over the tiles of every active sample, per-sample parameters on the
+ * device), checked every check_interval steps with per-sample FP64 reductions, and retired
+ * when |k_n - k_(n-interval)| <= tol |k_n|.  A sample's result does not depend on which other
+ * samples share its launches.  Calls are thread-safe; the loop runs on its own host thread. */
+typedef struct lbm_srv_params {
+    int64_t capacity_nodes;   /* arena size in fluid nodes; 0 = mem_fraction of free device memory */
+    double mem_fraction;      /* used when capacity_nodes == 0 (default 0.5) */
+    int32_t check_interval;   /* steps between permeability checks (freRe), default 100 */
+    double tol;               /* relative permeability change that retires a sample, default 1e-6 */
+    int32_t min_steps;        /* never retire before this many steps (default 2*check_interval) */
+    int32_t max_steps;        /* retire unconverged after this many steps (default 200000) */
+    int32_t precision;        /* LBM_FP32 (default) or LBM_FP64 */
+    int32_t block;            /* nodes per workgroup tile: 128, 256 or 512; 0 = autotune at create */
+    int32_t device;
+} lbm_srv_params;
+
+typedef struct lbm_srv_request {
+    int32_t nx, ny, nz;
+    const int32_t *solid_ijk; /* [i][j][k], 0 = pore; copied during lbm_srv_submit */
+    double nu;                /* kinematic viscosity */
+    double g[3];              /* body force */
+    int32_t perm_dir;         /* PerDir: 1 = x, 2 = y, 3 = z; the direction whose k is tested */
+} lbm_srv_request;
+
+typedef struct lbm_srv_result {
+    int64_t ticket;
+    double k[3];              /* permeability in lattice units, k_d = Q_d/V nu/g_d (Comput_Perm) */
+    double rel_change;        /* last relative change of k[perm_dir-1] */
+    int64_t steps;
+    int64_t fluid_nodes;
+    int32_t converged;        /* 1 converged, 0 stopped at max_steps, -1 non-finite velocity */
+    int32_t pad;
+    double t_submit, t_admit, t_done;   /* seconds on the server clock */
+} lbm_srv_result;
+
+typedef struct lbm_srv_stats {
+    int64_t pending, active, done;
+    int64_t arena_nodes, arena_used;
+    int64_t launches;        /* step launches issued */
+    int64_t node_updates;    /* fluid-node updates performed */
+    int32_t block;           /* tile size in use */
+    int32_t pad;
+    double busy_seconds;     /* device time of the step launches (HIP events) */
+} lbm_srv_stats;
+
+int lbm_srv_create(const lbm_srv_params *p, lbm_handle **out);
+/* queue a request (geometry numbering and neighbour lists are built in the caller's thread) */
+int lbm_srv_submit(lbm_handle *h, const lbm_srv_request *req, int64_t *ticket);
+/* take up to max finished results */
+int lbm_srv_poll(lbm_handle *h, lbm_srv_result *out, int32_t max, int32_t *n_out);
+/* block until nothing is pending or active (timeout_s <= 0: no limit); LBM_ERR_BUSY on timeout */
+int lbm_srv_wait_idle(lbm_handle *h, double timeout_s);
+int lbm_srv_get_stats(lbm_handle *h, lbm_srv_stats *out);
+/* seconds on the server clock (for latency accounting by callers) */
+double lbm_srv_clock(const lbm_handle *h);
+
 /* ------------------------------------------------------------------ common */
 const char *lbm_last_error(const lbm_handle *h);   /* NULL handle: last create() failure */
 void lbm_destroy(lbm_handle *h);

@@ -47,6 +47,33 @@ class CgDiag(ctypes.Structure):
                 ("wet_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
 
 
+class SrvParams(ctypes.Structure):
+    _fields_ = [("capacity_nodes", ctypes.c_int64), ("mem_fraction", ctypes.c_double),
+                ("check_interval", ctypes.c_int32), ("tol", ctypes.c_double), ("min_steps", ctypes.c_int32),
+                ("max_steps", ctypes.c_int32), ("precision", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+class SrvRequest(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("solid_ijk", ctypes.POINTER(ctypes.c_int32)), ("nu", ctypes.c_double), ("g", ctypes.c_double * 3),
+                ("perm_dir", ctypes.c_int32)]
+
+
+class SrvResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_int64), ("k", ctypes.c_double * 3), ("rel_change", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("fluid_nodes", ctypes.c_int64), ("converged", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
+class SrvStats(ctypes.Structure):
+    _fields_ = [("pending", ctypes.c_int64), ("active", ctypes.c_int64), ("done", ctypes.c_int64),
+                ("arena_nodes", ctypes.c_int64), ("arena_used", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("node_updates", ctypes.c_int64), ("block", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("busy_seconds", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -97,6 +124,13 @@ def lib():
         L.lbm_cg_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.lbm_cg_diagnostics.argtypes = [vp, ip, ctypes.POINTER(CgDiag)]
         L.lbm_cg_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_srv_create.argtypes = [ctypes.POINTER(SrvParams), ctypes.POINTER(vp)]
+        L.lbm_srv_submit.argtypes = [vp, ctypes.POINTER(SrvRequest), ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_srv_poll.argtypes = [vp, ctypes.POINTER(SrvResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.lbm_srv_wait_idle.argtypes = [vp, ctypes.c_double]
+        L.lbm_srv_get_stats.argtypes = [vp, ctypes.POINTER(SrvStats)]
+        L.lbm_srv_clock.argtypes = [vp]
+        L.lbm_srv_clock.restype = ctypes.c_double
         _lib = L
     return _lib
 

@@ -19,6 +19,8 @@ const lbm::CgSolver *as_cg(const lbm_handle *h)
 }
 }  // namespace
 
+void lbm::set_create_error(const std::string &msg) { g_create_error = msg; }
+
 extern "C" {
 
 const char *lbm_version(void) { return "lbm-mi355x 0.1 (gfx950, sparse D3Q19 MRT)"; }

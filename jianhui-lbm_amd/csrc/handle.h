@@ -23,6 +23,9 @@ struct lbm_handle {
 
 namespace lbm {
 
+// message returned by lbm_last_error(NULL) after a failed create
+void set_create_error(const std::string &msg);
+
 // Sparse fluid-node structure on the host: the reference's numbering (SP:1172-1193) and, per
 // node and moving direction q = 1..18, the upstream node x - e_q in pull form (BB_FLAG |
 // self when x - e_q is solid).  Stored direction-major with row stride P.

@@ -1,0 +1,751 @@
+// server.hip -- continuous-batching permeability server (many independent single-phase solves).
+//
+// The reference answers "permeability of many sub-samples" by writing one INPUT file per
+// sub-sample and running one MPI solver per file (CPP_SCRIPT/HPC_SUBDOMAIN_INPUT*.cpp).  Here
+// one process keeps a device arena and a request loop:
+//
+//   submit()   (caller's thread)  number the sample's pore voxels and build its pull neighbour
+//                                 lists with sample-local indices; queue it.
+//   loop thread                   admit queued samples whose node count fits a free arena
+//                                 segment (first fit, 256-node aligned); run check_interval
+//                                 steps, each ONE launch over the tiles of every active sample;
+//                                 then per-sample FP64 flux reductions; retire converged
+//                                 samples, free their segments, admit more.
+//
+// Arena layout (row stride CAP nodes, a sample owns rows [base, base + npad)):
+//   pops[2]  19 x CAP  populations (deviation form, FP32 or FP64)
+//   nbr      18 x CAP  upstream node, sample-local index, BB_FLAG = bounce back
+// Device tables: slots[] (per-sample parameters), tiles[] (slot, node range of one workgroup).
+// Every sample is cut into tiles from its own first node, and its flux is summed over its own
+// tiles in tile order, so its result is bitwise the same whatever else shares the launches.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "collide.h"
+#include "common.h"
+#include "handle.h"
+#include "pull.h"
+#include "storage.h"
+
+namespace lbm {
+
+struct SlotDev {
+    double gx, gy, gz, sv, sq, rho0;
+    int64_t first_step;   // global step of the sample's first (equilibrium) collision
+};
+
+struct TileDev {
+    int32_t slot;
+    int32_t pad;
+    int64_t begin, end;   // arena node range handled by one workgroup
+    int64_t base;         // the sample's first arena node (local index origin)
+};
+
+template <typename ST, int BLK>
+__global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
+                                                       const ST *__restrict__ src, ST *__restrict__ dst,
+                                                       const uint32_t *__restrict__ nbr, int64_t CAP, int64_t step)
+{
+    using C = typename Store<ST>::C;
+    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const int64_t n = t.begin + threadIdx.x;
+    if (n >= t.end) return;
+    const SlotDev sl = slots[t.slot];
+    const C gx = (C)sl.gx, gy = (C)sl.gy, gz = (C)sl.gz, rho0 = (C)sl.rho0;
+    C h[Q], m[Q];
+    Macro<C> mc;
+    if (step == sl.first_step) {   // f = feq(1, 0): the reference's init (SP:1343-1376)
+        mc.rho = C(1);
+        mc.drho = C(1) - rho0;
+        mc.ux = mc.uy = mc.uz = C(0);
+        sp_eq_moments(mc, m);
+    } else {
+        h[0] = Store<ST>::load(src[n], 1.0f);
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            const uint32_t v = nbr[(int64_t)(q - 1) * CAP + n];
+            const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+            h[q] = Store<ST>::load(src[row * CAP + t.base + (int64_t)(v & ~BB_FLAG)], 1.0f);
+        }
+        moments_fwd(h, m);
+        mc = macro_from_moments(m, rho0, gx, gy, gz);
+    }
+    sp_relax(m, mc, gx, gy, gz, (C)sl.sv, (C)sl.sq, h);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) dst[(int64_t)q * CAP + n] = Store<ST>::put(h[q], 1.0f);
+}
+
+// per-tile flux partials Q[3] and a non-finite flag, from the current (pulled) state
+template <typename ST, int BLK>
+__global__ void __launch_bounds__(BLK) srv_flux_kernel(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
+                                                       const ST *__restrict__ src, const uint32_t *__restrict__ nbr,
+                                                       int64_t CAP, double *__restrict__ partial)
+{
+    using C = typename Store<ST>::C;
+    const TileDev t = tiles[blockIdx.x];
+    const int64_t n = t.begin + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (n < t.end) {
+        const SlotDev sl = slots[t.slot];
+        C h[Q], m[Q];
+        h[0] = Store<ST>::load(src[n], 1.0f);
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            const uint32_t w = nbr[(int64_t)(q - 1) * CAP + n];
+            const int64_t row = (w & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+            h[q] = Store<ST>::load(src[row * CAP + t.base + (int64_t)(w & ~BB_FLAG)], 1.0f);
+        }
+        moments_fwd(h, m);
+        const Macro<C> mc = macro_from_moments(m, (C)sl.rho0, (C)sl.gx, (C)sl.gy, (C)sl.gz);
+        v[0] = (double)mc.ux;
+        v[1] = (double)mc.uy;
+        v[2] = (double)mc.uz;
+        v[3] = (isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2])) ? 0.0 : 1.0;
+    }
+    // block reduction in a fixed tree (BLK/64 waves)
+    __shared__ double red[BLK / WAVE][4];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double r = wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double r = red[0][threadIdx.x];
+        for (int w = 1; w < BLK / WAVE; ++w) r += red[w][threadIdx.x];
+        partial[(int64_t)blockIdx.x * 4 + threadIdx.x] = r;
+    }
+}
+
+// one workgroup per active sample: sum its tiles' partials in tile order
+__global__ void __launch_bounds__(BLOCK) srv_slot_sum_kernel(const double *__restrict__ partial,
+                                                             const int32_t *__restrict__ tile_lo,
+                                                             const int32_t *__restrict__ tile_hi, double *__restrict__ out)
+{
+    const int a = blockIdx.x;
+    const int lo = tile_lo[a], hi = tile_hi[a];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = lo + (int)threadIdx.x; t < hi; t += BLOCK)
+        for (int k = 0; k < 4; ++k) v[k] += partial[(int64_t)t * 4 + k];
+    const bool is_max[4] = {false, false, false, false};
+    block_reduce_store<4>(v, is_max, out + (int64_t)a * 4);
+}
+
+// ---------------------------------------------------------------------------- host
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct Prepared {
+    int64_t ticket = 0;
+    int64_t N = 0, npad = 0;
+    std::vector<uint32_t> nbr;   // 18 x npad, local indices
+    double nu = 0, g[3] = {0, 0, 0};
+    int dir = 1;
+    int64_t volume = 0;
+    double t_submit = 0;
+};
+
+struct Active {
+    Prepared req;
+    int64_t base = 0;
+    int64_t first_step = 0;
+    int64_t steps = 0;
+    double k_prev = 0;
+    bool has_prev = false;
+    double t_admit = 0;
+};
+
+struct Segment {
+    int64_t start, len;
+};
+}  // namespace
+
+class Server : public lbm_handle {
+public:
+    Server() : lbm_handle(SERVER) {}
+    ~Server() override { shutdown(); }
+
+    int create(const lbm_srv_params &p);
+    int submit(const lbm_srv_request &r, int64_t *ticket);
+    int poll(lbm_srv_result *out, int max, int *n_out);
+    int wait_idle(double timeout_s);
+    int stats(lbm_srv_stats *out);
+    double clock() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
+
+private:
+    void shutdown();
+    void loop();
+    int run_chunk(std::unique_lock<std::mutex> &lk);
+    bool try_admit_locked();
+    int rebuild_tables();
+    int64_t alloc_segment(int64_t len);
+    void free_segment(int64_t start, int64_t len);
+    template <int BLK>
+    hipError_t launch_steps(int nsteps);
+    hipError_t launch_steps_dispatch(int nsteps);
+    template <int BLK>
+    hipError_t launch_flux();
+    int autotune();
+
+    lbm_srv_params prm_{};
+    Clock::time_point t0_ = Clock::now();
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    size_t st_ = 4;
+    int64_t cap_ = 0;
+    int blk_ = 256;
+    void *pops_[2] = {nullptr, nullptr};
+    int cur_ = 0;
+    uint32_t *d_nbr_ = nullptr;
+    SlotDev *d_slots_ = nullptr;
+    TileDev *d_tiles_ = nullptr;
+    int32_t *d_tlo_ = nullptr, *d_thi_ = nullptr;
+    double *d_partial_ = nullptr, *d_sums_ = nullptr, *h_sums_ = nullptr;
+    int64_t max_slots_ = 0, max_tiles_ = 0;
+    int64_t ntiles_ = 0;
+    int64_t global_step_ = 0;
+
+    std::mutex mu_;
+    std::condition_variable cv_, idle_cv_;
+    std::thread thr_;
+    bool stop_ = false;
+    bool loop_error_ = false;
+    std::string loop_msg_;
+    int64_t next_ticket_ = 1;
+    std::deque<Prepared> pending_;
+    std::vector<Active> active_;   // in tile order
+    std::vector<Segment> free_;
+    std::deque<lbm_srv_result> done_;
+    int64_t n_done_ = 0, launches_ = 0, node_updates_ = 0;
+    double busy_s_ = 0;
+};
+
+int64_t Server::alloc_segment(int64_t len)
+{
+    for (size_t i = 0; i < free_.size(); ++i) {
+        if (free_[i].len >= len) {
+            const int64_t s = free_[i].start;
+            free_[i].start += len;
+            free_[i].len -= len;
+            if (free_[i].len == 0) free_.erase(free_.begin() + (long)i);
+            return s;
+        }
+    }
+    return -1;
+}
+
+void Server::free_segment(int64_t start, int64_t len)
+{
+    free_.push_back({start, len});
+    std::sort(free_.begin(), free_.end(), [](const Segment &a, const Segment &b) { return a.start < b.start; });
+    std::vector<Segment> merged;
+    for (const Segment &s : free_) {
+        if (!merged.empty() && merged.back().start + merged.back().len == s.start)
+            merged.back().len += s.len;
+        else
+            merged.push_back(s);
+    }
+    free_.swap(merged);
+}
+
+int Server::create(const lbm_srv_params &p)
+{
+    prm_ = p;
+    if (prm_.check_interval <= 0) prm_.check_interval = 100;
+    if (!(prm_.tol > 0)) prm_.tol = 1e-6;
+    if (prm_.min_steps <= 0) prm_.min_steps = 2 * prm_.check_interval;
+    if (prm_.max_steps <= 0) prm_.max_steps = 200000;
+    if (prm_.precision != LBM_FP64) prm_.precision = LBM_FP32;
+    if (prm_.block != 0 && prm_.block != 128 && prm_.block != 256 && prm_.block != 512)
+        return fail(LBM_ERR_ARG, "block must be 0, 128, 256 or 512");
+    st_ = prm_.precision == LBM_FP64 ? 8 : 4;
+    device_ = p.device;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    LBM_HIP_TRY(hipEventCreate(&ev0_));
+    LBM_HIP_TRY(hipEventCreate(&ev1_));
+    int64_t cap = p.capacity_nodes;
+    if (cap <= 0) {
+        size_t free_b = 0, total_b = 0;
+        LBM_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        const double frac = p.mem_fraction > 0 && p.mem_fraction < 0.95 ? p.mem_fraction : 0.5;
+        const double bytes = std::min(frac * (double)free_b, 64e9);
+        cap = (int64_t)(bytes / (double)(2 * Q * st_ + 18 * 4));
+    }
+    cap = std::min<int64_t>(cap, (int64_t)0x7fffffff);
+    cap_ = ((cap + 511) / 512) * 512;
+    if (cap_ < 512) return fail(LBM_ERR_ARG, "arena too small");
+    for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * cap_ * st_));
+    LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * cap_ * sizeof(uint32_t)));
+    max_slots_ = 1 << 16;
+    max_tiles_ = cap_ / 128 + 1;
+    LBM_HIP_TRY(hipMalloc(&d_slots_, (size_t)max_slots_ * sizeof(SlotDev)));
+    LBM_HIP_TRY(hipMalloc(&d_tiles_, (size_t)max_tiles_ * sizeof(TileDev)));
+    LBM_HIP_TRY(hipMalloc(&d_tlo_, (size_t)max_slots_ * sizeof(int32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_thi_, (size_t)max_slots_ * sizeof(int32_t)));
+    LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)max_tiles_ * 4 * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&d_sums_, (size_t)max_slots_ * 4 * sizeof(double)));
+    LBM_HIP_TRY(hipHostMalloc((void **)&h_sums_, (size_t)max_slots_ * 4 * sizeof(double), hipHostMallocDefault));
+    free_.push_back({0, cap_});
+    blk_ = prm_.block ? prm_.block : 256;
+    if (prm_.block == 0) {
+        int rc = autotune();
+        if (rc != LBM_OK) return rc;
+    }
+    thr_ = std::thread([this] { loop(); });
+    return LBM_OK;
+}
+
+void Server::shutdown()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (thr_.joinable()) thr_.join();
+    (void)hipSetDevice(device_);
+    for (void *&b : pops_)
+        if (b) (void)hipFree(b), b = nullptr;
+    void **bufs[] = {(void **)&d_nbr_, (void **)&d_slots_, (void **)&d_tiles_, (void **)&d_tlo_, (void **)&d_thi_,
+                     (void **)&d_partial_, (void **)&d_sums_};
+    for (void **b : bufs)
+        if (*b) (void)hipFree(*b), *b = nullptr;
+    if (h_sums_) (void)hipHostFree(h_sums_), h_sums_ = nullptr;
+    if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
+    if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
+    if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
+}
+
+int Server::submit(const lbm_srv_request &r, int64_t *ticket)
+{
+    if (!r.solid_ijk || r.nx <= 0 || r.ny <= 0 || r.nz <= 0) return fail(LBM_ERR_ARG, "invalid request geometry");
+    if (!(r.nu > 0)) return fail(LBM_ERR_ARG, "viscosity must be positive");
+    if (r.perm_dir < 1 || r.perm_dir > 3) return fail(LBM_ERR_ARG, "perm_dir must be 1, 2 or 3");
+    if (r.g[r.perm_dir - 1] == 0.0) return fail(LBM_ERR_ARG, "body force along perm_dir is zero");
+    SparseGeometry geo;
+    std::string err;
+    int rc = build_sparse_geometry(r.nx, r.ny, r.nz, r.solid_ijk, geo, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    if (geo.P > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
+    Prepared pr;
+    pr.N = geo.N;
+    pr.npad = geo.P;
+    pr.nbr.swap(geo.nbr);
+    pr.nu = r.nu;
+    for (int c = 0; c < 3; ++c) pr.g[c] = r.g[c];
+    pr.dir = r.perm_dir;
+    pr.volume = (int64_t)r.nx * r.ny * r.nz;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        pr.ticket = next_ticket_++;
+        pr.t_submit = clock();
+        if (ticket) *ticket = pr.ticket;
+        pending_.push_back(std::move(pr));
+    }
+    cv_.notify_all();
+    return LBM_OK;
+}
+
+int Server::poll(lbm_srv_result *out, int max, int *n_out)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    int n = 0;
+    while (n < max && !done_.empty()) {
+        out[n++] = done_.front();
+        done_.pop_front();
+    }
+    if (n_out) *n_out = n;
+    if (loop_error_) return fail(LBM_ERR_HIP, loop_msg_);
+    return LBM_OK;
+}
+
+int Server::wait_idle(double timeout_s)
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    auto idle = [this] { return loop_error_ || (pending_.empty() && active_.empty()); };
+    if (timeout_s > 0) {
+        if (!idle_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), idle)) return LBM_ERR_BUSY;
+    } else {
+        idle_cv_.wait(lk, idle);
+    }
+    if (loop_error_) return fail(LBM_ERR_HIP, loop_msg_);
+    return LBM_OK;
+}
+
+int Server::stats(lbm_srv_stats *out)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    out->pending = (int64_t)pending_.size();
+    out->active = (int64_t)active_.size();
+    out->done = n_done_;
+    out->arena_nodes = cap_;
+    int64_t used = 0;
+    for (const Active &a : active_) used += a.req.npad;
+    out->arena_used = used;
+    out->launches = launches_;
+    out->node_updates = node_updates_;
+    out->block = blk_;
+    out->pad = 0;
+    out->busy_seconds = busy_s_;
+    return LBM_OK;
+}
+
+// Admit queued samples in arrival order while they fit; a sample that does not fit waits and
+// later, smaller ones may pass it (admission by HBM footprint).
+bool Server::try_admit_locked()
+{
+    bool any = false;
+    for (auto it = pending_.begin(); it != pending_.end();) {
+        if ((int64_t)active_.size() >= max_slots_) break;
+        const int64_t base = alloc_segment(it->npad);
+        if (base < 0) {
+            ++it;
+            continue;
+        }
+        Active a;
+        a.req = std::move(*it);
+        a.base = base;
+        a.first_step = global_step_;
+        a.t_admit = clock();
+        // sample-local neighbour rows go straight into the arena rows of the segment
+        const hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(uint32_t), a.req.nbr.data(),
+                                              (size_t)a.req.npad * sizeof(uint32_t), (size_t)a.req.npad * sizeof(uint32_t),
+                                              18, hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) {
+            loop_error_ = true;
+            loop_msg_ = std::string("neighbour upload failed: ") + hipGetErrorString(e);
+            free_segment(base, a.req.npad);
+            return any;
+        }
+        active_.push_back(std::move(a));
+        it = pending_.erase(it);
+        any = true;
+    }
+    if (any) {
+        // the uploads read pageable host memory: finish them before the host copies go away
+        const hipError_t e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) {
+            loop_error_ = true;
+            loop_msg_ = std::string("neighbour upload failed: ") + hipGetErrorString(e);
+        }
+        for (Active &a : active_) std::vector<uint32_t>().swap(a.req.nbr);
+    }
+    return any;
+}
+
+int Server::rebuild_tables()
+{
+    // order active samples by arena position (keeps neighbouring tiles adjacent in memory)
+    std::sort(active_.begin(), active_.end(), [](const Active &a, const Active &b) { return a.base < b.base; });
+    std::vector<SlotDev> slots(active_.size());
+    std::vector<TileDev> tiles;
+    std::vector<int32_t> tlo(active_.size()), thi(active_.size());
+    const double c_s2 = 1.0 / 3.0;
+    for (size_t s = 0; s < active_.size(); ++s) {
+        const Active &a = active_[s];
+        SlotDev d{};
+        d.gx = a.req.g[0];
+        d.gy = a.req.g[1];
+        d.gz = a.req.g[2];
+        const double tau = a.req.nu / (c_s2 * 1.0) + 0.5;   // SP:1302-1307
+        d.sv = 1.0 / tau;
+        d.sq = 8.0 * (2.0 - d.sv) / (8.0 - d.sv);
+        d.rho0 = 1.0;
+        d.first_step = a.first_step;
+        slots[s] = d;
+        tlo[s] = (int32_t)tiles.size();
+        for (int64_t b = 0; b < a.req.N; b += blk_) {
+            TileDev t{};
+            t.slot = (int32_t)s;
+            t.begin = a.base + b;
+            t.end = a.base + std::min<int64_t>(b + blk_, a.req.N);
+            t.base = a.base;
+            tiles.push_back(t);
+        }
+        thi[s] = (int32_t)tiles.size();
+    }
+    ntiles_ = (int64_t)tiles.size();
+    if (ntiles_ > max_tiles_) return fail(LBM_ERR_ARG, "tile table overflow");
+    if (!slots.empty()) {
+        LBM_HIP_TRY(hipMemcpyAsync(d_slots_, slots.data(), slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipMemcpyAsync(d_tiles_, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipMemcpyAsync(d_tlo_, tlo.data(), tlo.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipMemcpyAsync(d_thi_, thi.data(), thi.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+        // pageable sources: make sure the copies have consumed them before the vectors go away
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    }
+    return LBM_OK;
+}
+
+template <int BLK>
+hipError_t Server::launch_steps(int nsteps)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        if (prm_.precision == LBM_FP64)
+            srv_step_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_slots_, (const double *)pops_[cur_], (double *)pops_[cur_ ^ 1], d_nbr_, cap_, global_step_);
+        else
+            srv_step_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_slots_, (const float *)pops_[cur_], (float *)pops_[cur_ ^ 1], d_nbr_, cap_, global_step_);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        cur_ ^= 1;
+        ++global_step_;
+    }
+    return hipSuccess;
+}
+
+hipError_t Server::launch_steps_dispatch(int nsteps)
+{
+    switch (blk_) {
+    case 128: return launch_steps<128>(nsteps);
+    case 512: return launch_steps<512>(nsteps);
+    default: return launch_steps<256>(nsteps);
+    }
+}
+
+template <int BLK>
+hipError_t Server::launch_flux()
+{
+    if (prm_.precision == LBM_FP64)
+        srv_flux_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(d_tiles_, d_slots_, (const double *)pops_[cur_],
+                                                                             d_nbr_, cap_, d_partial_);
+    else
+        srv_flux_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(d_tiles_, d_slots_, (const float *)pops_[cur_],
+                                                                            d_nbr_, cap_, d_partial_);
+    return hipGetLastError();
+}
+
+// Chunk of check_interval steps for every active sample, then the permeability check.
+// Called with the lock held; the lock is released while the device works.
+int Server::run_chunk(std::unique_lock<std::mutex> &lk)
+{
+    const int nsteps = prm_.check_interval;
+    const int64_t nodes = [&] {
+        int64_t s = 0;
+        for (const Active &a : active_) s += a.req.N;
+        return s;
+    }();
+    const int nact = (int)active_.size();
+    lk.unlock();
+    hipError_t e = hipEventRecord(ev0_, stream_);
+    if (e == hipSuccess) e = launch_steps_dispatch(nsteps);
+    if (e == hipSuccess) e = hipEventRecord(ev1_, stream_);
+    if (e == hipSuccess) e = blk_ == 128 ? launch_flux<128>() : (blk_ == 512 ? launch_flux<512>() : launch_flux<256>());
+    if (e == hipSuccess) {
+        srv_slot_sum_kernel<<<(unsigned)nact, BLOCK, 0, stream_>>>(d_partial_, d_tlo_, d_thi_, d_sums_);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h_sums_, d_sums_, (size_t)nact * 4 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0_, ev1_);
+    lk.lock();
+    if (e != hipSuccess) {
+        loop_error_ = true;
+        loop_msg_ = std::string("server chunk failed: ") + hipGetErrorString(e);
+        return LBM_ERR_HIP;
+    }
+    launches_ += nsteps;
+    node_updates_ += nodes * nsteps;
+    busy_s_ += ms * 1e-3;
+    // retire converged samples (Comput_Perm k = Q/V nu/g, SP:4675-4677; relative change test)
+    bool changed = false;
+    std::vector<Active> keep;
+    keep.reserve(active_.size());
+    for (int s = 0; s < nact; ++s) {
+        Active &a = active_[s];
+        a.steps += nsteps;
+        const double *q = h_sums_ + 4 * s;
+        double k[3];
+        for (int c = 0; c < 3; ++c) k[c] = a.req.g[c] != 0.0 ? q[c] / (double)a.req.volume * a.req.nu / a.req.g[c] : 0.0;
+        const double kd = k[a.req.dir - 1];
+        const double rel = a.has_prev ? std::fabs(kd - a.k_prev) / std::max(std::fabs(kd), 1e-300) : INFINITY;
+        int status = 2;   // keep running
+        if (q[3] != 0.0 || !std::isfinite(kd))
+            status = -1;
+        else if (a.steps >= prm_.min_steps && a.has_prev && rel <= prm_.tol)
+            status = 1;
+        else if (a.steps >= prm_.max_steps)
+            status = 0;
+        a.k_prev = kd;
+        a.has_prev = true;
+        if (status == 2) {
+            keep.push_back(std::move(a));
+            continue;
+        }
+        lbm_srv_result r{};
+        r.ticket = a.req.ticket;
+        for (int c = 0; c < 3; ++c) r.k[c] = k[c];
+        r.rel_change = rel;
+        r.steps = a.steps;
+        r.fluid_nodes = a.req.N;
+        r.converged = status;
+        r.t_submit = a.req.t_submit;
+        r.t_admit = a.t_admit;
+        r.t_done = clock();
+        done_.push_back(r);
+        ++n_done_;
+        free_segment(a.base, a.req.npad);
+        changed = true;
+    }
+    active_.swap(keep);
+    return changed ? 1 : 0;
+}
+
+void Server::loop()
+{
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> lk(mu_);
+    bool tables_dirty = true;
+    while (!stop_) {
+        if (!pending_.empty() && try_admit_locked()) tables_dirty = true;
+        if (loop_error_) break;
+        if (active_.empty()) {
+            idle_cv_.notify_all();
+            cv_.wait(lk, [this] { return stop_ || !pending_.empty(); });
+            continue;
+        }
+        if (tables_dirty) {
+            if (rebuild_tables() != LBM_OK) {
+                loop_error_ = true;
+                loop_msg_ = last_error;
+                break;
+            }
+            tables_dirty = false;
+        }
+        const int rc = run_chunk(lk);
+        if (rc < 0) break;
+        if (rc > 0) tables_dirty = true;
+        if (pending_.empty() && active_.empty()) idle_cv_.notify_all();
+    }
+    idle_cv_.notify_all();
+}
+
+// Pick the tile size by timing the step kernel on a synthetic porous sample.
+int Server::autotune()
+{
+    // a 96^3 random porous sample (hashed blobs, porosity ~0.35)
+    const int L = 96;
+    std::vector<int32_t> solid((size_t)L * L * L);
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j)
+            for (int k = 0; k < L; ++k) {
+                const double v = std::sin(0.37 * i + 1.3) * std::sin(0.29 * j + 0.7) * std::sin(0.41 * k + 2.1) +
+                                 0.5 * std::sin(0.13 * (i + 2 * j) + 0.4) * std::cos(0.17 * (k - j));
+                solid[((size_t)i * L + j) * L + k] = v > -0.05 ? 1 : 0;
+            }
+    SparseGeometry geo;
+    std::string err;
+    if (build_sparse_geometry(L, L, L, solid.data(), geo, err) != LBM_OK || geo.P > cap_) {
+        blk_ = 256;
+        return LBM_OK;
+    }
+    Active a;
+    a.req.N = geo.N;
+    a.req.npad = geo.P;
+    a.req.nu = 1.0 / 6.0;
+    a.req.g[0] = 1e-6;
+    a.req.volume = (int64_t)L * L * L;
+    a.base = 0;
+    a.first_step = 0;
+    LBM_HIP_TRY(hipMemcpy2DAsync(d_nbr_, (size_t)cap_ * sizeof(uint32_t), geo.nbr.data(), (size_t)geo.P * sizeof(uint32_t),
+                                 (size_t)geo.P * sizeof(uint32_t), 18, hipMemcpyHostToDevice, stream_));
+    active_.push_back(std::move(a));
+    float best = 1e30f;
+    int best_blk = 256;
+    for (int blk : {128, 256, 512}) {
+        blk_ = blk;
+        global_step_ = 0;
+        if (rebuild_tables() != LBM_OK) break;
+        LBM_HIP_TRY(launch_steps_dispatch(5));   // warm-up (first step is the equilibrium init)
+        LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
+        LBM_HIP_TRY(launch_steps_dispatch(40));
+        LBM_HIP_TRY(hipEventRecord(ev1_, stream_));
+        LBM_HIP_TRY(hipEventSynchronize(ev1_));
+        float ms = 0.f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+        if (ms < best) {
+            best = ms;
+            best_blk = blk;
+        }
+    }
+    active_.clear();
+    global_step_ = 0;
+    blk_ = best_blk;
+    return LBM_OK;
+}
+
+}  // namespace lbm
+
+// ---------------------------------------------------------------------------- C ABI
+namespace {
+lbm::Server *as_srv(lbm_handle *h) { return (h && h->kind == lbm_handle::SERVER) ? static_cast<lbm::Server *>(h) : nullptr; }
+}  // namespace
+
+extern "C" {
+
+int lbm_srv_create(const lbm_srv_params *p, lbm_handle **out)
+{
+    if (!p || !out) return LBM_ERR_ARG;
+    *out = nullptr;
+    auto *s = new (std::nothrow) lbm::Server();
+    if (!s) return LBM_ERR_NOMEM;
+    const int rc = s->create(*p);
+    if (rc != LBM_OK) {
+        lbm::set_create_error(s->last_error);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_srv_submit(lbm_handle *h, const lbm_srv_request *req, int64_t *ticket)
+{
+    auto *s = as_srv(h);
+    if (!s || !req) return LBM_ERR_ARG;
+    return s->submit(*req, ticket);
+}
+
+int lbm_srv_poll(lbm_handle *h, lbm_srv_result *out, int32_t max, int32_t *n_out)
+{
+    auto *s = as_srv(h);
+    if (!s || (!out && max > 0)) return LBM_ERR_ARG;
+    int n = 0;
+    const int rc = s->poll(out, max, &n);
+    if (n_out) *n_out = n;
+    return rc;
+}
+
+int lbm_srv_wait_idle(lbm_handle *h, double timeout_s)
+{
+    auto *s = as_srv(h);
+    return s ? s->wait_idle(timeout_s) : LBM_ERR_ARG;
+}
+
+int lbm_srv_get_stats(lbm_handle *h, lbm_srv_stats *out)
+{
+    auto *s = as_srv(h);
+    if (!s || !out) return LBM_ERR_ARG;
+    return s->stats(out);
+}
+
+double lbm_srv_clock(const lbm_handle *h)
+{
+    auto *s = (h && h->kind == lbm_handle::SERVER) ? static_cast<const lbm::Server *>(h) : nullptr;
+    return s ? s->clock() : -1.0;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+"""Continuous-batching permeability server over the C ABI (``lbm_srv_*``).
+
+Many independent single-phase permeability solves -- the reference's sub-sample workflow of
+one INPUT file and one solver run per sub-sample (CPP_SCRIPT/HPC_SUBDOMAIN_INPUT*.cpp) --
+served from one device arena: requests are admitted by HBM footprint as they arrive, stepped
+together, and retired when k changes by less than ``tol`` (relative) over ``check_interval``
+steps.  The request loop runs on a native thread inside the library.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import numpy as np
+
+from . import _capi
+
+
+class PermeabilityServer:
+    def __init__(self, capacity_nodes: int = 0, mem_fraction: float = 0.5, check_interval: int = 100,
+                 tol: float = 1e-6, min_steps: int = 0, max_steps: int = 200000, precision: str = "fp32",
+                 block: int = 0, device: int = 0):
+        p = _capi.SrvParams()
+        p.capacity_nodes = int(capacity_nodes)
+        p.mem_fraction = float(mem_fraction)
+        p.check_interval = int(check_interval)
+        p.tol = float(tol)
+        p.min_steps = int(min_steps)
+        p.max_steps = int(max_steps)
+        p.precision = _capi.PRECISIONS[precision]
+        p.block = int(block)
+        p.device = int(device)
+        h = ctypes.c_void_p()
+        _capi.check(_capi.lib().lbm_srv_create(ctypes.byref(p), ctypes.byref(h)), None)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _capi.lib().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, solid: np.ndarray, nu: float, g: Sequence[float], perm_dir: int = 1) -> int:
+        solid = np.ascontiguousarray(solid, np.int32)
+        r = _capi.SrvRequest()
+        r.nx, r.ny, r.nz = solid.shape
+        r.solid_ijk = solid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        r.nu = float(nu)
+        for a in range(3):
+            r.g[a] = float(g[a])
+        r.perm_dir = int(perm_dir)
+        t = ctypes.c_int64()
+        _capi.check(_capi.lib().lbm_srv_submit(self._h, ctypes.byref(r), ctypes.byref(t)), self._h)
+        return int(t.value)
+
+    def poll(self, max_results: int = 1024) -> List[dict]:
+        buf = (_capi.SrvResult * max_results)()
+        n = ctypes.c_int32()
+        _capi.check(_capi.lib().lbm_srv_poll(self._h, buf, max_results, ctypes.byref(n)), self._h)
+        out = []
+        for r in buf[: n.value]:
+            out.append({"ticket": r.ticket, "k": np.array(r.k[:]), "rel_change": r.rel_change, "steps": r.steps,
+                        "fluid_nodes": r.fluid_nodes, "converged": r.converged, "t_submit": r.t_submit,
+                        "t_admit": r.t_admit, "t_done": r.t_done})
+        return out
+
+    def wait_idle(self, timeout_s: float = 0.0) -> bool:
+        rc = _capi.lib().lbm_srv_wait_idle(self._h, float(timeout_s))
+        if rc == -16:
+            return False
+        _capi.check(rc, self._h)
+        return True
+
+    def stats(self) -> dict:
+        s = _capi.SrvStats()
+        _capi.check(_capi.lib().lbm_srv_get_stats(self._h, ctypes.byref(s)), self._h)
+        return {k: getattr(s, k) for k, _ in _capi.SrvStats._fields_ if k != "pad"}
+
+    def clock(self) -> float:
+        return _capi.lib().lbm_srv_clock(self._h)

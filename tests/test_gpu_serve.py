@@ -1,0 +1,107 @@
+"""Continuous-batching permeability server: results independent of batch mates, admission by
+HBM footprint, and agreement with a plain single-sample solve."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _samples(pkg):
+    geo = pkg.geometry
+    return [geo.grf_sample(24, 3.0, 0.4, seed=101),
+            geo.grf_sample(32, 3.0, 0.35, seed=102),
+            geo.sphere_pack((28, 20, 36), 3.0, 0.5, seed=103),
+            geo.grf_sample(20, 2.5, 0.45, seed=104),
+            geo.sphere_pack((40, 24, 24), 4.0, 0.55, seed=105)]
+
+
+def _run(srv, reqs):
+    tickets = [srv.submit(s, nu, g) for (s, nu, g) in reqs]
+    assert srv.wait_idle(120.0)
+    res = {r["ticket"]: r for r in srv.poll()}
+    return [res[t] for t in tickets]
+
+
+def test_result_is_the_same_alone_and_in_the_mix(pkg):
+    _log("[gpu] serve: alone vs mixed")
+    samples = _samples(pkg)
+    reqs = [(s, 1.0 / 6.0 + 0.01 * i, (1e-5, 0.0, 0.0)) for i, s in enumerate(samples)]
+    alone = []
+    for r in reqs:
+        with pkg.PermeabilityServer(capacity_nodes=400_000, check_interval=50, tol=1e-5, block=256) as srv:
+            alone.append(_run(srv, [r])[0])
+    with pkg.PermeabilityServer(capacity_nodes=400_000, check_interval=50, tol=1e-5, block=256) as srv:
+        mixed = _run(srv, reqs)
+    for a, m in zip(alone, mixed):
+        assert a["converged"] == 1 and m["converged"] == 1
+        assert a["steps"] == m["steps"]
+        np.testing.assert_array_equal(a["k"], m["k"])
+
+
+def test_staggered_arrivals_and_footprint_admission(pkg):
+    """An arena that holds only ~two samples at a time: later requests wait for memory, are
+    admitted as earlier ones retire, and still give the same answers."""
+    _log("[gpu] serve: footprint admission")
+    samples = _samples(pkg)
+    reqs = [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0)) for s in samples]
+    with pkg.PermeabilityServer(capacity_nodes=400_000, check_interval=50, tol=1e-5, block=256) as srv:
+        ref = _run(srv, reqs)
+    biggest = max(int(np.count_nonzero(s == 0)) for s in samples)
+    cap = 2 * ((biggest + 255) // 256 * 256)
+    with pkg.PermeabilityServer(capacity_nodes=cap, check_interval=50, tol=1e-5, block=256) as srv:
+        tickets = []
+        saw_pending = False
+        for (s, nu, g) in reqs:
+            tickets.append(srv.submit(s, nu, g))
+            st = srv.stats()
+            saw_pending |= st["pending"] > 0
+            assert st["arena_used"] <= st["arena_nodes"]
+            time.sleep(0.01)
+        assert srv.wait_idle(120.0)
+        got = {r["ticket"]: r for r in srv.poll()}
+        st = srv.stats()
+    assert st["done"] == len(reqs)
+    for t, r in zip(tickets, ref):
+        np.testing.assert_array_equal(got[t]["k"], r["k"])
+        assert got[t]["t_admit"] >= got[t]["t_submit"] and got[t]["t_done"] >= got[t]["t_admit"]
+
+
+def test_matches_plain_solver(pkg):
+    _log("[gpu] serve: vs plain solve")
+    s = pkg.geometry.grf_sample(32, 3.0, 0.35, seed=102)
+    with pkg.PermeabilityServer(capacity_nodes=200_000, check_interval=100, tol=1e-6, block=256) as srv:
+        r = _run(srv, [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0))])[0]
+    assert r["converged"] == 1
+    sp = pkg.SinglePhase(s, 1.0 / 6.0, (1e-5, 0.0, 0.0), precision="fp32")
+    sp.step(r["steps"])
+    k, _, _ = sp.Comput_Perm(1)
+    assert abs(r["k"][0] - k[0]) <= 1e-6 * abs(k[0])
+    # and the converged value is the FP64 answer to the stated accuracy
+    sp64 = pkg.SinglePhase(s, 1.0 / 6.0, (1e-5, 0.0, 0.0), precision="fp64")
+    sp64.step(r["steps"] + 2000)
+    k64, _, _ = sp64.Comput_Perm(1)
+    _log(f"[gpu] serve k {r['k'][0]:.8e} after {r['steps']} steps, fp64 long run {k64[0]:.8e}")
+    assert abs(r["k"][0] - k64[0]) <= 1e-4 * abs(k64[0])
+
+
+def test_rejects_bad_requests(pkg):
+    with pkg.PermeabilityServer(capacity_nodes=50_000, block=256) as srv:
+        with pytest.raises(pkg.LBMError):
+            srv.submit(np.ones((8, 8, 8), np.int32), 0.1, (1e-5, 0, 0))       # no pore voxels
+        with pytest.raises(pkg.LBMError):
+            srv.submit(np.zeros((8, 8, 8), np.int32), 0.1, (0.0, 1e-5, 0))    # no force along x
+        with pytest.raises(pkg.LBMError):
+            srv.submit(np.zeros((64, 64, 64), np.int32), 0.1, (1e-5, 0, 0))   # larger than the arena
+
+
+def test_autotuned_block(pkg):
+    with pkg.PermeabilityServer(capacity_nodes=2_000_000, block=0) as srv:
+        assert srv.stats()["block"] in (128, 256, 512)
