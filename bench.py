@@ -8,6 +8,12 @@ node (collision + comput_macro_variables of the reference, SP:662-684).
 
 Prints one JSON line: fluid MLUPS (whole job), the HBM roofline of the step kernel, and the
 FP64 CPU oracle timed on this host's cores on a bounded sample of the same workload.
+
+The same line carries the serving figures ("serving"): a seeded Poisson stream of single-phase
+permeability requests on thresholded Gaussian-random-field subsamples (edge 48..160 lu,
+correlation length 6 lu, porosity 0.15..0.45, FP32 populations, FP64 reductions, stop when k
+changes by < 1e-6 relative over 100 steps) served by the continuous-batching server, and the
+same stream served by sequential plain solves; permeabilities per hour and p50/p99 latency.
 """
 import argparse
 import importlib.util
@@ -68,6 +74,80 @@ def cpu_baseline(solid, nu, g, budget_s):
             "sample": f"FP64 C oracle (OpenMP) on the same 256^3 geometry, {n} steps after 1 warm-up step, {dt:.1f} s"}
 
 
+def serving_bench(pkg, nreq, rate, seed=7, max_steps=20000):
+    """Server vs sequential plain solves on the same seeded request stream."""
+    geo = pkg.geometry
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(48, 161, nreq)
+    pors = rng.uniform(0.15, 0.45, nreq)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    samples = [geo.grf_sample(int(e), 6.0, float(p), seed=seed * 1000 + i) for i, (e, p) in enumerate(zip(edges, pors))]
+    nu, g, tol, interval = 1.0 / 6.0, (1e-6, 0.0, 0.0), 1e-6, 100
+    nodes = [int(np.count_nonzero(s == 0)) for s in samples]
+
+    # continuous batching
+    with pkg.PermeabilityServer(check_interval=interval, tol=tol, max_steps=max_steps, block=0) as srv:
+        t0 = srv.clock()
+        tickets = []
+        for i in range(nreq):
+            while srv.clock() - t0 < arrivals[i]:
+                time.sleep(min(0.002, max(0.0, arrivals[i] - (srv.clock() - t0))))
+            tickets.append(srv.submit(samples[i], nu, g))
+        srv.wait_idle()
+        res = {r["ticket"]: r for r in srv.poll(4 * nreq)}
+        st = srv.stats()
+    srv_res = [res[t] for t in tickets]
+    srv_lat = np.array([r["t_done"] - r["t_submit"] for r in srv_res])
+    srv_span = max(r["t_done"] for r in srv_res) - min(r["t_submit"] for r in srv_res)
+
+    # sequential plain solves, FIFO on the same arrival schedule
+    seq_lat, seq_k = [], []
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        now = time.perf_counter() - t0
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        s = pkg.SinglePhase(samples[i], nu, g, precision="fp32")
+        steps, k_prev = 0, None
+        while True:
+            s.step(interval)
+            steps += interval
+            k, _, _ = s.Comput_Perm(1)
+            kd = k[0]
+            if not np.isfinite(kd):
+                break
+            if k_prev is not None and steps >= 2 * interval and abs(kd - k_prev) <= tol * abs(kd):
+                break
+            if steps >= max_steps:
+                break
+            k_prev = kd
+        s.close()
+        seq_k.append(kd)
+        seq_lat.append(time.perf_counter() - t0 - arrivals[i])
+    seq_span = time.perf_counter() - t0
+    seq_lat = np.array(seq_lat)
+    k_srv = np.array([r["k"][0] for r in srv_res])
+    k_seq = np.array(seq_k)
+    ok = np.isfinite(k_seq) & (np.abs(k_seq) > 0)
+    k_dev = float(np.max(np.abs(k_srv[ok] - k_seq[ok]) / np.abs(k_seq[ok]))) if ok.any() else None
+    return {
+        "requests": nreq, "arrival_rate_per_s": rate, "seed": seed,
+        "fluid_nodes_total": int(sum(nodes)),
+        "perms_per_hour": round(nreq / srv_span * 3600.0, 1),
+        "p50_latency_s": round(float(np.percentile(srv_lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(srv_lat, 99)), 4),
+        "sequential_perms_per_hour": round(nreq / seq_span * 3600.0, 1),
+        "sequential_p50_latency_s": round(float(np.percentile(seq_lat, 50)), 4),
+        "sequential_p99_latency_s": round(float(np.percentile(seq_lat, 99)), 4),
+        "speedup_vs_sequential": round(seq_span / srv_span, 2),
+        "converged": int(sum(1 for r in srv_res if r["converged"] == 1)),
+        "max_rel_k_diff_vs_sequential": k_dev,
+        "tile": st["block"], "server_device_busy_s": round(st["busy_seconds"], 3),
+        "server_mlups": round(st["node_updates"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +157,10 @@ def main():
     ap.add_argument("--edge", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
@@ -84,6 +168,9 @@ def main():
         sys.exit(2)
 
     pkg = load_package()
+    if args.serve_only:
+        print(json.dumps({"serving": serving_bench(pkg, args.serve_requests, args.serve_rate)}))
+        return
     edge = args.edge
     solid = pkg.geometry.sphere_pack((edge, edge, edge), 8.0, 0.35, seed=2024)
     nu, g = 1.0 / 6.0, (1e-6, 0.0, 0.0)
@@ -124,8 +211,11 @@ def main():
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
         "cpu_baseline": None,
     }
+    s.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds)
+    if not args.no_serve:
+        out["serving"] = serving_bench(pkg, args.serve_requests, args.serve_rate)
     print(json.dumps(out))
 
 

@@ -107,6 +107,7 @@ typedef struct lbm_cg_params {
     double rho0;              /* rest density of the shifted storage, 1.0 */
     int32_t precision;        /* LBM_FP64 / LBM_FP32 (FP16 is not offered for the colour-gradient path) */
     int32_t device;
+    int32_t perm_dir;         /* PerDir 1..3: flow axis of the saturation displacement sums */
 } lbm_cg_params;
 
 typedef struct lbm_cg_diag {
@@ -115,6 +116,7 @@ typedef struct lbm_cg_diag {
     double err, umax, sum_speed, nan_count;   /* as lbm_sp_diag (Error, CG:4979-5040) */
     double wet_count;    /* nodes with psi >= 0 in the box -- Comput_Saturation, CG:6640-6678 */
     double count_in_box; /* fluid nodes in the box */
+    double sd_l, sd_g;   /* sum of psi * (coordinate along perm_dir) over psi >= 0 / psi < 0 (CG:6655-6667) */
 } lbm_cg_diag;
 
 /* geometry, numbering, neighbour lists, allocation (CG:1887-2308, CG:598-660) */

@@ -38,13 +38,14 @@ class CgParams(ctypes.Structure):
                 ("nu_l", ctypes.c_double), ("nu_g", ctypes.c_double), ("cos_theta", ctypes.c_double),
                 ("cap_a", ctypes.c_double), ("g", ctypes.c_double * 3), ("bodyforce_apply", ctypes.c_int32),
                 ("rel_perm_psi", ctypes.c_double), ("rel_perm_psi2", ctypes.c_double), ("rho0", ctypes.c_double),
-                ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("device", ctypes.c_int32), ("perm_dir", ctypes.c_int32)]
 
 
 class CgDiag(ctypes.Structure):
     _fields_ = [("Ql", ctypes.c_double * 3), ("Qg", ctypes.c_double * 3), ("err", ctypes.c_double),
                 ("umax", ctypes.c_double), ("sum_speed", ctypes.c_double), ("nan_count", ctypes.c_double),
-                ("wet_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
+                ("wet_count", ctypes.c_double), ("count_in_box", ctypes.c_double), ("sd_l", ctypes.c_double),
+                ("sd_g", ctypes.c_double)]
 
 
 class SrvParams(ctypes.Structure):

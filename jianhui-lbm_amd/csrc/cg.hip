@@ -256,11 +256,11 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(const uint32_t *__rest
 }
 
 // ---------------------------------------------------------------- diagnostics
-constexpr int CG_NDIAG = 13;   // Ql[3] Qg[3] err_num err_den speed umax2 nan wet box
+constexpr int CG_NDIAG = 15;   // Ql[3] Qg[3] err_num err_den speed umax2 nan wet box Sd_l Sd_g
 constexpr uint32_t CG_MAX_MASK = 1u << 9;
 
 struct CgBox {
-    int32_t on, xn, xp, yn, yp, zn, zp, ny, nz;
+    int32_t on, xn, xp, yn, yp, zn, zp, ny, nz, perdir;
 };
 
 template <typename ST, int MODE>
@@ -278,14 +278,18 @@ __global__ void __launch_bounds__(BLOCK) cg_diag_kernel(CgArgs<ST> a, const int3
         Macro<C> mc;
         cg_node_state<ST, MODE>(a, n, h, m, nb, mc);
         const double ux = (double)mc.ux, uy = (double)mc.uy, uz = (double)mc.uz, ps = (double)a.psi[n];
+        const int32_t s = supinv[n];
+        const int32_t nyz = box.ny * box.nz;
+        const int32_t i = s / nyz, j = (s % nyz) / box.nz, k = s % box.nz;
         bool inside = true;
-        if (box.on) {
-            const int32_t s = supinv[n];
-            const int32_t nyz = box.ny * box.nz;
-            const int32_t i = s / nyz, j = (s % nyz) / box.nz, k = s % box.nz;
-            inside = i >= box.xn && i <= box.xp && j >= box.yn && j <= box.yp && k >= box.zn && k <= box.zp;
-        }
+        if (box.on) inside = i >= box.xn && i <= box.xp && j >= box.yn && j <= box.yp && k >= box.zn && k <= box.zp;
         if (inside) {
+            // saturation "displacement": psi-weighted position along the flow axis (CG:6655-6667)
+            const double sdir = box.perdir == 2 ? (double)j : (box.perdir == 3 ? (double)k : (double)i);
+            if (ps >= 0.0)
+                v[13] = ps * sdir;
+            else
+                v[14] = ps * sdir;
             if (ps >= (double)a.rel1) {
                 v[0] = ux;
                 v[1] = uy;
@@ -466,6 +470,7 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
         return fail(LBM_ERR_ARG, "colour gradient supports LBM_FP64 and LBM_FP32");
     if (!(p.nu_l > 0.0) || !(p.nu_g > 0.0)) return fail(LBM_ERR_ARG, "viscosities must be positive");
     if (prm_.rho0 == 0.0) prm_.rho0 = 1.0;
+    if (prm_.perm_dir < 1 || prm_.perm_dir > 3) prm_.perm_dir = 1;
     std::string err;
     int rc = build_sparse_geometry(p.nx, p.ny, p.nz, solid, geo_, err);
     if (rc != LBM_OK) return fail(rc, err);
@@ -695,6 +700,7 @@ int CgSolver::diagnostics(const int32_t *box, lbm_cg_diag *out)
     CgBox cb{};
     cb.ny = geo_.ny;
     cb.nz = geo_.nz;
+    cb.perdir = prm_.perm_dir;
     if (box) {
         cb.on = 1;
         cb.xn = box[0];
@@ -742,6 +748,8 @@ int CgSolver::diagnostics(const int32_t *box, lbm_cg_diag *out)
     out->nan_count = r[10];
     out->wet_count = r[11];
     out->count_in_box = r[12];
+    out->sd_l = r[13];
+    out->sd_g = r[14];
     return LBM_OK;
 }
 

@@ -1,0 +1,245 @@
+// sp_main.cpp -- lbm_sp: drop-in single-phase driver for SINGLE_PHASE_MPI_Software_Spars.
+//
+//   lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N]
+//
+// Reads the reference's INPUT.dat and geometry file, runs the time loop of SP:662-841 on one
+// GPU through the C ABI, and writes the reference's result files under the output prefix:
+// Results.txt, Permeability_error_local_Perm.txt (wr_per = 1), bodyforce.txt,
+// Velocity_ave_max.txt, General_Results.txt, and the checkpoint files of Backup (SP:4507).
+// Features outside the hot path (pressure/velocity boundaries, non-Newtonian collision, VTK
+// output, lattice velocity scaling, group/local permeability) are reported and refused or
+// skipped, never silently approximated.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "host_io.h"
+#include "lbm_mi355x.h"
+
+using lbmhost::SpInput;
+
+namespace {
+
+int die(const std::string &msg)
+{
+    std::cerr << "lbm_sp: " << msg << std::endl;
+    return 1;
+}
+
+int check(int rc, lbm_handle *h, const char *what)
+{
+    if (rc != LBM_OK) {
+        const char *m = lbm_last_error(h);
+        std::cerr << "lbm_sp: " << what << " failed (" << rc << "): " << (m ? m : "") << std::endl;
+    }
+    return rc;
+}
+
+void hms(double s, int &h, int &m, int &sec)
+{
+    h = (int)(s / 3600);
+    m = (int)((s - h * 3600) / 60);
+    sec = (int)(s - (h * 3600 + m * 60));
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::cerr << "usage: lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N]\n";
+        return 2;
+    }
+    std::string input = argv[1], pfix = "./";
+    int precision = LBM_FP64, device = 0;
+    for (int a = 2; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--precision") && a + 1 < argc) {
+            const std::string p = argv[++a];
+            precision = p == "fp32" ? LBM_FP32 : (p == "fp16" ? LBM_FP16 : LBM_FP64);
+        } else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) {
+            device = std::atoi(argv[++a]);
+        } else {
+            pfix = argv[a];
+        }
+    }
+    SpInput in;
+    std::string err;
+    if (!lbmhost::read_sp_input(input, in, err)) return die(err);
+    if (in.in_BC != 0) return die("pressure/velocity boundaries (in_BC = 1) are not part of this build");
+    if (in.nnf_module != 0) return die("the non-Newtonian module is not part of this build");
+    if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
+        return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
+    if (in.freVe > 0 || in.freDe > 0 || in.vel_sol > 0 || in.gperm > 0 || in.loc_perm == 1)
+        std::cerr << "lbm_sp: note: VTK, solute-velocity, group and local permeability outputs are skipped\n";
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    std::vector<int32_t> solid;
+    if (!lbmhost::read_geometry(in.filename, nx, ny, nz, in.decbin, solid, err)) return die(err);
+    int box[6];
+    bool partial = false;
+    lbmhost::perm_box(in.par_per, in.per_x, in.per_y, in.per_z, nx, ny, nz, box, partial);
+    const double porosity = lbmhost::box_porosity(solid, nx, ny, nz, box);
+    const double vbox = (double)(box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1);
+    const double vfull = (double)nx * ny * nz;
+
+    lbm_sp_params p{};
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    p.nu = in.in_vis;
+    for (int a = 0; a < 3; ++a) p.g[a] = in.g[a];
+    p.rho0 = 1.0;
+    p.precision = precision;
+    p.device = device;
+    lbm_handle *h = nullptr;
+    if (check(lbm_sp_create(&p, solid.data(), &h), nullptr, "lbm_sp_create")) return 1;
+    int64_t count = 0;
+    lbm_sp_count(h, &count);
+    const auto t_start = std::chrono::steady_clock::now();
+
+    if (in.mode_backup_ini == 0) {
+        if (check(lbm_sp_init(h, 1.0, in.ini_v), h, "lbm_sp_init")) return 1;
+    } else {   // Backup_init (SP:4715-4828): files of label mode_backup_ini under the prefix
+        std::vector<double> f((size_t)count * 19), u((size_t)count * 3), rho((size_t)count);
+        if (!lbmhost::read_checkpoint(lbmhost::checkpoint_name(pfix, "rho", in.mode_backup_ini), rho.data(), count, 1, err) ||
+            !lbmhost::read_checkpoint(lbmhost::checkpoint_name(pfix, "velocity", in.mode_backup_ini), u.data(), count, 3, err) ||
+            !lbmhost::read_checkpoint(lbmhost::checkpoint_name(pfix, "f", in.mode_backup_ini), f.data(), count, 19, err))
+            return die(err);
+        if (check(lbm_sp_load_state(h, f.data(), u.data(), rho.data()), h, "lbm_sp_load_state")) return 1;
+    }
+    std::cout << "\nINITIALIZATION COMPLETED  (" << count << " fluid nodes, porosity " << porosity << ")\n\n";
+
+    const std::string out = in.outputfile;
+    const std::string fResults = out + "Results.txt", fPerm = out + "Permeability_error_local_Perm.txt",
+                      fBody = out + "bodyforce.txt", fVel = out + "Velocity_ave_max.txt";
+    std::ofstream(fResults, std::ios::trunc).close();
+    if (in.wr_per == 1) std::ofstream(fPerm, std::ios::trunc).close();
+    std::ofstream(fBody, std::ios::out).close();
+    std::ofstream(fVel, std::ios::out).close();
+
+    auto backup = [&](long label) -> bool {
+        std::vector<double> f((size_t)count * 19), u((size_t)count * 3), rho((size_t)count);
+        if (check(lbm_sp_get_state(h, f.data(), u.data(), rho.data()), h, "lbm_sp_get_state")) return false;
+        return lbmhost::write_checkpoint(lbmhost::checkpoint_name(out, "velocity", label), u.data(), count, 3, err) &&
+               lbmhost::write_checkpoint(lbmhost::checkpoint_name(out, "rho", label), rho.data(), count, 1, err) &&
+               lbmhost::write_checkpoint(lbmhost::checkpoint_name(out, "f", label), f.data(), count, 19, err);
+    };
+
+    const double rr = in.reso * in.reso * 1000;   // printed permeability unit (SP:742)
+    double Permia[3] = {0, 0, 0}, error_perm = 0, u_max = 0, u_ave = 0, u_ave2 = -10000000, error = 0;
+    double Re = 0;
+    int U_max_ref = 0;
+    const int dir = (in.PerDir >= 1 && in.PerDir <= 3) ? in.PerDir - 1 : 0;
+    const double s_v = 1.0 / (in.in_vis * 3.0 + 0.5);
+    int n = 0;
+    long steps_done = 0;
+    int th = 0, tm = 0, ts = 0, the = 0, tme = 0, tse = 0;
+    while (n <= in.n_max) {
+        // advance to the next diagnostics step (n % freRe == 0) or the end, in one call
+        int next = (in.freRe > 0) ? ((n % in.freRe == 0) ? n : (n / in.freRe + 1) * in.freRe) : in.n_max;
+        if (next > in.n_max) next = in.n_max;
+        const int todo = next - n + 1;
+        if (check(lbm_sp_step(h, todo), h, "lbm_sp_step")) return 1;
+        steps_done += todo;
+        n = next;
+        if (in.freRe <= 0 || n % in.freRe != 0) {
+            ++n;
+            continue;
+        }
+        {   // rewrite the previous block (SP:692-707)
+            std::ofstream fin(fResults, std::ios::out);
+            fin << "The" << n - in.freRe << "th computation result:" << std::endl;
+            fin << "The permiability is: " << Permia[0] * rr << ", " << Permia[1] * rr << ", " << Permia[2] * rr << std::endl;
+            fin << "The relative error of permiability computing is: " << error_perm << std::endl;
+            fin << "The LOCAL permiability is: " << 0 << ", " << 0 << ", " << 0 << std::endl;
+            fin << "The Maximum velocity is: " << std::setprecision(6) << u_max << "   Re=" << Re
+                << "     Courant Number=" << u_max << std::endl;
+            fin << "The max relative error of velocity is: " << std::setiosflags(std::ios::scientific) << error << std::endl;
+            fin << "Elapsed time is " << the << "h" << tme << "m" << tse << "s" << std::endl;
+            fin << "The expected completion time is " << th << "h" << tm << "m" << ts << "s" << std::endl << std::endl;
+        }
+        // Error (SP:3625-3704) and Comput_Perm (SP:4565-4712) from the device reductions
+        u_ave2 = u_ave;
+        lbm_sp_diag dfull{}, dbox{};
+        if (check(lbm_sp_diagnostics(h, nullptr, &dfull), h, "lbm_sp_diagnostics")) return 1;
+        if (partial) {
+            if (check(lbm_sp_diagnostics(h, box, &dbox), h, "lbm_sp_diagnostics")) return 1;
+        } else {
+            dbox = dfull;
+        }
+        error = dfull.err;
+        u_max = dfull.umax;
+        u_ave = dfull.sum_speed / (vfull * porosity);
+        if (u_max >= 10.0) U_max_ref += 1;
+        double Perm[3];
+        for (int a = 0; a < 3; ++a) Perm[a] = dbox.Q[a] / vbox * in.in_vis / (in.g[a] + 0.0);
+        error_perm = (Perm[dir] - Permia[dir]) / Permia[dir];
+        for (int a = 0; a < 3; ++a) Permia[a] = Perm[a];
+        {
+            const double ax = dbox.Q[0] / (vbox * porosity), ay = dbox.Q[1] / (vbox * porosity), az = dbox.Q[2] / (vbox * porosity);
+            u_ave = std::sqrt(ax * ax + ay * ay + az * az);
+        }
+        const double elaps = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        const double remain = n > 0 ? (in.n_max - n) * (elaps / n) : 0.0;
+        hms(remain, th, tm, ts);
+        hms(elaps, the, tme, tse);
+        Re = u_ave * ny / (1.0 / 3.0 * (1 / s_v - 0.5));
+        {   // append this block (SP:725-757)
+            std::ofstream fin(fResults, std::ios::app);
+            fin << "The" << n << "th computation result:" << std::endl;
+            fin << "The permiability is: " << Permia[0] * rr << ", " << Permia[1] * rr << ", " << Permia[2] * rr << std::endl;
+            fin << "The LOCAL permiability is: " << 0 << ", " << 0 << ", " << 0 << std::endl;
+            fin << "The relative error of permiability computing is: " << error_perm << std::endl;
+            fin << "The Maximum velocity is: " << std::setprecision(6) << u_max << "   Re=" << Re
+                << "     Courant Number=" << u_max << std::endl;
+            fin << "The max relative error of velocity is: " << std::setiosflags(std::ios::scientific) << error << std::endl;
+            fin << "Elapsed time is " << the << "h" << tme << "m" << tse << "s" << std::endl;
+            fin << "The expected completion time is " << th << "h" << tm << "m" << ts << "s" << std::endl << std::endl;
+        }
+        if (in.wr_per == 1) {
+            std::ofstream f(fPerm, std::ios::app);
+            f << Permia[dir] * rr << " " << error_perm << " " << 0 << std::endl;
+        }
+        std::ofstream(fBody, std::ios::app) << in.g[0] << std::endl;
+        std::ofstream(fVel, std::ios::app) << u_ave << "  " << u_max << " " << error << std::endl;
+        std::cout << "The" << n << "th computation result:\n"
+                  << "The permiability is: " << Permia[0] * rr << ", " << Permia[1] * rr << ", " << Permia[2] * rr << "\n"
+                  << "The relative error of permiability computing is: " << error_perm << "\n"
+                  << "The Maximum velocity is: " << u_max << "   Re=" << Re << "\n"
+                  << "The max relative error of uv is: " << error << "\n\n";
+        bool stop = false;
+        if (in.Out_Mode == 1 && std::fabs((u_ave2 - u_ave) / u_ave2) < 1e-7) stop = true;   // SP:812-816
+        if (in.fre_backup > 0 && n % in.fre_backup == 0 && n > 0 && !backup(n)) return die(err);
+        if (error != error) {
+            std::cout << "PROGRAM STOP" << std::endl;
+            stop = true;
+        }
+        if (U_max_ref >= 5) {
+            std::cout << "PROGRAM STOP DUE TO HIGH VELOCITY" << std::endl;
+            stop = true;
+        }
+        if (stop) break;
+        ++n;
+    }
+    if (in.fre_backup >= 0 && !backup(in.n_max)) return die(err);
+    const double elaps = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    hms(elaps, the, tme, tse);
+    std::cout << "Elapsed time is " << the << "h" << tme << "m" << tse << "s" << std::endl;
+    std::cout << "Fluid-node updates: " << (double)count * steps_done << "  (" << count * (double)steps_done / elaps / 1e6
+              << " MLUPS including I/O)" << std::endl;
+    {
+        std::ofstream fin(fResults, std::ios::app);
+        fin << "Elapsed time is " << the << "h" << tme << "m" << tse << "s" << std::endl;
+    }
+    {   // FOR SCRIPT RUN (SP:886-897)
+        std::ofstream fin(out + "General_Results.txt", std::ios::app);
+        fin << in.filename << "  " << Permia[0] * rr << " " << Permia[1] * rr << " " << Permia[2] * rr << std::endl << std::endl;
+    }
+    lbm_destroy(h);
+    return 0;
+}

@@ -164,7 +164,8 @@ class ColorGradient:
 
     def __init__(self, solid: np.ndarray, nu_l: float, nu_g: float, cos_theta: float, cap_a: float,
                  g: Sequence[float], precision: str = "fp32", bodyforce_apply: int = 0,
-                 rel_perm_psi: float = 0.99, rel_perm_psi2: float = 0.92, rho0: float = 1.0, device: int = 0):
+                 rel_perm_psi: float = 0.99, rel_perm_psi2: float = 0.92, rho0: float = 1.0, device: int = 0,
+                 perm_dir: int = 1):
         L = _capi.lib()
         solid = np.ascontiguousarray(solid, np.int32)
         self.shape = solid.shape
@@ -184,6 +185,7 @@ class ColorGradient:
         p.rho0 = float(rho0)
         p.precision = _capi.PRECISIONS[precision]
         p.device = int(device)
+        p.perm_dir = int(perm_dir)
         h = ctypes.c_void_p()
         _capi.check(L.lbm_cg_create(ctypes.byref(p), solid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                     ctypes.byref(h)), None)
@@ -250,7 +252,7 @@ class ColorGradient:
         _capi.check(_capi.lib().lbm_cg_diagnostics(self._h, b, ctypes.byref(d)), self._h)
         return {"Ql": np.array(d.Ql[:]), "Qg": np.array(d.Qg[:]), "err": d.err, "umax": d.umax,
                 "sum_speed": d.sum_speed, "nan_count": d.nan_count, "wet_count": d.wet_count,
-                "count_in_box": d.count_in_box}
+                "count_in_box": d.count_in_box, "sd_l": d.sd_l, "sd_g": d.sd_g}
 
     def box_volume(self, box=None) -> int:
         if box is None:

@@ -1,0 +1,148 @@
+"""The drop-in driver programs (lbm_sp, lbm_cg) on the reference's own input files.
+
+Each case writes the INPUT file, geometry and phase file that produced a golden fixture,
+runs the driver in a scratch directory, and compares its result files and checkpoint files
+with what the reference wrote for the same input."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import PKG_DIR, load_fixture, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rows(text):
+    return [[float(x) for x in ln.split()] for ln in text.strip().splitlines() if ln.strip()]
+
+
+def _read_ckpt(path, width):
+    a = np.fromfile(path, dtype="<f8")
+    return (a.reshape(-1, width) if width > 1 else a)[1:]
+
+
+def _run(exe, cwd, *args):
+    proc = subprocess.run([os.path.join(PKG_DIR, exe), "INPUT.dat", *args], cwd=cwd, capture_output=True, text=True,
+                          timeout=300)
+    assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr[-2000:]
+    return proc.stdout
+
+
+def _sp_case(pkg, tmp_path, name, n_max, **over):
+    d, p = load_fixture(name)
+    solid = d["solid"].astype(np.int32)
+    pkg.geometry.write_binary(str(tmp_path / "geometry.bin"), solid)
+    inp = pkg.inputs.SPInput(filename="geometry.bin", n_xyz=tuple(p["shape"]), n_max=n_max, reso=1.0,
+                             g=tuple(p["g"]), in_vis=p["nu"], ini_v=tuple(p["u0"]), wr_per=1, PerDir=1,
+                             freRe=p["freRe"], Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
+    box = p["box"]
+    if box is not None:
+        inp.par_per = (1, 1, 1)
+        inp.per_x, inp.per_y, inp.per_z = (box[1], box[0]), (box[3], box[2]), (box[5], box[4])
+    for k, v in over.items():
+        setattr(inp, k, v)
+    pkg.inputs.write_sp_input(inp, str(tmp_path / "INPUT.dat"))
+    return d, p
+
+
+@pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12"])
+def test_lbm_sp_outputs_match_reference(pkg, tmp_path, name):
+    _log(f"[gpu] lbm_sp on {name}")
+    d, p = _sp_case(pkg, tmp_path, name, n_max=p_last(name) - 1)
+    _run("lbm_sp", str(tmp_path))
+    last = p["steps"][-1]
+    got = _rows(open(tmp_path / "Permeability_error_local_Perm.txt").read())
+    ref = _rows(str(d[f"perm_txt_{last}"]))
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0])             # k, 6 significant digits
+        if np.isfinite(r[1]) and abs(r[1]) > 1e-3:
+            assert abs(g[1] - r[1]) <= 1e-4 * abs(r[1]) + 1e-6   # relative change of k
+    vg = _rows(open(tmp_path / "Velocity_ave_max.txt").read())
+    vr = _rows(str(d[f"vel_txt_{last}"]))
+    for g, r in zip(vg, vr):
+        assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0]) and abs(g[1] - r[1]) <= 2e-5 * abs(r[1])
+    gen = open(tmp_path / "General_Results.txt").read().split()
+    assert gen[0] == "geometry.bin"
+    assert abs(float(gen[1]) - float(str(d[f"general_txt_{last}"]).split()[1])) <= 2e-5 * abs(float(gen[1]))
+    # the final checkpoint (label n_max) is the reference's state after n_max + 1 steps
+    f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
+    u = _read_ckpt(tmp_path / f"LBM_checkpoint_velocity_{last - 1}.0.bin_input", 3)
+    assert rel_err(f, d[f"f_{last}"]) < 1e-13
+    assert np.abs(u - d[f"u_{last}"]).max() < 1e-14
+
+
+def p_last(name):
+    return load_fixture(name)[1]["steps"][-1]
+
+
+def test_lbm_sp_restart_from_its_own_checkpoint(pkg, tmp_path):
+    """Backup at label 50 (51 steps), restart with mode_backup_ini = 50 for 50 more steps,
+    compare with the reference's state after 101 steps."""
+    _log("[gpu] lbm_sp restart")
+    d, p = _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=50)
+    _run("lbm_sp", str(tmp_path))
+    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=49, mode_backup_ini=50)
+    _run("lbm_sp", str(tmp_path), "./")
+    f = _read_ckpt(tmp_path / "LBM_checkpoint_f_49.0.bin_input", 19)
+    assert rel_err(f, d["f_101"]) < 1e-13
+
+
+def test_lbm_sp_fp32_precision_flag(pkg, tmp_path):
+    d, p = _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=100)
+    _run("lbm_sp", str(tmp_path), "--precision", "fp32")
+    f = _read_ckpt(tmp_path / "LBM_checkpoint_f_100.0.bin_input", 19)
+    assert rel_err(f, d["f_101"]) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["cg_binary_16x12x8", "cg_mixed_gated_14x10x8"])
+def test_lbm_cg_outputs_match_reference(pkg, tmp_path, name):
+    _log(f"[gpu] lbm_cg on {name}")
+    d, p = load_fixture(name)
+    last = p["steps"][-1]
+    pkg.geometry.write_binary(str(tmp_path / "geometry.bin"), d["solid"].astype(np.int32))
+    pkg.geometry.write_phase_ascii(str(tmp_path / "phase.dat"), d["psi_field"])
+    inp = pkg.inputs.CGInput(filename="geometry.bin", filenamepsi="phase.dat", n_xyz=tuple(p["shape"]),
+                             n_max=last - 1, reso=1.0, g=tuple(p["g"]), niu_l=p["nu_l"], niu_g=p["nu_g"],
+                             cos_theta=p["cos_theta"], CapA=p["cap_a"], Permeability=1.0, wr_per=1, PerDir=1,
+                             freRe=p["freRe"], outputfile="./", stab=(0, 0), ini_Sat=-1.0,
+                             rel_perm_psi=tuple(p["rel_perm_psi"]), fre_backup=0, decbin=1,
+                             bodyforce_apply=p["bodyforce_apply"])
+    inp.least_square = (0, 10)
+    pkg.inputs.write_cg_input(inp, str(tmp_path / "INPUT.dat"))
+    _run("lbm_cg", str(tmp_path))
+    bg = _rows(open(tmp_path / "bodyforce.txt").read())
+    br = _rows(str(d[f"bodyforce_txt_{last}"]))
+    assert len(bg) == len(br)
+    # S_l counts psi >= 0: nodes whose psi is zero up to round-off (exact colour ties, common
+    # right after a +-1 start) may fall on either side; the first row's band is the number of
+    # such nodes in the reference's own 1-step state, later rows allow two nodes
+    n = d["psi_1"].size
+    ties = int(np.count_nonzero(np.abs(d["psi_1"]) < 1e-12))
+    for i, (g, r) in enumerate(zip(bg, br)):
+        band = (ties if i == 0 else 2) / n + 2e-6
+        assert abs(g[0] - r[0]) <= band
+        assert abs(g[3] - r[3]) <= 2e-3 * abs(r[3]) + 0.5 and abs(g[4] - r[4]) <= 2e-3 * abs(r[4]) + 0.5
+    kg = _rows(open(tmp_path / "Relative_Permeability_Component1.txt").read())
+    kr = _rows(str(d[f"relperm1_txt_{last}"]))
+    for g, r in zip(kg, kr):
+        assert abs(g[0] - r[0]) <= 1e-4 * abs(r[0]) + 1e-6 * max(1.0, abs(r[0]))
+    f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
+    psi = _read_ckpt(tmp_path / f"LBM_checkpoint_psi_{last - 1}.0.bin_input", 1)
+    assert rel_err(f, d[f"f_{last}"]) < 1e-12
+    assert np.abs(psi - d[f"psi_{last}"]).max() < 1e-12
+
+
+def test_driver_refuses_out_of_path_features(pkg, tmp_path):
+    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, in_BC=1)
+    proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
+                          text=True, timeout=60)
+    assert proc.returncode != 0 and "not part of this build" in proc.stderr
