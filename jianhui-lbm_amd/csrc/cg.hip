@@ -16,7 +16,7 @@
 //
 // Data layout (row stride P):
 //   pops[2] 19 x P  deviations f - w rho0, canonical (SP) velocity order
-//   nbr     18 x P  upstream node per direction, BB_FLAG = solid upstream site
+//   nbr     packed upstream node per direction or bounce back (nbrpack.h)
 //   col      2 x P  rho_r, rho_b after colour streaming (the reference's rho_r/rho_b)
 //   psi      1 x P  (rho_r - rho_b)/(rho_r + rho_b)
 //   cs       8 x P  per node, contiguous: rho_r, rho_b, ux, uy, uz, nx, ny, nz
@@ -37,7 +37,7 @@ struct CgArgs {
     using C = typename Store<ST>::C;
     const ST *__restrict__ src;
     ST *__restrict__ dst;
-    const uint32_t *__restrict__ nbr;
+    NbrDev nbr;
     C *__restrict__ col;
     C *__restrict__ psi;
     C *__restrict__ cs;
@@ -97,8 +97,7 @@ __device__ __forceinline__ void cg_node_state(const CgArgs<ST> &a, int64_t n, ty
                                               Macro<typename Store<ST>::C> &mc)
 {
     using C = typename Store<ST>::C;
-#pragma unroll
-    for (int q = 1; q < Q; ++q) nb[q - 1] = a.nbr[(int64_t)(q - 1) * a.P + n];
+    load_nbr(a.nbr, a.P, n, nb);
     if (MODE == K_INIT) {
         mc.rho = C(1);
         mc.drho = C(1) - a.rho0;
@@ -152,7 +151,9 @@ __global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
         const uint32_t v = nb[opp(q) - 1];
         const bool solid = (v & BB_FLAG) != 0;
         wall |= solid;
-        const C pq = solid ? a.psi_solid : a.psi[v & ~BB_FLAG];
+        // unconditional load (a solid site's index is the node itself), then select
+        const C pv = a.psi[v & ~BB_FLAG];
+        const C pq = solid ? a.psi_solid : pv;
         const C wq = C(3.0 * weight(q));
         if (EX[q] != 0) cx += C(EX[q]) * wq * pq;
         if (EY[q] != 0) cy += C(EY[q]) * wq * pq;
@@ -186,13 +187,15 @@ __global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
     }
     cg_relax(m, mc, gx, gy, gz, sv, a.cap_a * cc, nx, ny, nz, h);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) a.dst[(int64_t)q * a.P + n] = Store<ST>::put(h[q], a.scale);
+    for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&a.dst[(int64_t)q * a.P + n], h[q], a.scale);
     const C st[8] = {rr, rb, mc.ux, mc.uy, mc.uz, nx, ny, nz};
     store_cs(a.cs, n, st);
 }
 
 // g_r, g_b leaving a node with colour state v in direction sgn*e_q (sgn = -1: the opposite
-// direction, used for the bounce-back term of the pull)
+// direction, used for the bounce-back term of the pull): the equilibrium split
+// g_c = w rho_c phi_q, phi_q = 1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u (CG:3477-3485), then the
+// recolouring g_r += d, g_b -= d, d = min(g_r,q, g_r,opp, g_b,q, g_b,opp) e.C/|C| (CG:3493-3508)
 template <typename C>
 __device__ __forceinline__ void colour_pull_term(const int q, const C (&v)[8], C sgn, C &gr, C &gb)
 {
@@ -215,11 +218,15 @@ __device__ __forceinline__ void colour_pull_term(const int q, const C (&v)[8], C
 }
 
 // The colour sums are formed in FP64 whatever the storage type: each node's rho_r is split
-// over 19 directions and re-summed at 19 receivers, and in FP32 the rounding of those products
-// is biased enough to drift the total colour mass (about 3e-6 of it per 200 steps); in FP64 the
-// sums are exact to ~1e-16 and only the unbiased rounding of the stored totals remains.
+// over 19 directions and re-summed at 19 receivers, and in FP32 the rounding of
+// 1 - 1.5 u.u + ... next to 1 is biased enough to drift the total colour mass (about 3e-6 of
+// it per 200 steps); in FP64 the sums are exact to ~1e-16 and only the unbiased rounding of
+// the stored totals remains.  The gathers are not staged through LDS: windows of colour states
+// around the block's node range and its two x-neighbour planes catch 99.6 % of the reads on a
+// 192^3 pack, but measured slower (148 us against 119 us for this form), the occupancy lost to
+// the windows costing more than the L2 traffic they save.
 template <typename C>
-__global__ void __launch_bounds__(BLOCK) cg_colour_kernel(const uint32_t *__restrict__ nbr, const C *__restrict__ cs,
+__global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *__restrict__ cs,
                                                           C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P)
 {
     const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -233,22 +240,30 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(const uint32_t *__rest
     const double uu = own[2] * own[2] + own[3] * own[3] + own[4] * own[4];
     const double w0 = weight(0) * (1.0 - 1.5 * uu);
     double rr = w0 * own[0], rb = w0 * own[1];
+    // All 18 source indices first, then the colour states in groups of G directions with no
+    // branch between them, so that each group's gathers are in flight together.  A solid
+    // upstream site sends back this node's own opposite term: same load address n, sign -1.
+    uint32_t src[Q - 1];
+    load_nbr(nbr, P, n, src);
+    constexpr int G = sizeof(C) == 4 ? 18 : 6;
 #pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        const uint32_t v = nbr[(int64_t)(q - 1) * P + n];
-        double gr, gb;
-        if (v & BB_FLAG) {
-            colour_pull_term<double>(q, own, -1.0, gr, gb);
-        } else {
-            C up_s[8];
-            load_cs(cs, (int64_t)v, up_s);
+    for (int q0 = 1; q0 < Q; q0 += G) {
+        C up_s[G][8];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const uint32_t v = src[q0 + j - 1];
+            load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, up_s[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
             double up[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) up[k] = (double)up_s[k];
-            colour_pull_term<double>(q, up, 1.0, gr, gb);
+            for (int k = 0; k < 8; ++k) up[k] = (double)up_s[j][k];
+            double gr, gb;
+            colour_pull_term<double>(q0 + j, up, (src[q0 + j - 1] & BB_FLAG) ? -1.0 : 1.0, gr, gb);
+            rr += gr;
+            rb += gb;
         }
-        rr += gr;
-        rb += gb;
     }
     col[n] = (C)rr;
     col[P + n] = (C)rb;
@@ -402,7 +417,7 @@ namespace {
 
 template <typename ST>
 CgArgs<ST> make_cg_args(const lbm_cg_params &p, const double *visc, const double *u_i, const void *src, void *dst,
-                        const uint32_t *nbr, void *col, void *psi, void *cs, void *u_aux, const void *rho_aux,
+                        const NbrDev &nbr, void *col, void *psi, void *cs, void *u_aux, const void *rho_aux,
                         int64_t N, int64_t P)
 {
     using C = typename Store<ST>::C;
@@ -453,7 +468,8 @@ void CgSolver::release()
     (void)hipSetDevice(device_);
     for (void *&p : pops_)
         if (p) (void)hipFree(p), p = nullptr;
-    void **bufs[] = {(void **)&d_nbr_, (void **)&d_supinv_, &d_col_, &d_psi_, &d_cs_, &d_u_, &d_rho_,
+    nbr_.release();
+    void **bufs[] = {(void **)&d_supinv_, &d_col_, &d_psi_, &d_cs_, &d_u_, &d_rho_,
                      (void **)&d_partial_, (void **)&d_result_};
     for (void **b : bufs)
         if (*b) (void)hipFree(*b), *b = nullptr;
@@ -492,7 +508,6 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
     const int64_t P = geo_.P;
     grid_ = (geo_.N + BLOCK - 1) / BLOCK;
     for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * P * st));
-    LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * P * sizeof(uint32_t)));
     LBM_HIP_TRY(hipMalloc(&d_supinv_, (size_t)P * sizeof(int32_t)));
     LBM_HIP_TRY(hipMalloc(&d_col_, (size_t)2 * P * ct_bytes_));
     LBM_HIP_TRY(hipMalloc(&d_psi_, (size_t)P * ct_bytes_));
@@ -502,8 +517,10 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
     LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)grid_ * CG_NDIAG * sizeof(double)));
     LBM_HIP_TRY(hipMalloc(&d_result_, 32 * sizeof(double)));
     LBM_HIP_TRY(hipHostMalloc((void **)&h_result_, 32 * sizeof(double), hipHostMallocDefault));
-    LBM_HIP_TRY(hipMemcpyAsync(d_nbr_, geo_.nbr.data(), geo_.nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               stream_));
+    PackedNbr packed;
+    rc = pack_neighbours(geo_, packed, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    LBM_HIP_TRY(nbr_.upload(packed, P, stream_));
     LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
@@ -558,11 +575,11 @@ int CgSolver::load_state(const double *f, const double *u, const double *rho, co
     if (e == hipSuccess) {
         dim3 g((unsigned)grid_), b(BLOCK);
         if (prm_.precision == LBM_FP64) {
-            auto a = make_cg_args<double>(prm_, visc_, u_i_, nullptr, pops_[cur_], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+            auto a = make_cg_args<double>(prm_, visc_, u_i_, nullptr, pops_[cur_], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                           d_rho_, N, geo_.P);
             cg_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (double *)d_rho_);
         } else {
-            auto a = make_cg_args<float>(prm_, visc_, u_i_, nullptr, pops_[cur_], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+            auto a = make_cg_args<float>(prm_, visc_, u_i_, nullptr, pops_[cur_], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                          d_rho_, N, geo_.P);
             cg_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (float *)d_rho_);
         }
@@ -605,20 +622,20 @@ int CgSolver::launch_step(bool store_u)
     dim3 g((unsigned)grid_), b(BLOCK);
     hipError_t e;
     if (prm_.precision == LBM_FP64) {
-        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                       d_rho_, N, P);
         e = cg_launch_collide(a, mode, store_u, grid_, stream_);
         if (e == hipSuccess) {
-            cg_colour_kernel<double><<<g, b, 0, stream_>>>(d_nbr_, (const double *)d_cs_, (double *)d_col_,
+            cg_colour_kernel<double><<<g, b, 0, stream_>>>(nbr_.view(), (const double *)d_cs_, (double *)d_col_,
                                                             (double *)d_psi_, N, P);
             e = hipGetLastError();
         }
     } else {
-        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                      d_rho_, N, P);
         e = cg_launch_collide(a, mode, store_u, grid_, stream_);
         if (e == hipSuccess) {
-            cg_colour_kernel<float><<<g, b, 0, stream_>>>(d_nbr_, (const float *)d_cs_, (float *)d_col_,
+            cg_colour_kernel<float><<<g, b, 0, stream_>>>(nbr_.view(), (const float *)d_cs_, (float *)d_col_,
                                                            (float *)d_psi_, N, P);
             e = hipGetLastError();
         }
@@ -664,7 +681,7 @@ int CgSolver::get_state(double *f, double *u, double *rho, double *psi, double *
     dim3 g((unsigned)grid_), b(BLOCK);
     hipError_t e;
     if (prm_.precision == LBM_FP64) {
-        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                       d_rho_, N, geo_.P);
         if (mode == K_INIT)
             cg_get_state_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
@@ -673,7 +690,7 @@ int CgSolver::get_state(double *f, double *u, double *rho, double *psi, double *
         else
             cg_get_state_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
     } else {
-        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                      d_rho_, N, geo_.P);
         if (mode == K_INIT)
             cg_get_state_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
@@ -714,7 +731,7 @@ int CgSolver::diagnostics(const int32_t *box, lbm_cg_diag *out)
     dim3 g((unsigned)grid_), b(BLOCK);
     const int64_t N = geo_.N;
     if (prm_.precision == LBM_FP64) {
-        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                       d_rho_, N, geo_.P);
         if (mode == K_INIT)
             cg_diag_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
@@ -723,7 +740,7 @@ int CgSolver::diagnostics(const int32_t *box, lbm_cg_diag *out)
         else
             cg_diag_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);
     } else {
-        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, d_nbr_, d_col_, d_psi_, d_cs_, d_u_,
+        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                      d_rho_, N, geo_.P);
         if (mode == K_INIT)
             cg_diag_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, d_supinv_, cb, d_partial_);

@@ -6,6 +6,7 @@
 // node and bounces back in place when that node is solid (SP:1874-1926), the device kernels
 // pull: population q of node x comes from x - e_q, or from the opposite population of x
 // itself when x - e_q is solid.  Both give the same post-streaming state.
+#include <algorithm>
 #include <cstring>
 
 #include "handle.h"
@@ -60,6 +61,105 @@ int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid, SparseGe
         }
     }
     return LBM_OK;
+}
+
+int pack_neighbours(const SparseGeometry &g, PackedNbr &out, std::string &err)
+{
+    const int64_t P = g.P, N = g.N, W = P / 64;
+    out.res.assign((size_t)(18 * P), RES_BB);
+    out.hdr.assign((size_t)(W * HDR_WORDS), 0u);
+    std::vector<uint8_t> is_wide((size_t)W, 0);
+    // bases and the compact/wide decision per wave
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < W; ++w) {
+        const int64_t n0 = w * 64, n1 = std::min(n0 + 64, N);
+        uint32_t *h = &out.hdr[(size_t)(w * HDR_WORDS)];
+        bool wide = false;
+        for (int q = 0; q < 18; ++q) {
+            uint32_t lo = 0xFFFFFFFFu, hi = 0;
+            for (int64_t n = n0; n < n1; ++n) {
+                const uint32_t v = g.nbr[(size_t)(q * P + n)];
+                if (v & BB_FLAG) continue;
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+            if (lo == 0xFFFFFFFFu) lo = hi = 0;   // no fluid upstream in this direction
+            h[q] = lo;
+            if (hi - lo >= (uint32_t)RES_BB) wide = true;
+        }
+        is_wide[(size_t)w] = wide;
+    }
+    int64_t nw = 0;
+    for (int64_t w = 0; w < W; ++w) {
+        uint32_t *h = &out.hdr[(size_t)(w * HDR_WORDS)];
+        h[HDR_WIDE] = is_wide[(size_t)w] ? (uint32_t)nw++ : HDR_COMPACT;
+        h[HDR_WIDE + 1] = 0;
+    }
+    out.n_wide = nw;
+    out.wide.assign((size_t)(nw * 18 * 64), 0u);
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < W; ++w) {
+        const int64_t n0 = w * 64, n1 = std::min(n0 + 64, N);
+        const uint32_t *h = &out.hdr[(size_t)(w * HDR_WORDS)];
+        for (int q = 0; q < 18; ++q)
+            for (int64_t n = n0; n < n1; ++n) {
+                const uint32_t v = g.nbr[(size_t)(q * P + n)];
+                if (h[HDR_WIDE] != HDR_COMPACT)
+                    out.wide[(size_t)(((int64_t)h[HDR_WIDE] * 18 + q) * 64 + (n - n0))] = v;
+                else if (!(v & BB_FLAG))
+                    out.res[(size_t)(q * P + n)] = (uint16_t)(v - h[q]);
+            }
+    }
+    // decode everything as the device does and compare
+    int64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+    for (int64_t n = 0; n < N; ++n) {
+        const uint32_t *h = &out.hdr[(size_t)((n >> 6) * HDR_WORDS)];
+        for (int q = 0; q < 18; ++q) {
+            uint32_t v;
+            if (h[HDR_WIDE] == HDR_COMPACT) {
+                const uint16_t r = out.res[(size_t)(q * P + n)];
+                v = r == RES_BB ? ((uint32_t)n | BB_FLAG) : h[q] + r;
+            } else {
+                v = out.wide[(size_t)(((int64_t)h[HDR_WIDE] * 18 + q) * 64 + (n & 63))];
+            }
+            bad += v != g.nbr[(size_t)(q * P + n)];
+        }
+    }
+    if (bad) {
+        err = "neighbour table packing failed its decode check (" + std::to_string(bad) + " entries)";
+        return LBM_ERR_STATE;
+    }
+    return LBM_OK;
+}
+
+hipError_t DeviceNbr::upload(const PackedNbr &p, int64_t P, hipStream_t s)
+{
+    release();
+    n_wide = p.n_wide;
+    const size_t b_res = (size_t)18 * P * sizeof(uint16_t), b_hdr = p.hdr.size() * sizeof(uint32_t);
+    const size_t b_wide = std::max<size_t>(p.wide.size(), 1) * sizeof(uint32_t);
+    bytes = b_res + b_hdr + b_wide;
+    hipError_t e;
+    if ((e = hipMalloc(&res, b_res)) != hipSuccess) return e;
+    if ((e = hipMalloc(&hdr, b_hdr)) != hipSuccess) return e;
+    if ((e = hipMalloc(&wide, b_wide)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(res, p.res.data(), b_res, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(hdr, p.hdr.data(), b_hdr, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if (!p.wide.empty() &&
+        (e = hipMemcpyAsync(wide, p.wide.data(), p.wide.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s)) !=
+            hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+void DeviceNbr::release()
+{
+    if (res) (void)hipFree(res), res = nullptr;
+    if (hdr) (void)hipFree(hdr), hdr = nullptr;
+    if (wide) (void)hipFree(wide), wide = nullptr;
+    n_wide = 0;
+    bytes = 0;
 }
 
 }  // namespace lbm

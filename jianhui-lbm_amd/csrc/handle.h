@@ -44,6 +44,43 @@ int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid_ijk, Spar
 // Round up to a multiple of the block size so every direction row starts 256-byte aligned.
 inline int64_t padded_stride(int64_t n) { return ((n + 255) / 256) * 256; }
 
+// Device form of the neighbour lists (decoded by load_nbr in nbrpack.h): per wavefront of 64
+// node ids and direction, a 32-bit base and 16-bit residuals; waves whose spread does not fit
+// keep their full words in a side table.
+constexpr int HDR_WORDS = 20;                  // per wave: 18 bases, wide slot, pad
+constexpr int HDR_WIDE = 18;
+constexpr uint16_t RES_BB = 0xFFFF;            // residual of a bounce-back link
+constexpr uint32_t HDR_COMPACT = 0xFFFFFFFFu;  // wide slot of a compact wave
+
+struct PackedNbr {
+    std::vector<uint16_t> res;    // 18 x P
+    std::vector<uint32_t> hdr;    // (P / 64) x HDR_WORDS
+    std::vector<uint32_t> wide;   // n_wide x 18 x 64
+    int64_t n_wide = 0;
+};
+
+// Pack g.nbr, then decode every entry and compare with the full table (LBM_ERR_STATE if any
+// differs, which would be a packing bug).
+int pack_neighbours(const SparseGeometry &g, PackedNbr &out, std::string &err);
+
+struct NbrDev {
+    const uint16_t *__restrict__ res;   // 18 x P
+    const uint32_t *__restrict__ hdr;   // (P / 64) x HDR_WORDS
+    const uint32_t *__restrict__ wide;  // n_wide x 18 x 64
+};
+
+// Owner of the packed table in device memory.
+struct DeviceNbr {
+    uint16_t *res = nullptr;
+    uint32_t *hdr = nullptr, *wide = nullptr;
+    int64_t n_wide = 0;
+    size_t bytes = 0;
+    // allocate and queue the copies on s; p must stay alive until s has been synchronised
+    hipError_t upload(const PackedNbr &p, int64_t P, hipStream_t s);
+    void release();
+    NbrDev view() const { return NbrDev{res, hdr, wide}; }
+};
+
 enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
 
 class SpSolver : public lbm_handle {
@@ -75,7 +112,7 @@ private:
     size_t st_bytes_ = 8, ct_bytes_ = 8;
     void *pops_[2] = {nullptr, nullptr};
     int cur_ = 0;
-    uint32_t *d_nbr_ = nullptr;
+    DeviceNbr nbr_;
     int32_t *d_supinv_ = nullptr;
     void *d_u_ = nullptr;     // 3 x P compute-type velocity of the previous step
     void *d_rho_ = nullptr;   // P compute-type density (loaded states only)
@@ -118,7 +155,7 @@ private:
     size_t ct_bytes_ = 8;
     void *pops_[2] = {nullptr, nullptr};
     int cur_ = 0;
-    uint32_t *d_nbr_ = nullptr;
+    DeviceNbr nbr_;
     int32_t *d_supinv_ = nullptr;
     void *d_col_ = nullptr;   // 2 x P: rho_r, rho_b after colour streaming
     void *d_psi_ = nullptr;   // P: psi = (rho_r - rho_b)/(rho_r + rho_b)

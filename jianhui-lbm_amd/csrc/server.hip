@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict
     }
     sp_relax(m, mc, gx, gy, gz, (C)sl.sv, (C)sl.sq, h);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) dst[(int64_t)q * CAP + n] = Store<ST>::put(h[q], 1.0f);
+    for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&dst[(int64_t)q * CAP + n], h[q], 1.0f);
 }
 
 // per-tile flux partials Q[3] and a non-finite flag, from the current (pulled) state

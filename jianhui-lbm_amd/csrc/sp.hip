@@ -9,7 +9,8 @@
 //
 // Data layout in HBM (structure of arrays, row stride P = N rounded up to 256):
 //   pops[2]  19 x P  storage type (double / float / half), deviations f - w*rho0
-//   nbr      18 x P  uint32 upstream node of direction q = 1..18, bit 31 = bounce back
+//   nbr      packed upstream node of direction q = 1..18 or bounce back (nbrpack.h):
+//            18 x P uint16 residuals + per-wave 32-bit bases
 //   u_aux     3 x P  compute type, u of the previous step (for the Error diagnostic)
 //   rho_aux   1 x P  compute type, density of a loaded checkpoint (first step only)
 #include <algorithm>
@@ -29,7 +30,7 @@ struct SpArgs {
     using C = typename Store<ST>::C;
     const ST *__restrict__ src;
     ST *__restrict__ dst;
-    const uint32_t *__restrict__ nbr;
+    NbrDev nbr;
     C *__restrict__ u_aux;
     const C *__restrict__ rho_aux;
     int64_t N, P;
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
     }
     sp_relax(m, mc, a.gx, a.gy, a.gz, a.sv, a.sq, h);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) a.dst[(int64_t)q * a.P + n] = Store<ST>::put(h[q], a.scale);
+    for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&a.dst[(int64_t)q * a.P + n], h[q], a.scale);
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(BLOCK) sp_load_state_kernel(SpArgs<ST> a, cons
 namespace {
 
 template <typename ST>
-SpArgs<ST> make_args(const void *src, void *dst, const uint32_t *nbr, void *u_aux, const void *rho_aux, int64_t N,
+SpArgs<ST> make_args(const void *src, void *dst, const NbrDev &nbr, void *u_aux, const void *rho_aux, int64_t N,
                      int64_t P, const lbm_sp_params &p, double sv, double sq, double rho_i, const double *u_i,
                      float scale, float inv_scale)
 {
@@ -285,7 +286,7 @@ int SpSolver::release()
     if (device_ >= 0) (void)hipSetDevice(device_);
     for (void *&p : pops_)
         if (p) (void)hipFree(p), p = nullptr;
-    if (d_nbr_) (void)hipFree(d_nbr_), d_nbr_ = nullptr;
+    nbr_.release();
     if (d_supinv_) (void)hipFree(d_supinv_), d_supinv_ = nullptr;
     if (d_u_) (void)hipFree(d_u_), d_u_ = nullptr;
     if (d_rho_) (void)hipFree(d_rho_), d_rho_ = nullptr;
@@ -329,20 +330,21 @@ int SpSolver::create(const lbm_sp_params &p, const int32_t *solid)
     const int64_t P = geo_.P;
     grid_ = (geo_.N + BLOCK - 1) / BLOCK;
     for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * P * st_bytes_));
-    LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * P * sizeof(uint32_t)));
     LBM_HIP_TRY(hipMalloc(&d_supinv_, (size_t)P * sizeof(int32_t)));
     LBM_HIP_TRY(hipMalloc(&d_u_, (size_t)3 * P * ct_bytes_));
     LBM_HIP_TRY(hipMalloc(&d_rho_, (size_t)P * ct_bytes_));
     LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)grid_ * (NDIAG + 1) * sizeof(double)));
     LBM_HIP_TRY(hipMalloc(&d_result_, 32 * sizeof(double)));
     LBM_HIP_TRY(hipHostMalloc((void **)&h_result_, 32 * sizeof(double), hipHostMallocDefault));
-    LBM_HIP_TRY(hipMemcpyAsync(d_nbr_, geo_.nbr.data(), geo_.nbr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               stream_));
+    PackedNbr packed;
+    rc = pack_neighbours(geo_, packed, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    LBM_HIP_TRY(nbr_.upload(packed, P, stream_));
     LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
-    // the host copy of the neighbour table is no longer needed
+    // the host copies of the neighbour table are no longer needed
     std::vector<uint32_t>().swap(geo_.nbr);
     const double zero[3] = {0, 0, 0};
     return init(1.0, zero);
@@ -378,15 +380,15 @@ int SpSolver::load_state(const double *f, const double *u, const double *rho)
         const double zero[3] = {0, 0, 0};
         dim3 g((unsigned)grid_), b(BLOCK);
         if (prm_.precision == LBM_FP64) {
-            auto a = make_args<double>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+            auto a = make_args<double>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                        scale_, inv_scale_);
             sp_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, (double *)d_rho_);
         } else if (prm_.precision == LBM_FP32) {
-            auto a = make_args<float>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+            auto a = make_args<float>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                       scale_, inv_scale_);
             sp_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
         } else {
-            auto a = make_args<__half>(nullptr, pops_[nxt], d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
+            auto a = make_args<__half>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                        scale_, inv_scale_);
             sp_load_state_kernel<__half><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
         }
@@ -408,15 +410,15 @@ int SpSolver::launch_step(bool store_u)
     const int nxt = cur_ ^ 1;
     hipError_t e;
     if (prm_.precision == LBM_FP64)
-        e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+        e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
                           mode, store_u, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
-        e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+        e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
                           mode, store_u, grid_, stream_);
     else
-        e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
+        e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
                           mode, store_u, grid_, stream_);
     LBM_HIP_TRY(e);
@@ -460,15 +462,15 @@ int SpSolver::get_state(double *f, double *u, double *rho)
     const int mode = kmode_of(mode_);
     hipError_t e;
     if (prm_.precision == LBM_FP64)
-        e = launch_get_t(make_args<double>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+        e = launch_get_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                            u_i_, scale_, inv_scale_),
                          mode, df, du, dr, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
-        e = launch_get_t(make_args<float>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+        e = launch_get_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                           u_i_, scale_, inv_scale_),
                          mode, df, du, dr, grid_, stream_);
     else
-        e = launch_get_t(make_args<__half>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
+        e = launch_get_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                            u_i_, scale_, inv_scale_),
                          mode, df, du, dr, grid_, stream_);
     if (e == hipSuccess && f) e = hipMemcpyAsync(f, df, (size_t)N * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
@@ -502,15 +504,15 @@ int SpSolver::diagnostics(const int32_t *box, lbm_sp_diag *out)
     const int mode = kmode_of(mode_);
     hipError_t e;
     if (prm_.precision == LBM_FP64)
-        e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+        e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                             rho_i_, u_i_, scale_, inv_scale_),
                           mode, d_supinv_, db, d_partial_, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
-        e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+        e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                            rho_i_, u_i_, scale_, inv_scale_),
                           mode, d_supinv_, db, d_partial_, grid_, stream_);
     else
-        e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, d_nbr_, d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+        e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                             rho_i_, u_i_, scale_, inv_scale_),
                           mode, d_supinv_, db, d_partial_, grid_, stream_);
     LBM_HIP_TRY(e);
