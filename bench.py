@@ -11,9 +11,10 @@ FP64 CPU oracle timed on this host's cores on a bounded sample of the same workl
 
 The same line carries the serving figures ("serving"): a seeded Poisson stream of single-phase
 permeability requests on thresholded Gaussian-random-field subsamples (edge 48..160 lu,
-correlation length 6 lu, porosity 0.15..0.45, FP32 populations, FP64 reductions, stop when k
-changes by < 1e-6 relative over 100 steps) served by the continuous-batching server, and the
-same stream served by sequential plain solves; permeabilities per hour and p50/p99 latency.
+correlation length 6 lu, porosity 0.15..0.45, FP64 reductions, stop when k changes by < 1e-6
+relative over 100 steps) served by the continuous-batching server (FP32 arena), and the same stream served by
+sequential plain FP32 solves;
+permeabilities per hour, p50/p99 latency, and each served k against the sequential FP32 one.
 """
 import argparse
 import importlib.util
@@ -45,10 +46,11 @@ def load_package():
     return mod
 
 
-def read_traffic(precision):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
-    the same counters on a copy kernel of known size), if a summary has been committed."""
-    path = os.path.join(ROOT, "profiles", f"sp_step_{precision}_256cube_traffic.md")
+def read_traffic(tag):
+    """HBM bytes per launch (per step for the two-kernel colour-gradient step) measured with
+    rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy kernel of
+    known size), if a summary for this kernel and workload has been committed."""
+    path = os.path.join(ROOT, "profiles", f"{tag}_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -86,20 +88,25 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=20000):
     nu, g, tol, interval = 1.0 / 6.0, (1e-6, 0.0, 0.0), 1e-6, 100
     nodes = [int(np.count_nonzero(s == 0)) for s in samples]
 
-    # continuous batching
-    with pkg.PermeabilityServer(check_interval=interval, tol=tol, max_steps=max_steps, block=0) as srv:
-        t0 = srv.clock()
-        tickets = []
-        for i in range(nreq):
-            while srv.clock() - t0 < arrivals[i]:
-                time.sleep(min(0.002, max(0.0, arrivals[i] - (srv.clock() - t0))))
-            tickets.append(srv.submit(samples[i], nu, g))
-        srv.wait_idle()
-        res = {r["ticket"]: r for r in srv.poll(4 * nreq)}
-        st = srv.stats()
-    srv_res = [res[t] for t in tickets]
-    srv_lat = np.array([r["t_done"] - r["t_submit"] for r in srv_res])
-    srv_span = max(r["t_done"] for r in srv_res) - min(r["t_submit"] for r in srv_res)
+    def serve(precision):
+        """The stream through the continuous-batching server with the given arena precision."""
+        with pkg.PermeabilityServer(check_interval=interval, tol=tol, max_steps=max_steps, block=0,
+                                    precision=precision) as srv:
+            t0 = srv.clock()
+            tickets = []
+            for i in range(nreq):
+                while srv.clock() - t0 < arrivals[i]:
+                    time.sleep(min(0.002, max(0.0, arrivals[i] - (srv.clock() - t0))))
+                tickets.append(srv.submit(samples[i], nu, g))
+            srv.wait_idle()
+            res = {r["ticket"]: r for r in srv.poll(4 * nreq)}
+            st = srv.stats()
+        rs = [res[t] for t in tickets]
+        lat = np.array([r["t_done"] - r["t_submit"] for r in rs])
+        span = max(r["t_done"] for r in rs) - min(r["t_submit"] for r in rs)
+        return rs, lat, span, st
+
+    runs = {p: serve(p) for p in ("fp32",)}
 
     # sequential plain solves, FIFO on the same arrival schedule
     seq_lat, seq_k = [], []
@@ -127,54 +134,33 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=20000):
         seq_lat.append(time.perf_counter() - t0 - arrivals[i])
     seq_span = time.perf_counter() - t0
     seq_lat = np.array(seq_lat)
-    k_srv = np.array([r["k"][0] for r in srv_res])
     k_seq = np.array(seq_k)
     ok = np.isfinite(k_seq) & (np.abs(k_seq) > 0)
-    k_dev = float(np.max(np.abs(k_srv[ok] - k_seq[ok]) / np.abs(k_seq[ok]))) if ok.any() else None
-    return {
-        "requests": nreq, "arrival_rate_per_s": rate, "seed": seed,
-        "fluid_nodes_total": int(sum(nodes)),
-        "perms_per_hour": round(nreq / srv_span * 3600.0, 1),
-        "p50_latency_s": round(float(np.percentile(srv_lat, 50)), 4),
-        "p99_latency_s": round(float(np.percentile(srv_lat, 99)), 4),
-        "sequential_perms_per_hour": round(nreq / seq_span * 3600.0, 1),
-        "sequential_p50_latency_s": round(float(np.percentile(seq_lat, 50)), 4),
-        "sequential_p99_latency_s": round(float(np.percentile(seq_lat, 99)), 4),
-        "speedup_vs_sequential": round(seq_span / srv_span, 2),
-        "converged": int(sum(1 for r in srv_res if r["converged"] == 1)),
-        "max_rel_k_diff_vs_sequential": k_dev,
-        "tile": st["block"], "server_device_busy_s": round(st["busy_seconds"], 3),
-        "server_mlups": round(st["node_updates"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
-    }
 
+    def summary(precision):
+        rs, lat, span, st = runs[precision]
+        k_srv = np.array([r["k"][0] for r in rs])
+        dev = float(np.max(np.abs(k_srv[ok] - k_seq[ok]) / np.abs(k_seq[ok]))) if ok.any() else None
+        return {
+            "arena_precision": precision,
+            "perms_per_hour": round(nreq / span * 3600.0, 1),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "speedup_vs_sequential": round(seq_span / span, 2),
+            "converged": int(sum(1 for r in rs if r["converged"] == 1)),
+            "max_rel_k_diff_vs_sequential_fp32": dev,
+            "tile": st["block"], "server_device_busy_s": round(st["busy_seconds"], 3),
+            "server_mlups": round(st["node_updates"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
+        }
 
-def cg_bench(pkg, steps, warmup, precision, edge=192):
-    """Colour-gradient workload: 192^3 sphere pack (r = 6, porosity 0.45), S_w = 0.5 by voxel
-    (seeded), nu_l = nu_g = 1/6, CapA = 0.01, cos theta = 0, g = 1e-5."""
-    geo = pkg.geometry
-    solid = geo.sphere_pack((edge, edge, edge), 6.0, 0.45, seed=2025)
-    psi = np.where(np.random.default_rng(2026).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
-    s = pkg.ColorGradient(solid, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), precision=precision)
-    s.init(psi)
-    s.step(warmup)
-    s.diagnostics()
-    t0 = time.perf_counter()
-    ms = s.step_timed(steps)
-    wall = time.perf_counter() - t0
-    d = s.diagnostics()
-    N = s.count
-    b_alg = 2 * 21 * (8 if precision == "fp64" else 4) + 18 * 4   # f + rho_r + rho_b read and written, indices
-    achieved = b_alg * N / (ms / 1e3 / steps) / 1e9
-    out = {"metric": "fluid MLUPS (colour gradient)", "value": round(N * steps / wall / 1e6, 1), "unit": "MLUPS",
-           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(wall * 1e3 / steps, 5),
-           "higher_is_better": True, "dtype": DTYPE[precision], "data": "synthetic",
-           "config": {"workload": f"cg_d3q19_mrt_{edge}cube_spheres_r6_phi0.45_sw0.5", "fluid_nodes": N,
-                      "precision": precision},
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_node": b_alg,
-                        "kernels": "cg_collide_kernel + cg_colour_kernel per step"},
-           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"]}
-    s.close()
+    head = summary("fp32")
+    out = {"requests": nreq, "arrival_rate_per_s": rate, "seed": seed, "fluid_nodes_total": int(sum(nodes)),
+           "stop_rule": f"|k_n - k_(n-{interval})| <= {tol} |k_n|, same for server and sequential solves"}
+    out.update(head)
+    out["sequential_plain_fp32"] = {
+        "perms_per_hour": round(nreq / seq_span * 3600.0, 1),
+        "p50_latency_s": round(float(np.percentile(seq_lat, 50)), 4),
+        "p99_latency_s": round(float(np.percentile(seq_lat, 99)), 4)}
     return out
 
 
@@ -222,7 +208,7 @@ def main():
     value = N * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = B_ALG[args.precision] * N / avg_launch_s / 1e9
-    traffic = read_traffic(args.precision)
+    traffic = read_traffic(f"sp_step_{args.precision}_256cube") if edge == 256 else None
     out = {
         "metric": "fluid MLUPS",
         "value": round(value, 1),

@@ -158,7 +158,7 @@ typedef struct lbm_srv_params {
     double tol;               /* relative permeability change that retires a sample, default 1e-6 */
     int32_t min_steps;        /* never retire before this many steps (default 2*check_interval) */
     int32_t max_steps;        /* retire unconverged after this many steps (default 200000) */
-    int32_t precision;        /* LBM_FP32 (default) or LBM_FP64 */
+    int32_t precision;        /* LBM_FP32 (default) or LBM_FP64; LBM_FP16 is refused (LBM_ERR_ARG) */
     int32_t block;            /* nodes per workgroup tile: 128, 256 or 512; 0 = autotune at create */
     int32_t device;
 } lbm_srv_params;

@@ -24,7 +24,7 @@ def _samples(pkg):
 
 
 def _run(srv, reqs):
-    tickets = [srv.submit(s, nu, g) for (s, nu, g) in reqs]
+    tickets = [srv.submit(*r) for r in reqs]
     assert srv.wait_idle(120.0)
     res = {r["ticket"]: r for r in srv.poll()}
     return [res[t] for t in tickets]
@@ -92,6 +92,26 @@ def test_matches_plain_solver(pkg):
     assert abs(r["k"][0] - k64[0]) <= 1e-4 * abs(k64[0])
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_matches_plain_solver_with_short_and_degenerate_axes(pkg, precision):
+    """The arena's 16-bit upstream deltas wrap modulo the sample's node count: a box one voxel
+    thick in x (every x link is a self link) and a box with a short z axis give the plain
+    solver's answer."""
+    _log(f"[gpu] serve: degenerate axes, {precision}")
+    rng = np.random.default_rng(31)
+    thin = (rng.uniform(size=(1, 24, 30)) < 0.3).astype(np.int32)
+    short = (rng.uniform(size=(20, 18, 3)) < 0.25).astype(np.int32)
+    for s in (thin, short):
+        with pkg.PermeabilityServer(capacity_nodes=100_000, check_interval=50, tol=1e-6, block=256,
+                                    precision=precision) as srv:
+            r = _run(srv, [(s, 1.0 / 6.0, (0.0, 1e-5, 0.0), 2)])[0]
+        assert r["converged"] == 1
+        sp = pkg.SinglePhase(s, 1.0 / 6.0, (0.0, 1e-5, 0.0), precision=precision)
+        sp.step(r["steps"])
+        k, _, _ = sp.Comput_Perm(2)
+        assert abs(r["k"][1] - k[1]) <= 1e-6 * abs(k[1])
+
+
 def test_rejects_bad_requests(pkg):
     with pkg.PermeabilityServer(capacity_nodes=50_000, block=256) as srv:
         with pytest.raises(pkg.LBMError):
@@ -100,6 +120,11 @@ def test_rejects_bad_requests(pkg):
             srv.submit(np.zeros((8, 8, 8), np.int32), 0.1, (0.0, 1e-5, 0))    # no force along x
         with pytest.raises(pkg.LBMError):
             srv.submit(np.zeros((64, 64, 64), np.int32), 0.1, (1e-5, 0, 0))   # larger than the arena
+    with pkg.PermeabilityServer(capacity_nodes=200_000, block=256) as srv:
+        with pytest.raises(pkg.LBMError):
+            srv.submit(np.zeros((3, 190, 190), np.int32), 0.1, (1e-5, 0, 0))  # x-plane of 36100 nodes
+    with pytest.raises(pkg.LBMError):
+        pkg.PermeabilityServer(capacity_nodes=50_000, block=256, precision="fp16")
 
 
 def test_autotuned_block(pkg):
