@@ -161,6 +161,19 @@ typedef struct lbm_srv_params {
     int32_t precision;        /* LBM_FP32 (default) or LBM_FP64; LBM_FP16 is refused (LBM_ERR_ARG) */
     int32_t block;            /* nodes per workgroup tile: 128, 256 or 512; 0 = autotune at create */
     int32_t device;
+    /* Anderson acceleration of each sample's steady state (no counterpart in the reference,
+     * which only time-steps): every accel_period steps the K-step map x -> g(x) of the sample
+     * is extrapolated from its last accel_depth residuals, x <- g - dG gamma with gamma the
+     * least-squares fit of the newest residual by the residual differences.  The sample then
+     * retires when k changes by <= tol relative between two accelerations. */
+    int32_t accel_depth;      /* history depth m, 0 = plain time stepping, at most 8 */
+    int32_t accel_period;     /* steps between accelerations, a multiple of check_interval (0 = 2*check_interval) */
+    int32_t accel_warmup;     /* plain steps before the first acceleration, multiple of check_interval (0 = 5*check_interval) */
+    /* Precision escalation of an FP32 server: a sample whose k (lattice units) falls below
+     * escalate_k after the warm-up, or that reaches max_steps unsettled, is solved again from
+     * rest in FP64 by an internal FP64 server (its flux is then too small for FP32 deviations
+     * to resolve: a nearly disconnected pore space).  0 = never. */
+    double escalate_k;
 } lbm_srv_params;
 
 typedef struct lbm_srv_request {
@@ -178,6 +191,8 @@ typedef struct lbm_srv_result {
     int64_t steps;
     int64_t fluid_nodes;
     int32_t converged;        /* 1 converged, 0 stopped at max_steps, -1 non-finite velocity */
+    int32_t accelerations;    /* Anderson extrapolations applied to this sample */
+    int32_t precision;        /* LBM_FP32 or LBM_FP64: the arena the result came from */
     int32_t pad;
     double t_submit, t_admit, t_done;   /* seconds on the server clock */
 } lbm_srv_result;

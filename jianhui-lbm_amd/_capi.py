@@ -52,7 +52,8 @@ class SrvParams(ctypes.Structure):
     _fields_ = [("capacity_nodes", ctypes.c_int64), ("mem_fraction", ctypes.c_double),
                 ("check_interval", ctypes.c_int32), ("tol", ctypes.c_double), ("min_steps", ctypes.c_int32),
                 ("max_steps", ctypes.c_int32), ("precision", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("accel_depth", ctypes.c_int32), ("accel_period", ctypes.c_int32),
+                ("accel_warmup", ctypes.c_int32), ("escalate_k", ctypes.c_double)]
 
 
 class SrvRequest(ctypes.Structure):
@@ -64,7 +65,8 @@ class SrvRequest(ctypes.Structure):
 class SrvResult(ctypes.Structure):
     _fields_ = [("ticket", ctypes.c_int64), ("k", ctypes.c_double * 3), ("rel_change", ctypes.c_double),
                 ("steps", ctypes.c_int64), ("fluid_nodes", ctypes.c_int64), ("converged", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("accelerations", ctypes.c_int32), ("precision", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
                 ("t_done", ctypes.c_double)]
 
 

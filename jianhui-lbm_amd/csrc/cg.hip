@@ -134,7 +134,7 @@ template <typename ST, int MODE, bool STORE_U>
 __global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = stream_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (n >= a.N) return;
     C h[Q], m[Q];
     uint32_t nb[Q - 1];
@@ -229,7 +229,7 @@ template <typename C>
 __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *__restrict__ cs,
                                                           C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P)
 {
-    const int64_t n = xcd_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = xcd_slab_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (n >= N) return;
     C own_s[8];
     load_cs(cs, n, own_s);

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -69,7 +70,7 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict
                                                        const int16_t *__restrict__ nbr, int64_t CAP, int64_t step)
 {
     using C = typename Store<ST>::C;
-    const TileDev t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[stream_tile(blockIdx.x, gridDim.x)];
     const int64_t n = t.begin + threadIdx.x;
     if (n >= t.end) return;
     const SlotDev sl = slots[t.slot];
@@ -144,18 +145,136 @@ __global__ void __launch_bounds__(BLK) srv_flux_kernel(const TileDev *__restrict
     }
 }
 
-// one workgroup per active sample: sum its tiles' partials in tile order
+// one workgroup per active sample: sum its tiles' NV partials in tile order
+template <int NV>
 __global__ void __launch_bounds__(BLOCK) srv_slot_sum_kernel(const double *__restrict__ partial,
                                                              const int32_t *__restrict__ tile_lo,
                                                              const int32_t *__restrict__ tile_hi, double *__restrict__ out)
 {
     const int a = blockIdx.x;
     const int lo = tile_lo[a], hi = tile_hi[a];
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v[NV];
+    bool is_max[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        v[k] = 0.0;
+        is_max[k] = false;
+    }
     for (int t = lo + (int)threadIdx.x; t < hi; t += BLOCK)
-        for (int k = 0; k < 4; ++k) v[k] += partial[(int64_t)t * 4 + k];
-    const bool is_max[4] = {false, false, false, false};
-    block_reduce_store<4>(v, is_max, out + (int64_t)a * 4);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += partial[(int64_t)t * NV + k];
+    block_reduce_store<NV>(v, is_max, out + (int64_t)a * NV);
+}
+
+// ---------------------------------------------------------------------------- Anderson acceleration
+// Per sample, the steady state is the fixed point of the K-step map x -> g(x) (K = the
+// acceleration period; x is the stored post-collision state of the sample's nodes).  Every K
+// steps, with r = g - x:
+//   SAVE    (end of the warm-up)      X = x
+//   FIRST   (one period later)        R_prev = r, G_prev = g, X = g  (plain step)
+//   RESID                             dR_h = r - R_prev, dG_h = g - G_prev into ring slot h,
+//                                     dot products dR_i.dR_h, dR_i.r, r.r for the host
+//   UPDATE  (after the host's solve)  x = g - sum_i gamma_i dG_i, written to the state and to X
+// gamma minimises |r - dR gamma| (Walker & Ni's form of Anderson mixing).  The dot products are
+// reduced per tile and then per sample in tile order, so they are the same whatever shares
+// the launch.
+constexpr int AA_MAX = 8;
+constexpr int AA_NV = 2 * AA_MAX + 1;   // dR_i.dR_head, dR_i.r, r.r
+enum AaMode : int32_t { AA_NONE = 0, AA_SAVE = 1, AA_FIRST = 2, AA_RESID = 3 };
+
+struct AaDev {
+    int32_t mode;
+    int32_t head;    // ring slot written by this RESID
+    int32_t count;   // valid ring slots after this RESID (0..m-1 are candidates)
+    int32_t pad;
+    double gamma[AA_MAX];
+};
+
+template <int BLK, int K>
+__device__ __forceinline__ void tile_reduce(double (&v)[K], double *out)
+{
+    __shared__ double red[BLK / WAVE][K];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double r = wave_sum(v[k]);
+        if (lane == 0) red[wid][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double r = red[0][threadIdx.x];
+        for (int w = 1; w < BLK / WAVE; ++w) r += red[w][threadIdx.x];
+        out[threadIdx.x] = r;
+    }
+}
+
+template <typename ST, int BLK>
+__global__ void __launch_bounds__(BLK) srv_aa_resid_kernel(const TileDev *__restrict__ tiles,
+                                                           const AaDev *__restrict__ aa, const ST *__restrict__ pops,
+                                                           ST *__restrict__ X, ST *__restrict__ RP, ST *__restrict__ GP,
+                                                           ST *__restrict__ DR, ST *__restrict__ DG, int64_t CAP,
+                                                           double *__restrict__ partial)
+{
+    using C = typename Store<ST>::C;
+    const TileDev t = tiles[blockIdx.x];
+    const AaDev a = aa[t.slot];
+    const int64_t n = t.begin + threadIdx.x;
+    const int64_t V = (int64_t)Q * CAP;
+    double acc[AA_NV];
+#pragma unroll
+    for (int k = 0; k < AA_NV; ++k) acc[k] = 0.0;
+    if (a.mode != AA_NONE && n < t.end) {
+        for (int q = 0; q < Q; ++q) {
+            const int64_t i = (int64_t)q * CAP + n;
+            const ST gs = pops[i];
+            if (a.mode == AA_SAVE) {
+                X[i] = gs;
+                continue;
+            }
+            const C g = (C)gs, r = g - (C)X[i];
+            if (a.mode == AA_RESID) {
+                const C dr = r - (C)RP[i], dg = g - (C)GP[i];
+                DR[a.head * V + i] = (ST)dr;
+                DG[a.head * V + i] = (ST)dg;
+                const double drs = (double)(ST)dr;
+#pragma unroll
+                for (int h = 0; h < AA_MAX; ++h) {
+                    if (h < a.count) {
+                        const double v = h == a.head ? drs : (double)DR[h * V + i];
+                        acc[h] += v * drs;
+                        acc[AA_MAX + h] += v * (double)r;
+                    }
+                }
+                acc[2 * AA_MAX] += (double)r * (double)r;
+            } else {
+                X[i] = gs;   // FIRST: the next iterate is g itself
+            }
+            RP[i] = (ST)r;
+            GP[i] = gs;
+        }
+    }
+    tile_reduce<BLK, AA_NV>(acc, partial + (int64_t)blockIdx.x * AA_NV);
+}
+
+template <typename ST, int BLK>
+__global__ void __launch_bounds__(BLK) srv_aa_update_kernel(const TileDev *__restrict__ tiles,
+                                                            const AaDev *__restrict__ aa, ST *__restrict__ pops,
+                                                            ST *__restrict__ X, const ST *__restrict__ DG, int64_t CAP)
+{
+    const TileDev t = tiles[blockIdx.x];
+    const AaDev a = aa[t.slot];
+    const int64_t n = t.begin + threadIdx.x;
+    if (a.mode != AA_RESID || n >= t.end) return;
+    const int64_t V = (int64_t)Q * CAP;
+    for (int q = 0; q < Q; ++q) {
+        const int64_t i = (int64_t)q * CAP + n;
+        double x = (double)pops[i];
+#pragma unroll
+        for (int h = 0; h < AA_MAX; ++h)
+            if (h < a.count) x -= a.gamma[h] * (double)DG[h * V + i];
+        pops[i] = (ST)x;
+        X[i] = (ST)x;
+    }
 }
 
 // ---------------------------------------------------------------------------- host
@@ -164,6 +283,8 @@ using Clock = std::chrono::steady_clock;
 
 struct Prepared {
     int64_t ticket = 0;
+    int64_t prior_steps = 0;     // steps already spent in an FP32 arena before escalation
+    int prior_acc = 0;
     int64_t N = 0, npad = 0;
     std::vector<int16_t> nbr;    // 18 x npad upstream deltas (SRV_BB = bounce back)
     double nu = 0, g[3] = {0, 0, 0};
@@ -180,7 +301,49 @@ struct Active {
     double k_prev = 0;
     bool has_prev = false;
     double t_admit = 0;
+    // Anderson acceleration (host side of the per-sample state)
+    int aa_stage = 0;          // 0 before SAVE, 1 before FIRST, 2 running
+    int64_t aa_next = 0;       // sample step count of the next acceleration event
+    int aa_head = 0, aa_count = 0, aa_updates = 0;
+    double gram[AA_MAX][AA_MAX] = {};
+    double r2_min = INFINITY;
+    double k_ev = 0;           // k at the previous event
+    bool has_k_ev = false;
+    int small_events = 0;      // consecutive events whose k changed by <= tol
 };
+
+// gamma = (G + lambda I)^-1 b for the n x n leading block, Gaussian elimination with partial
+// pivoting; false if the system is singular or the result not finite
+bool solve_small(const double (&G)[AA_MAX][AA_MAX], const double *b, int n, double *x)
+{
+    double A[AA_MAX][AA_MAX + 1];
+    double dmax = 0.0;
+    for (int i = 0; i < n; ++i) dmax = std::max(dmax, std::fabs(G[i][i]));
+    const double lam = 1e-12 * dmax;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) A[i][j] = G[i][j] + (i == j ? lam : 0.0);
+        A[i][n] = b[i];
+    }
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+        if (!(std::fabs(A[p][c]) > 0.0)) return false;
+        if (p != c)
+            for (int j = 0; j <= n; ++j) std::swap(A[p][j], A[c][j]);
+        for (int r = c + 1; r < n; ++r) {
+            const double f = A[r][c] / A[c][c];
+            for (int j = c; j <= n; ++j) A[r][j] -= f * A[c][j];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double v = A[i][n];
+        for (int j = i + 1; j < n; ++j) v -= A[i][j] * x[j];
+        x[i] = v / A[i][i];
+        if (!std::isfinite(x[i])) return false;
+    }
+    return true;
+}
 
 struct Segment {
     int64_t start, len;
@@ -223,9 +386,14 @@ public:
     int stats(lbm_srv_stats *out);
     double clock() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
 
+    // queue an already prepared sample (precision escalation from an FP32 server)
+    void enqueue(Prepared &&pr);
+    bool fits(int64_t npad) const { return npad <= cap_; }
+
 private:
     void shutdown();
     void loop();
+    bool escalate_locked(Active &a);
     int run_chunk(std::unique_lock<std::mutex> &lk);
     bool try_admit_locked();
     int rebuild_tables();
@@ -236,6 +404,8 @@ private:
     hipError_t launch_steps_dispatch(int nsteps);
     template <int BLK>
     hipError_t launch_flux();
+    template <int BLK>
+    hipError_t launch_aa(bool update);
     int autotune();
 
     lbm_srv_params prm_{};
@@ -249,6 +419,11 @@ private:
     void *pops_[2] = {nullptr, nullptr};
     int cur_ = 0;
     int16_t *d_nbr_ = nullptr;
+    // Anderson acceleration: X, R_prev, G_prev, then m dR and m dG vectors, each 19 x CAP
+    int aa_m_ = 0, aa_period_ = 0, aa_warm_ = 0;
+    void *d_aa_vec_ = nullptr;
+    AaDev *d_aa_ = nullptr, *h_aa_ = nullptr;
+    double *d_aa_partial_ = nullptr, *d_aa_sums_ = nullptr, *h_aa_sums_ = nullptr;
     SlotDev *d_slots_ = nullptr;
     TileDev *d_tiles_ = nullptr;
     int32_t *d_tlo_ = nullptr, *d_thi_ = nullptr;
@@ -270,7 +445,50 @@ private:
     std::deque<lbm_srv_result> done_;
     int64_t n_done_ = 0, launches_ = 0, node_updates_ = 0;
     double busy_s_ = 0;
+    // FP64 server for escalated samples, created at the first escalation
+    std::unique_ptr<Server> fallback_;
+    int64_t escalated_ = 0;
 };
+
+void Server::enqueue(Prepared &&pr)
+{
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        pending_.push_back(std::move(pr));
+    }
+    cv_.notify_all();
+}
+
+// Read the sample's neighbour deltas back from the arena and hand the sample to the FP64
+// server; false (the FP32 result stands) if that server cannot be made or cannot hold it.
+bool Server::escalate_locked(Active &a)
+{
+    if (!fallback_) {
+        lbm_srv_params fp = prm_;
+        fp.precision = LBM_FP64;
+        fp.escalate_k = 0.0;
+        fp.block = blk_;
+        fp.capacity_nodes = std::max<int64_t>(int64_t(4) << 20, a.req.npad);
+        fp.mem_fraction = 0.0;
+        auto f = std::unique_ptr<Server>(new (std::nothrow) Server());
+        if (!f) return false;
+        f->t0_ = t0_;   // one clock for submit, admit and done times
+        if (f->create(fp) != LBM_OK) return false;
+        fallback_ = std::move(f);
+    }
+    if (!fallback_->fits(a.req.npad)) return false;
+    a.req.nbr.resize((size_t)18 * a.req.npad);
+    hipError_t e = hipMemcpy2DAsync(a.req.nbr.data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
+                                    (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
+                                    hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return false;
+    a.req.prior_steps += a.steps;
+    a.req.prior_acc += a.aa_updates;
+    fallback_->enqueue(std::move(a.req));
+    ++escalated_;
+    return true;
+}
 
 int64_t Server::alloc_segment(int64_t len)
 {
@@ -313,6 +531,12 @@ int Server::create(const lbm_srv_params &p)
     if (prm_.precision != LBM_FP64) prm_.precision = LBM_FP32;
     if (prm_.block != 0 && prm_.block != 128 && prm_.block != 256 && prm_.block != 512)
         return fail(LBM_ERR_ARG, "block must be 0, 128, 256 or 512");
+    if (prm_.accel_depth < 0 || prm_.accel_depth > AA_MAX) return fail(LBM_ERR_ARG, "accel_depth must be 0..8");
+    aa_m_ = prm_.accel_depth;
+    aa_period_ = prm_.accel_period > 0 ? prm_.accel_period : 2 * prm_.check_interval;
+    aa_warm_ = prm_.accel_warmup > 0 ? prm_.accel_warmup : 5 * prm_.check_interval;
+    if (aa_m_ > 0 && (aa_period_ % prm_.check_interval != 0 || aa_warm_ % prm_.check_interval != 0))
+        return fail(LBM_ERR_ARG, "accel_period and accel_warmup must be multiples of check_interval");
     st_ = prm_.precision == LBM_FP64 ? 8 : 4;
     device_ = p.device;
     LBM_HIP_TRY(hipSetDevice(device_));
@@ -325,7 +549,7 @@ int Server::create(const lbm_srv_params &p)
         LBM_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
         const double frac = p.mem_fraction > 0 && p.mem_fraction < 0.95 ? p.mem_fraction : 0.5;
         const double bytes = std::min(frac * (double)free_b, 64e9);
-        cap = (int64_t)(bytes / (double)(2 * Q * st_ + 18 * sizeof(int16_t)));
+        cap = (int64_t)(bytes / (double)(2 * Q * st_ + 18 * sizeof(int16_t) + (aa_m_ ? (3 + 2 * aa_m_) * Q * st_ : 0)));
     }
     cap = std::min<int64_t>(cap, (int64_t)0x7fffffff);
     cap_ = ((cap + 511) / 512) * 512;
@@ -341,6 +565,14 @@ int Server::create(const lbm_srv_params &p)
     LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)max_tiles_ * 4 * sizeof(double)));
     LBM_HIP_TRY(hipMalloc(&d_sums_, (size_t)max_slots_ * 4 * sizeof(double)));
     LBM_HIP_TRY(hipHostMalloc((void **)&h_sums_, (size_t)max_slots_ * 4 * sizeof(double), hipHostMallocDefault));
+    if (aa_m_ > 0) {
+        LBM_HIP_TRY(hipMalloc(&d_aa_vec_, (size_t)(3 + 2 * aa_m_) * Q * cap_ * st_));
+        LBM_HIP_TRY(hipMalloc(&d_aa_, (size_t)max_slots_ * sizeof(AaDev)));
+        LBM_HIP_TRY(hipHostMalloc((void **)&h_aa_, (size_t)max_slots_ * sizeof(AaDev), hipHostMallocDefault));
+        LBM_HIP_TRY(hipMalloc(&d_aa_partial_, (size_t)max_tiles_ * AA_NV * sizeof(double)));
+        LBM_HIP_TRY(hipMalloc(&d_aa_sums_, (size_t)max_slots_ * AA_NV * sizeof(double)));
+        LBM_HIP_TRY(hipHostMalloc((void **)&h_aa_sums_, (size_t)max_slots_ * AA_NV * sizeof(double), hipHostMallocDefault));
+    }
     free_.push_back({0, cap_});
     blk_ = prm_.block ? prm_.block : 256;
     if (prm_.block == 0) {
@@ -363,10 +595,13 @@ void Server::shutdown()
     for (void *&b : pops_)
         if (b) (void)hipFree(b), b = nullptr;
     void **bufs[] = {(void **)&d_nbr_, (void **)&d_slots_, (void **)&d_tiles_, (void **)&d_tlo_, (void **)&d_thi_,
-                     (void **)&d_partial_, (void **)&d_sums_};
+                     (void **)&d_partial_, (void **)&d_sums_, &d_aa_vec_, (void **)&d_aa_, (void **)&d_aa_partial_,
+                     (void **)&d_aa_sums_};
     for (void **b : bufs)
         if (*b) (void)hipFree(*b), *b = nullptr;
     if (h_sums_) (void)hipHostFree(h_sums_), h_sums_ = nullptr;
+    if (h_aa_) (void)hipHostFree(h_aa_), h_aa_ = nullptr;
+    if (h_aa_sums_) (void)hipHostFree(h_aa_sums_), h_aa_sums_ = nullptr;
     if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
     if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
@@ -411,21 +646,39 @@ int Server::poll(lbm_srv_result *out, int max, int *n_out)
         out[n++] = done_.front();
         done_.pop_front();
     }
+    int rc = LBM_OK;
+    if (fallback_ && n < max) {
+        int m = 0;
+        rc = fallback_->poll(out + n, max - n, &m);
+        n += m;
+        if (rc != LBM_OK) fail(rc, fallback_->last_error);
+    }
     if (n_out) *n_out = n;
     if (loop_error_) return fail(LBM_ERR_HIP, loop_msg_);
-    return LBM_OK;
+    return rc;
 }
 
 int Server::wait_idle(double timeout_s)
 {
+    const auto t_end = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
     std::unique_lock<std::mutex> lk(mu_);
     auto idle = [this] { return loop_error_ || (pending_.empty() && active_.empty()); };
     if (timeout_s > 0) {
-        if (!idle_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), idle)) return LBM_ERR_BUSY;
+        if (!idle_cv_.wait_until(lk, t_end, idle)) return LBM_ERR_BUSY;
     } else {
         idle_cv_.wait(lk, idle);
     }
     if (loop_error_) return fail(LBM_ERR_HIP, loop_msg_);
+    // no sample is left here to escalate: wait for the FP64 server's share
+    if (fallback_) {
+        double rest = 0.0;
+        if (timeout_s > 0) {
+            rest = std::chrono::duration<double>(t_end - Clock::now()).count();
+            if (rest <= 0) return LBM_ERR_BUSY;
+        }
+        const int rc = fallback_->wait_idle(rest);
+        if (rc != LBM_OK) return rc == LBM_ERR_BUSY ? rc : fail(rc, fallback_->last_error);
+    }
     return LBM_OK;
 }
 
@@ -444,6 +697,16 @@ int Server::stats(lbm_srv_stats *out)
     out->block = blk_;
     out->pad = 0;
     out->busy_seconds = busy_s_;
+    if (fallback_) {
+        lbm_srv_stats f{};
+        fallback_->stats(&f);
+        out->pending += f.pending;
+        out->active += f.active;
+        out->done += f.done;
+        out->launches += f.launches;
+        out->node_updates += f.node_updates;
+        out->busy_seconds += f.busy_seconds;
+    }
     return LBM_OK;
 }
 
@@ -464,6 +727,7 @@ bool Server::try_admit_locked()
         a.base = base;
         a.first_step = global_step_;
         a.t_admit = clock();
+        a.aa_next = aa_warm_;
         // sample-local neighbour rows go straight into the arena rows of the segment
         const hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.data(),
                                               (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t),
@@ -576,6 +840,35 @@ hipError_t Server::launch_flux()
 
 // Chunk of check_interval steps for every active sample, then the permeability check.
 // Called with the lock held; the lock is released while the device works.
+template <int BLK>
+hipError_t Server::launch_aa(bool update)
+{
+    const size_t V = (size_t)Q * cap_ * st_;
+    char *base = (char *)d_aa_vec_;
+    void *X = base, *RP = base + V, *GP = base + 2 * V, *DR = base + 3 * V, *DG = base + (3 + aa_m_) * V;
+    if (prm_.precision == LBM_FP64) {
+        if (update)
+            srv_aa_update_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_aa_, (double *)pops_[cur_], (double *)X, (const double *)DG, cap_);
+        else
+            srv_aa_resid_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_aa_, (const double *)pops_[cur_], (double *)X, (double *)RP, (double *)GP, (double *)DR,
+                (double *)DG, cap_, d_aa_partial_);
+    } else {
+        if (update)
+            srv_aa_update_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_aa_, (float *)pops_[cur_], (float *)X, (const float *)DG, cap_);
+        else
+            srv_aa_resid_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+                d_tiles_, d_aa_, (const float *)pops_[cur_], (float *)X, (float *)RP, (float *)GP, (float *)DR,
+                (float *)DG, cap_, d_aa_partial_);
+    }
+    return hipGetLastError();
+}
+
+// Chunk of check_interval steps for every active sample, then the permeability check and the
+// acceleration events that fall on this chunk.  Called with the lock held; the lock is released
+// while the device works.
 int Server::run_chunk(std::unique_lock<std::mutex> &lk)
 {
     const int nsteps = prm_.check_interval;
@@ -585,14 +878,40 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
         return s;
     }();
     const int nact = (int)active_.size();
+    // acceleration events of this chunk
+    bool any_aa = false;
+    if (aa_m_ > 0) {
+        for (int s = 0; s < nact; ++s) {
+            const Active &a = active_[s];
+            AaDev d{};
+            d.mode = AA_NONE;
+            if (a.steps + nsteps == a.aa_next) {
+                d.mode = a.aa_stage == 0 ? AA_SAVE : (a.aa_stage == 1 ? AA_FIRST : AA_RESID);
+                d.head = a.aa_head;
+                d.count = std::min(a.aa_count + 1, aa_m_);
+                any_aa = true;
+            }
+            h_aa_[s] = d;
+        }
+    }
     lk.unlock();
     hipError_t e = hipEventRecord(ev0_, stream_);
     if (e == hipSuccess) e = launch_steps_dispatch(nsteps);
     if (e == hipSuccess) e = hipEventRecord(ev1_, stream_);
     if (e == hipSuccess) e = blk_ == 128 ? launch_flux<128>() : (blk_ == 512 ? launch_flux<512>() : launch_flux<256>());
     if (e == hipSuccess) {
-        srv_slot_sum_kernel<<<(unsigned)nact, BLOCK, 0, stream_>>>(d_partial_, d_tlo_, d_thi_, d_sums_);
+        srv_slot_sum_kernel<4><<<(unsigned)nact, BLOCK, 0, stream_>>>(d_partial_, d_tlo_, d_thi_, d_sums_);
         e = hipGetLastError();
+    }
+    if (e == hipSuccess && any_aa) {
+        e = hipMemcpyAsync(d_aa_, h_aa_, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = blk_ == 128 ? launch_aa<128>(false) : (blk_ == 512 ? launch_aa<512>(false) : launch_aa<256>(false));
+        if (e == hipSuccess) {
+            srv_slot_sum_kernel<AA_NV><<<(unsigned)nact, BLOCK, 0, stream_>>>(d_aa_partial_, d_tlo_, d_thi_, d_aa_sums_);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_aa_sums_, d_aa_sums_, (size_t)nact * AA_NV * sizeof(double), hipMemcpyDeviceToHost, stream_);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h_sums_, d_sums_, (size_t)nact * 4 * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -607,38 +926,115 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
     launches_ += nsteps;
     node_updates_ += nodes * nsteps;
     busy_s_ += ms * 1e-3;
-    // retire converged samples (Comput_Perm k = Q/V nu/g, SP:4675-4677; relative change test)
+    // retire converged samples (Comput_Perm k = Q/V nu/g, SP:4675-4677; relative change test:
+    // over one check interval when stepping plainly, between two acceleration events otherwise)
+    std::vector<int> status(nact, 2);
+    std::vector<double> rels(nact, INFINITY);
+    bool any_update = false;
+    for (int s = 0; s < nact; ++s) {
+        Active &a = active_[s];
+        a.steps += nsteps;
+        const double *q = h_sums_ + 4 * s;
+        double kd = 0.0;
+        {
+            const int c = a.req.dir - 1;
+            kd = q[c] / (double)a.req.volume * a.req.nu / a.req.g[c];
+        }
+        const bool event = aa_m_ > 0 && h_aa_[s].mode != AA_NONE;
+        double rel = INFINITY;
+        if (aa_m_ == 0) {
+            rel = a.has_prev ? std::fabs(kd - a.k_prev) / std::max(std::fabs(kd), 1e-300) : INFINITY;
+        } else if (event) {
+            rel = a.has_k_ev ? std::fabs(kd - a.k_ev) / std::max(std::fabs(kd), 1e-300) : INFINITY;
+            a.k_ev = kd;
+            a.has_k_ev = true;
+            a.small_events = rel <= prm_.tol ? a.small_events + 1 : 0;
+        }
+        rels[s] = rel;
+        // accelerated samples retire after two consecutive small changes: Anderson mixing can
+        // stall for one period on a slow mode before the next extrapolation moves k again
+        const bool settled = aa_m_ == 0 ? (a.has_prev && rel <= prm_.tol) : (event && a.small_events >= 2);
+        if (q[3] != 0.0 || !std::isfinite(kd))
+            status[s] = -1;
+        else if (a.steps >= prm_.min_steps && settled)
+            status[s] = 1;
+        else if (a.steps >= prm_.max_steps)
+            status[s] = 0;
+        // precision escalation: a flux too small for FP32 deviations, or no convergence
+        if (prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 &&
+            (status[s] == 0 ||
+             (status[s] == 2 && a.steps >= aa_warm_ && a.steps >= prm_.min_steps && std::fabs(kd) < prm_.escalate_k)))
+            status[s] = 3;
+        a.k_prev = kd;
+        a.has_prev = true;
+        if (!event) continue;
+        // advance the acceleration state; a sample that retires now needs no update
+        AaDev &d = h_aa_[s];
+        if (d.mode == AA_RESID && status[s] == 2) {
+            const double *v = h_aa_sums_ + (size_t)s * AA_NV;
+            const int head = d.head, cnt = d.count;
+            for (int i = 0; i < cnt; ++i) a.gram[i][head] = a.gram[head][i] = v[i];
+            const double r2 = v[2 * AA_MAX];
+            double gam[AA_MAX] = {0.0};
+            bool ok = std::isfinite(r2) && r2 <= 1e6 * a.r2_min && solve_small(a.gram, v + AA_MAX, cnt, gam);
+            if (std::isfinite(r2)) a.r2_min = std::min(a.r2_min, r2);
+            if (!ok) {   // restart the history: this step stays a plain one
+                for (int i = 0; i < AA_MAX; ++i) gam[i] = 0.0;
+                a.aa_count = 0;
+                a.aa_head = 0;
+            } else {
+                a.aa_count = cnt;
+                a.aa_head = (head + 1) % aa_m_;
+                ++a.aa_updates;
+            }
+            for (int i = 0; i < AA_MAX; ++i) d.gamma[i] = gam[i];
+            any_update = true;
+        } else {
+            d.mode = AA_NONE;
+        }
+        a.aa_stage = std::min(a.aa_stage + 1, 2);
+        a.aa_next += aa_period_;
+    }
+    if (any_update) {
+        // x = g - dG gamma for the samples that accelerated, in stream order before the next chunk
+        e = hipMemcpyAsync(d_aa_, h_aa_, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = blk_ == 128 ? launch_aa<128>(true) : (blk_ == 512 ? launch_aa<512>(true) : launch_aa<256>(true));
+        // h_aa_ is pinned and rewritten at the next chunk: wait for the copy to have read it
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) {
+            loop_error_ = true;
+            loop_msg_ = std::string("acceleration update failed: ") + hipGetErrorString(e);
+            return LBM_ERR_HIP;
+        }
+    }
     bool changed = false;
     std::vector<Active> keep;
     keep.reserve(active_.size());
     for (int s = 0; s < nact; ++s) {
         Active &a = active_[s];
-        a.steps += nsteps;
-        const double *q = h_sums_ + 4 * s;
-        double k[3];
-        for (int c = 0; c < 3; ++c) k[c] = a.req.g[c] != 0.0 ? q[c] / (double)a.req.volume * a.req.nu / a.req.g[c] : 0.0;
-        const double kd = k[a.req.dir - 1];
-        const double rel = a.has_prev ? std::fabs(kd - a.k_prev) / std::max(std::fabs(kd), 1e-300) : INFINITY;
-        int status = 2;   // keep running
-        if (q[3] != 0.0 || !std::isfinite(kd))
-            status = -1;
-        else if (a.steps >= prm_.min_steps && a.has_prev && rel <= prm_.tol)
-            status = 1;
-        else if (a.steps >= prm_.max_steps)
-            status = 0;
-        a.k_prev = kd;
-        a.has_prev = true;
-        if (status == 2) {
+        if (status[s] == 2) {
             keep.push_back(std::move(a));
             continue;
         }
+        if (status[s] == 3) {
+            const int64_t base = a.base, npad = a.req.npad;
+            if (escalate_locked(a)) {
+                free_segment(base, npad);
+                changed = true;
+                continue;
+            }
+            status[s] = 0;   // could not escalate: report the FP32 state as unsettled
+        }
+        const double *q = h_sums_ + 4 * s;
         lbm_srv_result r{};
         r.ticket = a.req.ticket;
-        for (int c = 0; c < 3; ++c) r.k[c] = k[c];
-        r.rel_change = rel;
-        r.steps = a.steps;
+        for (int c = 0; c < 3; ++c) r.k[c] = a.req.g[c] != 0.0 ? q[c] / (double)a.req.volume * a.req.nu / a.req.g[c] : 0.0;
+        r.rel_change = rels[s];
         r.fluid_nodes = a.req.N;
-        r.converged = status;
+        r.converged = status[s];
+        r.accelerations = a.req.prior_acc + a.aa_updates;
+        r.precision = prm_.precision;
+        r.steps = a.req.prior_steps + a.steps;
         r.t_submit = a.req.t_submit;
         r.t_admit = a.t_admit;
         r.t_done = clock();

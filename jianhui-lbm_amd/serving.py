@@ -19,7 +19,8 @@ from . import _capi
 class PermeabilityServer:
     def __init__(self, capacity_nodes: int = 0, mem_fraction: float = 0.5, check_interval: int = 100,
                  tol: float = 1e-6, min_steps: int = 0, max_steps: int = 200000, precision: str = "fp32",
-                 block: int = 0, device: int = 0):
+                 block: int = 0, device: int = 0, accel_depth: int = 0, accel_period: int = 0,
+                 accel_warmup: int = 0, escalate_k: float = 0.0):
         p = _capi.SrvParams()
         p.capacity_nodes = int(capacity_nodes)
         p.mem_fraction = float(mem_fraction)
@@ -30,6 +31,10 @@ class PermeabilityServer:
         p.precision = _capi.PRECISIONS[precision]
         p.block = int(block)
         p.device = int(device)
+        p.accel_depth = int(accel_depth)
+        p.accel_period = int(accel_period)
+        p.accel_warmup = int(accel_warmup)
+        p.escalate_k = float(escalate_k)
         h = ctypes.c_void_p()
         _capi.check(_capi.lib().lbm_srv_create(ctypes.byref(p), ctypes.byref(h)), None)
         self._h = h
@@ -71,7 +76,9 @@ class PermeabilityServer:
         out = []
         for r in buf[: n.value]:
             out.append({"ticket": r.ticket, "k": np.array(r.k[:]), "rel_change": r.rel_change, "steps": r.steps,
-                        "fluid_nodes": r.fluid_nodes, "converged": r.converged, "t_submit": r.t_submit,
+                        "fluid_nodes": r.fluid_nodes, "converged": r.converged,
+                        "accelerations": r.accelerations,
+                        "precision": {0: "fp64", 1: "fp32"}.get(r.precision, str(r.precision)), "t_submit": r.t_submit,
                         "t_admit": r.t_admit, "t_done": r.t_done})
         return out
 

@@ -112,6 +112,71 @@ def test_matches_plain_solver_with_short_and_degenerate_axes(pkg, precision):
         assert abs(r["k"][1] - k[1]) <= 1e-6 * abs(k[1])
 
 
+AA = dict(accel_depth=5, accel_period=100, accel_warmup=200)
+
+
+def test_accelerated_result_is_the_same_alone_and_in_the_mix(pkg):
+    """Anderson acceleration is per sample: its dot products are reduced over the sample's own
+    tiles in tile order, so the accelerated result does not depend on the batch mates either."""
+    _log("[gpu] serve: accelerated, alone vs mixed")
+    samples = _samples(pkg)
+    reqs = [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0)) for s in samples]
+    alone = []
+    for r in reqs:
+        with pkg.PermeabilityServer(capacity_nodes=400_000, check_interval=50, tol=1e-6, block=256, **AA) as srv:
+            alone.append(_run(srv, [r])[0])
+    with pkg.PermeabilityServer(capacity_nodes=400_000, check_interval=50, tol=1e-6, block=256, **AA) as srv:
+        mixed = _run(srv, reqs)
+    for a, m in zip(alone, mixed):
+        assert a["converged"] == 1 and m["converged"] == 1
+        assert a["steps"] == m["steps"] and a["accelerations"] == m["accelerations"]
+        np.testing.assert_array_equal(a["k"], m["k"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_acceleration_reaches_the_fp64_answer_in_fewer_steps(pkg, precision):
+    """A low-porosity sample whose slow pressure mode drags plain stepping: accelerated, the
+    served k is within 1e-5 of a long FP64 run after far fewer steps."""
+    _log(f"[gpu] serve: acceleration accuracy, {precision}")
+    s = pkg.geometry.grf_sample(56, 5.0, 0.2, seed=91)
+    with pkg.PermeabilityServer(capacity_nodes=200_000, check_interval=100, tol=1e-7, block=256,
+                                precision=precision, accel_depth=5, accel_period=200, accel_warmup=500) as srv:
+        r = _run(srv, [(s, 1.0 / 6.0, (1e-6, 0.0, 0.0))])[0]
+    assert r["converged"] == 1 and r["accelerations"] > 0
+    sp64 = pkg.SinglePhase(s, 1.0 / 6.0, (1e-6, 0.0, 0.0), precision="fp64")
+    sp64.step(40000)
+    k64 = sp64.Comput_Perm(1)[0][0]
+    sp64.step(200)
+    k64b = sp64.Comput_Perm(1)[0][0]
+    _log(f"[gpu] accelerated k {r['k'][0]:.10e} after {r['steps']} steps ({r['accelerations']} accelerations); "
+         f"FP64 after 40000 steps {k64:.10e} (still moving {abs(k64b - k64) / k64:.1e} per 200 steps), "
+         f"rel diff {abs(r['k'][0] - k64) / k64:.2e}")
+    assert r["steps"] < 20000
+    assert abs(r["k"][0] - k64) <= 1e-5 * abs(k64)
+
+
+def test_escalated_sample_gets_the_fp64_servers_answer(pkg):
+    """A threshold above every k sends each FP32 sample to the internal FP64 server after the
+    warm-up: the results come back marked fp64 and equal, bitwise, what an FP64 server gives
+    for the same requests; the steps include the FP32 warm-up."""
+    _log("[gpu] serve: precision escalation")
+    samples = _samples(pkg)[:3]
+    reqs = [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0)) for s in samples]
+    kw = dict(capacity_nodes=400_000, check_interval=50, tol=1e-6, block=256, **AA)
+    with pkg.PermeabilityServer(escalate_k=1e3, **kw) as srv:
+        esc = _run(srv, reqs)
+        st = srv.stats()
+    with pkg.PermeabilityServer(precision="fp64", **kw) as srv:
+        ref = _run(srv, reqs)
+    assert st["done"] == len(reqs) and st["pending"] == 0 and st["active"] == 0
+    for e, r in zip(esc, ref):
+        assert e["precision"] == "fp64" and r["precision"] == "fp64"
+        assert e["converged"] == 1
+        np.testing.assert_array_equal(e["k"], r["k"])
+        assert e["steps"] == r["steps"] + AA["accel_warmup"]
+        assert e["t_done"] >= e["t_admit"] >= e["t_submit"]
+
+
 def test_rejects_bad_requests(pkg):
     with pkg.PermeabilityServer(capacity_nodes=50_000, block=256) as srv:
         with pytest.raises(pkg.LBMError):
@@ -125,6 +190,10 @@ def test_rejects_bad_requests(pkg):
             srv.submit(np.zeros((3, 190, 190), np.int32), 0.1, (1e-5, 0, 0))  # x-plane of 36100 nodes
     with pytest.raises(pkg.LBMError):
         pkg.PermeabilityServer(capacity_nodes=50_000, block=256, precision="fp16")
+    with pytest.raises(pkg.LBMError):
+        pkg.PermeabilityServer(capacity_nodes=50_000, block=256, accel_depth=9)
+    with pytest.raises(pkg.LBMError):
+        pkg.PermeabilityServer(capacity_nodes=50_000, block=256, check_interval=100, accel_depth=3, accel_period=150)
 
 
 def test_autotuned_block(pkg):
