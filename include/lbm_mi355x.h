@@ -170,9 +170,9 @@ typedef struct lbm_srv_params {
     int32_t accel_period;     /* steps between accelerations, a multiple of check_interval (0 = 2*check_interval) */
     int32_t accel_warmup;     /* plain steps before the first acceleration, multiple of check_interval (0 = 5*check_interval) */
     /* Precision escalation of an FP32 server: a sample whose k (lattice units) falls below
-     * escalate_k after the warm-up, or that reaches max_steps unsettled, is solved again from
-     * rest in FP64 by an internal FP64 server (its flux is then too small for FP32 deviations
-     * to resolve: a nearly disconnected pore space).  0 = never. */
+     * escalate_k after the warm-up, or that reaches max_steps unsettled, moves with its state to
+     * an internal FP64 server and continues there (its flux is then too small for FP32
+     * deviations to resolve: a nearly disconnected pore space).  0 = never. */
     double escalate_k;
 } lbm_srv_params;
 
@@ -190,7 +190,7 @@ typedef struct lbm_srv_result {
     double rel_change;        /* last relative change of k[perm_dir-1] */
     int64_t steps;
     int64_t fluid_nodes;
-    int32_t converged;        /* 1 converged, 0 stopped at max_steps, -1 non-finite velocity */
+    int32_t converged;        /* 1 converged, 0 stopped at max_steps, -1 non-finite velocity, -2 not preparable */
     int32_t accelerations;    /* Anderson extrapolations applied to this sample */
     int32_t precision;        /* LBM_FP32 or LBM_FP64: the arena the result came from */
     int32_t pad;
@@ -208,7 +208,8 @@ typedef struct lbm_srv_stats {
 } lbm_srv_stats;
 
 int lbm_srv_create(const lbm_srv_params *p, lbm_handle **out);
-/* queue a request (geometry numbering and neighbour lists are built in the caller's thread) */
+/* queue a request: the solid array is copied and checked here (pore count, arena size, index
+ * span); numbering and neighbour deltas are built by a pool of preparation threads */
 int lbm_srv_submit(lbm_handle *h, const lbm_srv_request *req, int64_t *ticket);
 /* take up to max finished results */
 int lbm_srv_poll(lbm_handle *h, lbm_srv_result *out, int32_t max, int32_t *n_out);

@@ -29,6 +29,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <omp.h>
 #include <thread>
 #include <vector>
 
@@ -285,6 +286,7 @@ struct Prepared {
     int64_t ticket = 0;
     int64_t prior_steps = 0;     // steps already spent in an FP32 arena before escalation
     int prior_acc = 0;
+    std::vector<float> carried;  // 19 x npad FP32 state to continue from (escalation), else start at rest
     int64_t N = 0, npad = 0;
     std::vector<int16_t> nbr;    // 18 x npad upstream deltas (SRV_BB = bounce back)
     double nu = 0, g[3] = {0, 0, 0};
@@ -349,6 +351,36 @@ struct Segment {
     int64_t start, len;
 };
 
+// a submitted request waiting for its numbering and neighbour deltas (built on a worker)
+struct PrepJob {
+    int64_t ticket = 0;
+    double t_submit = 0;
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<int32_t> solid;
+    double nu = 0, g[3] = {0, 0, 0};
+    int dir = 1;
+};
+
+// Largest number of pore voxels in any cyclic run of L consecutive voxels (i-major order).  A
+// pull link from voxel v reaches back at most ny*nz + nz voxels (the xy diagonals), and its
+// node-index distance, taken modulo the node count, is the number of pores in between.
+int64_t max_window_pores(const int32_t *solid, int64_t V, int64_t L)
+{
+    if (L >= V) {
+        int64_t c = 0;
+        for (int64_t v = 0; v < V; ++v) c += solid[v] == 0;
+        return c;
+    }
+    int64_t c = 0;
+    for (int64_t v = V - L; v < V; ++v) c += solid[v] == 0;
+    int64_t best = c;
+    for (int64_t v = 0; v < V; ++v) {
+        c += (solid[v] == 0) - (solid[(v - L + V) % V] == 0);
+        best = std::max(best, c);
+    }
+    return best;
+}
+
 // The sample's pull lists as 16-bit deltas: for each link the shortest of up - n, up - n - N,
 // up - n + N (the device adds it to n and reduces modulo N).  False if a link does not fit,
 // which needs an x-plane of more than 32767 fluid nodes.
@@ -388,6 +420,7 @@ public:
 
     // queue an already prepared sample (precision escalation from an FP32 server)
     void enqueue(Prepared &&pr);
+    void prep_worker();
     bool fits(int64_t npad) const { return npad <= cap_; }
 
 private:
@@ -422,7 +455,8 @@ private:
     // Anderson acceleration: X, R_prev, G_prev, then m dR and m dG vectors, each 19 x CAP
     int aa_m_ = 0, aa_period_ = 0, aa_warm_ = 0;
     void *d_aa_vec_ = nullptr;
-    AaDev *d_aa_ = nullptr, *h_aa_ = nullptr;
+    AaDev *d_aa_ = nullptr, *h_aa_ = nullptr;   // h_aa_: two pinned halves, alternating by chunk
+    int aa_half_ = 0;
     double *d_aa_partial_ = nullptr, *d_aa_sums_ = nullptr, *h_aa_sums_ = nullptr;
     SlotDev *d_slots_ = nullptr;
     TileDev *d_tiles_ = nullptr;
@@ -440,6 +474,10 @@ private:
     std::string loop_msg_;
     int64_t next_ticket_ = 1;
     std::deque<Prepared> pending_;
+    std::deque<PrepJob> prep_q_;   // submitted, geometry not built yet
+    int prep_busy_ = 0;
+    std::vector<std::thread> prep_thr_;
+    std::condition_variable prep_cv_;
     std::vector<Active> active_;   // in tile order
     std::vector<Segment> free_;
     std::deque<lbm_srv_result> done_;
@@ -481,6 +519,13 @@ bool Server::escalate_locked(Active &a)
     hipError_t e = hipMemcpy2DAsync(a.req.nbr.data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
                                     (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                     hipMemcpyDeviceToHost, stream_);
+    // the FP64 solve continues from the FP32 state rather than from rest: the transient so
+    // far (the pressure build-up of the slow mode) is kept
+    a.req.carried.resize((size_t)Q * a.req.npad);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(a.req.carried.data(), (size_t)a.req.npad * sizeof(float), (const float *)pops_[cur_] + a.base,
+                             (size_t)cap_ * sizeof(float), (size_t)a.req.npad * sizeof(float), Q, hipMemcpyDeviceToHost,
+                             stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return false;
     a.req.prior_steps += a.steps;
@@ -568,7 +613,7 @@ int Server::create(const lbm_srv_params &p)
     if (aa_m_ > 0) {
         LBM_HIP_TRY(hipMalloc(&d_aa_vec_, (size_t)(3 + 2 * aa_m_) * Q * cap_ * st_));
         LBM_HIP_TRY(hipMalloc(&d_aa_, (size_t)max_slots_ * sizeof(AaDev)));
-        LBM_HIP_TRY(hipHostMalloc((void **)&h_aa_, (size_t)max_slots_ * sizeof(AaDev), hipHostMallocDefault));
+        LBM_HIP_TRY(hipHostMalloc((void **)&h_aa_, (size_t)2 * max_slots_ * sizeof(AaDev), hipHostMallocDefault));
         LBM_HIP_TRY(hipMalloc(&d_aa_partial_, (size_t)max_tiles_ * AA_NV * sizeof(double)));
         LBM_HIP_TRY(hipMalloc(&d_aa_sums_, (size_t)max_slots_ * AA_NV * sizeof(double)));
         LBM_HIP_TRY(hipHostMalloc((void **)&h_aa_sums_, (size_t)max_slots_ * AA_NV * sizeof(double), hipHostMallocDefault));
@@ -580,6 +625,11 @@ int Server::create(const lbm_srv_params &p)
         if (rc != LBM_OK) return rc;
     }
     thr_ = std::thread([this] { loop(); });
+    // geometry preparation off the caller's thread: each worker numbers a sample and builds
+    // its neighbour deltas while the device steps the admitted ones
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nprep = (int)std::min(8u, std::max(1u, hw / 2));
+    for (int i = 0; i < nprep; ++i) prep_thr_.emplace_back([this] { prep_worker(); });
     return LBM_OK;
 }
 
@@ -590,6 +640,10 @@ void Server::shutdown()
         stop_ = true;
     }
     cv_.notify_all();
+    prep_cv_.notify_all();
+    for (std::thread &t : prep_thr_)
+        if (t.joinable()) t.join();
+    prep_thr_.clear();
     if (thr_.joinable()) thr_.join();
     (void)hipSetDevice(device_);
     for (void *&b : pops_)
@@ -613,29 +667,81 @@ int Server::submit(const lbm_srv_request &r, int64_t *ticket)
     if (!(r.nu > 0)) return fail(LBM_ERR_ARG, "viscosity must be positive");
     if (r.perm_dir < 1 || r.perm_dir > 3) return fail(LBM_ERR_ARG, "perm_dir must be 1, 2 or 3");
     if (r.g[r.perm_dir - 1] == 0.0) return fail(LBM_ERR_ARG, "body force along perm_dir is zero");
-    SparseGeometry geo;
-    std::string err;
-    int rc = build_sparse_geometry(r.nx, r.ny, r.nz, r.solid_ijk, geo, err);
-    if (rc != LBM_OK) return fail(rc, err);
-    if (geo.P > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
-    Prepared pr;
-    pr.N = geo.N;
-    pr.npad = geo.P;
-    if (!pack_deltas(geo, pr.nbr))
-        return fail(LBM_ERR_ARG, "sample has an x-plane of more than 32767 fluid nodes (solve it with lbm_sp_*)");
-    pr.nu = r.nu;
-    for (int c = 0; c < 3; ++c) pr.g[c] = r.g[c];
-    pr.dir = r.perm_dir;
-    pr.volume = (int64_t)r.nx * r.ny * r.nz;
+    const int64_t V = (int64_t)r.nx * r.ny * r.nz;
+    if (V >= ((int64_t)1 << 31)) return fail(LBM_ERR_ARG, "box has 2^31 or more voxels");
+    // the checks that decide admissibility are O(voxels) and done here; the numbering and the
+    // neighbour deltas are built by a preparation worker
+    int64_t N = 0;
+    for (int64_t v = 0; v < V; ++v) N += r.solid_ijk[v] == 0;
+    if (N == 0) return fail(LBM_ERR_ARG, "geometry has no pore voxels");
+    if (padded_stride(N) > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
+    if (max_window_pores(r.solid_ijk, V, (int64_t)r.ny * r.nz + r.nz) > 32767)
+        return fail(LBM_ERR_ARG, "sample has more than 32767 fluid nodes within one x-plane span (solve it with lbm_sp_*)");
+    PrepJob job;
+    job.nx = r.nx;
+    job.ny = r.ny;
+    job.nz = r.nz;
+    job.solid.assign(r.solid_ijk, r.solid_ijk + V);
+    job.nu = r.nu;
+    for (int c = 0; c < 3; ++c) job.g[c] = r.g[c];
+    job.dir = r.perm_dir;
     {
         std::lock_guard<std::mutex> lk(mu_);
-        pr.ticket = next_ticket_++;
-        pr.t_submit = clock();
-        if (ticket) *ticket = pr.ticket;
-        pending_.push_back(std::move(pr));
+        job.ticket = next_ticket_++;
+        job.t_submit = clock();
+        if (ticket) *ticket = job.ticket;
+        prep_q_.push_back(std::move(job));
     }
-    cv_.notify_all();
+    prep_cv_.notify_one();
     return LBM_OK;
+}
+
+void Server::prep_worker()
+{
+    omp_set_num_threads(2);   // several workers at once: keep each one's parallel loops small
+    for (;;) {
+        PrepJob job;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            prep_cv_.wait(lk, [this] { return stop_ || !prep_q_.empty(); });
+            if (stop_) return;
+            job = std::move(prep_q_.front());
+            prep_q_.pop_front();
+            ++prep_busy_;
+        }
+        SparseGeometry geo;
+        std::string err;
+        Prepared pr;
+        bool ok = build_sparse_geometry(job.nx, job.ny, job.nz, job.solid.data(), geo, err) == LBM_OK;
+        std::vector<int32_t>().swap(job.solid);
+        if (ok) ok = pack_deltas(geo, pr.nbr);   // guaranteed by the window check in submit
+        pr.ticket = job.ticket;
+        pr.t_submit = job.t_submit;
+        pr.N = geo.N;
+        pr.npad = geo.P;
+        pr.nu = job.nu;
+        for (int c = 0; c < 3; ++c) pr.g[c] = job.g[c];
+        pr.dir = job.dir;
+        pr.volume = (int64_t)job.nx * job.ny * job.nz;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            --prep_busy_;
+            if (ok) {
+                pending_.push_back(std::move(pr));
+            } else {
+                lbm_srv_result r{};
+                r.ticket = job.ticket;
+                r.converged = -2;   // could not be prepared
+                r.precision = prm_.precision;
+                r.t_submit = r.t_admit = job.t_submit;
+                r.t_done = clock();
+                done_.push_back(r);
+                ++n_done_;
+            }
+        }
+        cv_.notify_all();
+        idle_cv_.notify_all();
+    }
 }
 
 int Server::poll(lbm_srv_result *out, int max, int *n_out)
@@ -662,7 +768,9 @@ int Server::wait_idle(double timeout_s)
 {
     const auto t_end = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
     std::unique_lock<std::mutex> lk(mu_);
-    auto idle = [this] { return loop_error_ || (pending_.empty() && active_.empty()); };
+    auto idle = [this] {
+        return loop_error_ || (prep_q_.empty() && prep_busy_ == 0 && pending_.empty() && active_.empty());
+    };
     if (timeout_s > 0) {
         if (!idle_cv_.wait_until(lk, t_end, idle)) return LBM_ERR_BUSY;
     } else {
@@ -685,7 +793,7 @@ int Server::wait_idle(double timeout_s)
 int Server::stats(lbm_srv_stats *out)
 {
     std::lock_guard<std::mutex> lk(mu_);
-    out->pending = (int64_t)pending_.size();
+    out->pending = (int64_t)(pending_.size() + prep_q_.size()) + prep_busy_;
     out->active = (int64_t)active_.size();
     out->done = n_done_;
     out->arena_nodes = cap_;
@@ -729,9 +837,19 @@ bool Server::try_admit_locked()
         a.t_admit = clock();
         a.aa_next = aa_warm_;
         // sample-local neighbour rows go straight into the arena rows of the segment
-        const hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.data(),
-                                              (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t),
-                                              18, hipMemcpyHostToDevice, stream_);
+        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.data(),
+                                        (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
+                                        hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess && !a.req.carried.empty()) {
+            // an escalated sample resumes from its FP32 state: no equilibrium step
+            std::vector<double> st(a.req.carried.begin(), a.req.carried.end());
+            std::vector<float>().swap(a.req.carried);
+            e = hipMemcpy2DAsync((char *)pops_[cur_] + (size_t)base * st_, (size_t)cap_ * st_, st.data(),
+                                 (size_t)a.req.npad * sizeof(double), (size_t)a.req.npad * sizeof(double), Q,
+                                 hipMemcpyHostToDevice, stream_);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+            a.first_step = -1;
+        }
         if (e != hipSuccess) {
             loop_error_ = true;
             loop_msg_ = std::string("neighbour upload failed: ") + hipGetErrorString(e);
@@ -878,7 +996,10 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
         return s;
     }();
     const int nact = (int)active_.size();
-    // acceleration events of this chunk
+    // acceleration events of this chunk; this chunk's half of the pinned table (the other half
+    // may still be read by the previous chunk's update copy)
+    AaDev *const hb = h_aa_ ? h_aa_ + (size_t)aa_half_ * max_slots_ : nullptr;
+    aa_half_ ^= 1;
     bool any_aa = false;
     if (aa_m_ > 0) {
         for (int s = 0; s < nact; ++s) {
@@ -891,7 +1012,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
                 d.count = std::min(a.aa_count + 1, aa_m_);
                 any_aa = true;
             }
-            h_aa_[s] = d;
+            hb[s] = d;
         }
     }
     lk.unlock();
@@ -904,7 +1025,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
         e = hipGetLastError();
     }
     if (e == hipSuccess && any_aa) {
-        e = hipMemcpyAsync(d_aa_, h_aa_, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
+        e = hipMemcpyAsync(d_aa_, hb, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess) e = blk_ == 128 ? launch_aa<128>(false) : (blk_ == 512 ? launch_aa<512>(false) : launch_aa<256>(false));
         if (e == hipSuccess) {
             srv_slot_sum_kernel<AA_NV><<<(unsigned)nact, BLOCK, 0, stream_>>>(d_aa_partial_, d_tlo_, d_thi_, d_aa_sums_);
@@ -940,7 +1061,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
             const int c = a.req.dir - 1;
             kd = q[c] / (double)a.req.volume * a.req.nu / a.req.g[c];
         }
-        const bool event = aa_m_ > 0 && h_aa_[s].mode != AA_NONE;
+        const bool event = aa_m_ > 0 && hb[s].mode != AA_NONE;
         double rel = INFINITY;
         if (aa_m_ == 0) {
             rel = a.has_prev ? std::fabs(kd - a.k_prev) / std::max(std::fabs(kd), 1e-300) : INFINITY;
@@ -969,7 +1090,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
         a.has_prev = true;
         if (!event) continue;
         // advance the acceleration state; a sample that retires now needs no update
-        AaDev &d = h_aa_[s];
+        AaDev &d = hb[s];
         if (d.mode == AA_RESID && status[s] == 2) {
             const double *v = h_aa_sums_ + (size_t)s * AA_NV;
             const int head = d.head, cnt = d.count;
@@ -997,10 +1118,8 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
     }
     if (any_update) {
         // x = g - dG gamma for the samples that accelerated, in stream order before the next chunk
-        e = hipMemcpyAsync(d_aa_, h_aa_, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
+        e = hipMemcpyAsync(d_aa_, hb, (size_t)nact * sizeof(AaDev), hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess) e = blk_ == 128 ? launch_aa<128>(true) : (blk_ == 512 ? launch_aa<512>(true) : launch_aa<256>(true));
-        // h_aa_ is pinned and rewritten at the next chunk: wait for the copy to have read it
-        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
         if (e != hipSuccess) {
             loop_error_ = true;
             loop_msg_ = std::string("acceleration update failed: ") + hipGetErrorString(e);

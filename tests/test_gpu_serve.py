@@ -155,14 +155,14 @@ def test_acceleration_reaches_the_fp64_answer_in_fewer_steps(pkg, precision):
     assert abs(r["k"][0] - k64) <= 1e-5 * abs(k64)
 
 
-def test_escalated_sample_gets_the_fp64_servers_answer(pkg):
+def test_escalated_sample_continues_in_fp64(pkg):
     """A threshold above every k sends each FP32 sample to the internal FP64 server after the
-    warm-up: the results come back marked fp64 and equal, bitwise, what an FP64 server gives
-    for the same requests; the steps include the FP32 warm-up."""
+    warm-up; it continues from its FP32 state and settles on the FP64 server's answer (to the
+    FP32 rounding carried over), marked fp64, with the FP32 steps counted."""
     _log("[gpu] serve: precision escalation")
     samples = _samples(pkg)[:3]
     reqs = [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0)) for s in samples]
-    kw = dict(capacity_nodes=400_000, check_interval=50, tol=1e-6, block=256, **AA)
+    kw = dict(capacity_nodes=400_000, check_interval=50, tol=1e-7, block=256, **AA)
     with pkg.PermeabilityServer(escalate_k=1e3, **kw) as srv:
         esc = _run(srv, reqs)
         st = srv.stats()
@@ -171,9 +171,8 @@ def test_escalated_sample_gets_the_fp64_servers_answer(pkg):
     assert st["done"] == len(reqs) and st["pending"] == 0 and st["active"] == 0
     for e, r in zip(esc, ref):
         assert e["precision"] == "fp64" and r["precision"] == "fp64"
-        assert e["converged"] == 1
-        np.testing.assert_array_equal(e["k"], r["k"])
-        assert e["steps"] == r["steps"] + AA["accel_warmup"]
+        assert e["converged"] == 1 and e["steps"] > AA["accel_warmup"]
+        assert abs(e["k"][0] - r["k"][0]) <= 1e-5 * abs(r["k"][0])
         assert e["t_done"] >= e["t_admit"] >= e["t_submit"]
 
 
