@@ -183,6 +183,38 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000):
     return out
 
 
+def cg_bench(pkg, steps, warmup, precision, edge=192):
+    """Colour-gradient workload: 192^3 sphere pack (r = 6, porosity 0.45), S_w = 0.5 by voxel
+    (seeded), nu_l = nu_g = 1/6, CapA = 0.01, cos theta = 0, g = 1e-5."""
+    geo = pkg.geometry
+    solid = geo.sphere_pack((edge, edge, edge), 6.0, 0.45, seed=2025)
+    psi = np.where(np.random.default_rng(2026).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
+    s = pkg.ColorGradient(solid, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), precision=precision)
+    s.init(psi)
+    s.step(warmup)
+    s.diagnostics()
+    t0 = time.perf_counter()
+    ms = s.step_timed(steps)
+    wall = time.perf_counter() - t0
+    d = s.diagnostics()
+    N = s.count
+    b_alg = 2 * 21 * (8 if precision == "fp64" else 4) + 18 * 4   # f + rho_r + rho_b read and written, indices
+    achieved = b_alg * N / (ms / 1e3 / steps) / 1e9
+    out = {"metric": "fluid MLUPS (colour gradient)", "value": round(N * steps / wall / 1e6, 1), "unit": "MLUPS",
+           "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(wall * 1e3 / steps, 5),
+           "higher_is_better": True, "dtype": DTYPE[precision], "data": "synthetic",
+           "config": {"workload": f"cg_d3q19_mrt_{edge}cube_spheres_r6_phi0.45_sw0.5", "fluid_nodes": N,
+                      "precision": precision},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": read_traffic(f"cg_step_{precision}_{edge}cube"), "bytes_per_node": b_alg,
+                        "kernels": "cg_collide_kernel + cg_colour_kernel per step",
+                        "avg_step_us": round(ms * 1e3 / steps, 2)},
+           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"]}
+    s.close()
+    return out
+
+
 def accuracy_study(pkg, edge=256, max_steps=100000):
     """The single-phase workload run to steady state (|k_n - k_(n-100)| <= 1e-6 |k_n|, the
     reference's relative-change test at freRe = 100) in FP64, FP32 and FP16 storage; each
