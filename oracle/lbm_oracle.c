@@ -530,3 +530,69 @@ double orc_cg_wet_count(int ny, int nz, int count, const int32_t *supinv, const 
         if (in_box(supinv[n], ny, nz, box) && psi[n] >= 0) sl += 1;
     return sl;
 }
+
+/* ------------------------------------------------------------------ passive solute */
+/* D3Q7 lattice in the reference's six-velocity order (e2, 6V.cpp:130) plus the rest direction:
+ * 0 rest, 1 +x, 2 -x, 3 +y, 4 -y, 5 +z, 6 -z; opposite pairs LR (6V.cpp:138). */
+static const int ST_E[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const int ST_LR[7] = {0, 2, 1, 4, 3, 6, 5};
+
+/* g_eq_q = w_q C (1 + e_q.u / c_s^2), w_q = (1 - w0)/6, c_s^2 = (1 - w0)/3, w_rest = w0.  With
+ * w0 = 0 this is 1/6 C (1 + 3 e.u) of init and collision (6V.cpp:1306-1310, 1473). */
+static double st_feq(int q, double w0, double C, const double *u)
+{
+    if (q == 0) return w0 * C;
+    const double w = (1.0 - w0) / 6.0, cs2 = (1.0 - w0) / 3.0;
+    const double eu = ST_E[q][0] * u[0] + ST_E[q][1] * u[1] + ST_E[q][2] * u[2];
+    return w * C * (1.0 + eu / cs2);
+}
+
+void orc_st_init(int count, double w0, const double *C, const double *u, double *g)
+{
+    for (int n = 1; n <= count; n++)
+        for (int q = 0; q < 7; q++) g[(long)n * 7 + q] = st_feq(q, w0, C[n], u + (long)n * 3);
+}
+
+/* nsteps x (collision + streaming + macro): BGK relaxation g* = g - (g - g_eq)/tau_s
+ * (6V.cpp:1466-1474), push to the downstream node or bounce back into the opposite population
+ * of the node itself when it is solid (6V.cpp:1481-1535), then C = sum of the populations
+ * (6V.cpp:1625-1640).  The rest population stays on the node.  C is updated in place. */
+void orc_st_step(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv, double w0,
+                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps)
+{
+    for (int step = 0; step < nsteps; step++) {
+        for (int ci = 1; ci <= count; ci++) {
+            int s = supinv[ci];
+            int i = s / (ny * nz), j = (s % (ny * nz)) / nz, m = s % nz;
+            for (int q = 0; q < 7; q++) {
+                const double gq = g[(long)ci * 7 + q];
+                const double v = gq - (gq - st_feq(q, w0, C[ci], u + (long)ci * 3)) / tau_s;
+                if (q == 0) {
+                    G[(long)ci * 7] = v;
+                    continue;
+                }
+                int ip = wrap(i + ST_E[q][0], nx), jp = wrap(j + ST_E[q][1], ny), kp = wrap(m + ST_E[q][2], nz);
+                int t = idx[((long)ip * ny + jp) * nz + kp];
+                if (t > 0)
+                    G[(long)t * 7 + q] = v;
+                else
+                    G[(long)ci * 7 + ST_LR[q]] = v;
+            }
+        }
+        for (int n = 1; n <= count; n++) {
+            double c = 0;
+            for (int q = 0; q < 7; q++) {
+                g[(long)n * 7 + q] = G[(long)n * 7 + q];
+                c += G[(long)n * 7 + q];
+            }
+            C[n] = c;
+        }
+    }
+}
+
+double orc_st_mass(int count, const double *C)
+{
+    double s = 0;
+    for (int n = 1; n <= count; n++) s += C[n];
+    return s;
+}

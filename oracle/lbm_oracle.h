@@ -85,6 +85,14 @@ void orc_cg_flux(int ny, int nz, int count, const int32_t *supinv, const double 
                  const int *box, double rel_perm_psi, double rel_perm_psi2, double Ql[3], double Qg[3]);
 double orc_cg_wet_count(int ny, int nz, int count, const int32_t *supinv, const double *psi, const int *box);
 
+/* ---- passive solute, D3Q7 BGK with rest weight w0 (w0 = 0: the reference's six-velocity
+ * scheme, Single_Phase_3D/Solute_Transport_MPI_Software_Spars_6V.cpp).  Arrays 1-based with a
+ * padding row 0 like the others: g[(count+1)*7] (index 0 = rest), C[count+1], u[(count+1)*3]. */
+void orc_st_init(int count, double w0, const double *C, const double *u, double *g);
+void orc_st_step(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv, double w0,
+                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps);
+double orc_st_mass(int count, const double *C);
+
 #ifdef __cplusplus
 }
 #endif

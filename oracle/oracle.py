@@ -39,6 +39,7 @@ def lib():
         L.orc_sp_error.restype = ctypes.c_double
         L.orc_cg_wet_count.restype = ctypes.c_double
         L.orc_invert19.restype = ctypes.c_int
+        L.orc_st_mass.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -217,3 +218,35 @@ class _CGParams(ctypes.Structure):
     _fields_ = [("g", ctypes.c_double * 3), ("cap_a", ctypes.c_double), ("psi_solid", ctypes.c_double),
                 ("visc", ctypes.c_double * 8), ("bodyforce_apply", ctypes.c_int),
                 ("rel_perm_psi", ctypes.c_double), ("rel_perm_psi2", ctypes.c_double)]
+
+
+class STOracle:
+    """Passive solute (D3Q7 BGK, rest weight w0; w0 = 0 is the reference's six-velocity scheme)
+    in a frozen velocity field, stepped like the six-velocity program's time loop."""
+
+    def __init__(self, solid, D, u_nodes, conc_field, w0=0.25, u_init=None):
+        self.solid = np.ascontiguousarray(solid, np.int32)
+        self.nx, self.ny, self.nz = self.solid.shape
+        self.count, self.idx, self.supinv = number_fluid(self.solid)
+        self.w0 = float(w0)
+        cs2 = (1.0 - self.w0) / 3.0
+        self.tau = float(D) / cs2 + 0.5          # tau_s = D/c_s^2 + 1/2 (6V.cpp:1274)
+        n1 = self.count + 1
+        self.u = np.zeros((n1, 3))
+        self.u[1:] = u_nodes
+        self.C = np.zeros(n1)
+        self.C[1:] = np.asarray(conc_field, np.float64).ravel()[self.supinv[1:]]
+        self.g = np.zeros((n1, 7))
+        self.G = np.zeros((n1, 7))
+        ui = self.u if u_init is None else np.ascontiguousarray(u_init, np.float64)
+        lib().orc_st_init(self.count, ctypes.c_double(self.w0), _d(self.C), _d(ui), _d(self.g))
+        self.steps = 0
+
+    def run(self, nsteps: int):
+        lib().orc_st_step(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv),
+                          ctypes.c_double(self.w0), ctypes.c_double(self.tau), _d(self.u), _d(self.g), _d(self.G),
+                          _d(self.C), int(nsteps))
+        self.steps += nsteps
+
+    def mass(self) -> float:
+        return lib().orc_st_mass(self.count, _d(self.C))

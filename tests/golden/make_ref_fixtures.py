@@ -143,6 +143,90 @@ def cg_case(name, shape, radius, por, seed, psi_mode, psi_seed, nu_l, nu_g, cos_
     print("wrote", path, "count =", out[f"rho_{steps_list[0]}"].shape[0])
 
 
+def _write_st_input(path, shape, n_max, niu_s, freRe, fre_backup, sol_ini_n, restart=0):
+    """INPUT file of the six-velocity solute program, by line as its reader takes it
+    (Solute_Transport_MPI_Software_Spars_6V.cpp:219-270); every boundary module off."""
+    L = ["=== solute input (generated) ===",
+         "geometry.bin :geometry", "phase.dat :initial concentration",
+         f"{shape[0]} {shape[1]} {shape[2]} :nx ny nz", f"{n_max} :n_max", "1.0 :reso", "0 :in_BC",
+         "0.0 0.0 :buoyancy", "0.0 :gravity", "0.0 0.0 0.0 :body force",
+         "0 1.0 0 1.0 :p x", "0 1.0 0 1.0 :p y", "0 1.0 0 1.0 :p z",
+         "0 0.0 0 0.0 :v x", "0 0.0 0 0.0 :v y", "0 0.0 0 0.0 :v z",
+         "0 1.0 0 1.0 :c x", "0 1.0 0 1.0 :c y", "0 1.0 0 1.0 :c z",
+         "0 0.0 0 0.0 :zf x", "0 0.0 0 0.0 :zf y", "0 0.0 0 0.0 :zf z",
+         "0.1 :viscosity", f"{niu_s!r} :diffusion coefficient", "0.0 0.0 0.0 :initial velocity",
+         "=== output ===", "0 :wr_per", "1 :PerDir", f"{freRe} :freRe", "0 :Out_Mode", "-1 :freVe",
+         "-1 :freDe", "-1 :frePsi", "0 1000 :dispersion statistics", "=== advanced ===",
+         "0 1.0 1.0 :lattice_v", "./ :output path", "2 :Sub_BC", "1 :Sub_BC_psi", "-1 :ini_psi",
+         "0 0 0 :partial perm", "0 0 :x", "0 0 :y", "0 0 :z", "=== backup ===",
+         f"{fre_backup} :fre_backup", f"{restart} :mode_backup_ini", f"{sol_ini_n} :velocity label",
+         "2 2 6 :peaks", "=== geometry reading ===", "1 :decbin"]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def st_case(name, sp_fixture, niu_s, u_scale, c_seed, n_max, labels, freRe=5):
+    """Six-velocity passive solute (the D3Q7 oracle with rest weight 0) on the geometry and
+    the reference velocity field of an existing single-phase fixture, the field scaled up so
+    that advection shows.
+
+    The program reads the velocity per rank from Velocity_for_solute_<label>.<rank>.bin
+    (6V.cpp:3640-3665), after its init has already formed the initial populations from the
+    not yet loaded velocity array (6V.cpp:473-475, 1306-1310), i.e. from uninitialised memory.
+    The fixture therefore starts it through its restart path (mode_backup_ini, 6V.cpp:3716-3770)
+    from an initial state written here: g_q = C/6 (1 + 3 e_q.u), the same formula, stored in the
+    fixture.  A first short run gives the ranks' node counts (from its checkpoint sizes)."""
+    d = np.load(os.path.join(HERE, f"{sp_fixture}.npz"))
+    solid = d["solid"].astype(np.int32)
+    shape = solid.shape
+    u = d["u_101"] * u_scale
+    rng = np.random.default_rng(c_seed)
+    conc = np.round(rng.uniform(0.0, 1.0, size=shape) * 64.0) / 64.0   # exact in the float read
+    wd = tempfile.mkdtemp(prefix="st_fix_")
+    try:
+        geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+        geometry.write_phase_ascii(os.path.join(wd, "phase.dat"), conc)
+        for r in range(2):
+            np.zeros((u.shape[0], 3)).tofile(os.path.join(wd, f"Velocity_for_solute_0.{r}.bin"))
+        _write_st_input(os.path.join(wd, "INPUT.dat"), shape, 1, niu_s, 1, 1, 0)
+        _run(os.path.join(REF_DIR, "st6_ref"), wd)
+        counts = [np.fromfile(os.path.join(wd, f"LBM_checkpoint_psi_1.{r}.bin_input"), "<f8").size - 1
+                  for r in range(2)]
+        assert sum(counts) == u.shape[0], (counts, u.shape)
+        for f in os.listdir(wd):
+            if f.startswith("LBM_checkpoint"):
+                os.remove(os.path.join(wd, f))
+        # initial state: node concentrations in the global order, populations from them
+        flat = conc.ravel()
+        pore = np.flatnonzero(solid.ravel() == 0)
+        c0 = flat[pore]
+        e = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float)
+        g0 = np.zeros((u.shape[0], 7))
+        for q in range(1, 7):
+            g0[:, q] = 1.0 / 6.0 * c0 * (1 + 3 * (u @ e[q]))
+        lo = 0
+        for r in range(2):
+            hi = lo + counts[r]
+            np.ascontiguousarray(u[lo:hi]).tofile(os.path.join(wd, f"Velocity_for_solute_0.{r}.bin"))
+            np.concatenate([np.zeros((1, 7)), g0[lo:hi]]).tofile(os.path.join(wd, f"LBM_checkpoint_fg_1.{r}.bin_input"))
+            np.concatenate([[0.0], c0[lo:hi]]).tofile(os.path.join(wd, f"LBM_checkpoint_psi_1.{r}.bin_input"))
+            lo = hi
+        _write_st_input(os.path.join(wd, "INPUT.dat"), shape, n_max, niu_s, freRe, freRe, 0, restart=1)
+        _run(os.path.join(REF_DIR, "st6_ref"), wd)
+        out = {"solid": solid.astype(np.int8), "conc_field": conc, "u": u, "g_0": g0, "c_0": c0}
+        for lab in labels:
+            out[f"g_{lab + 1}"] = _read_ckpt(wd, "fg", lab, 7)
+            out[f"c_{lab + 1}"] = _read_ckpt(wd, "psi", lab, 1)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    params = dict(shape=list(shape), niu_s=niu_s, u_scale=u_scale, c_seed=c_seed, sp_fixture=sp_fixture,
+                  steps=[lab + 1 for lab in labels], rank_counts=counts)
+    out["params"] = np.array(json.dumps(params))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "count =", u.shape[0], "ranks", counts)
+
+
 def main():
     _build_ref()
     sp_case("sp_mrt_16x12x8", (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
@@ -154,6 +238,7 @@ def main():
     cg_case("cg_mixed_gated_14x10x8", (14, 10, 8), 2.5, 0.55, 9, "uniform", 10, nu_l=1.0 / 6.0,
             nu_g=1.0 / 6.0, cos_theta=0.0, cap_a=0.005, g=(2e-5, 1e-5, 0.0), steps_list=[1, 21],
             bodyforce_apply=1)
+    st_case("st6_16x12x8", "sp_mrt_16x12x8", niu_s=0.02, u_scale=100.0, c_seed=21, n_max=20, labels=[5, 20])
 
 
 if __name__ == "__main__":

@@ -235,3 +235,63 @@ def test_error_skips_last_node(oracle_mod):
     ref = np.sqrt(((u - u0) ** 2).sum()) / (np.sqrt((u ** 2).sum()) + 1e-30)
     assert abs(err - ref) < 1e-12 * ref
     assert abs(umax - np.sqrt((u ** 2).sum(1)).max()) < 1e-18
+
+
+# ----------------------------------------------------------------------------- passive solute
+def test_st_oracle_w0_zero_matches_reference_six_velocity_program(oracle_mod):
+    """Rest weight 0 is the reference's six-velocity scheme: from the same initial state the
+    oracle follows the reference's checkpoints (6V.cpp) to round-off."""
+    d, p = load_fixture("st6_16x12x8")
+    o = oracle_mod.STOracle(d["solid"].astype(np.int32), p["niu_s"], d["u"], d["conc_field"], w0=0.0)
+    np.testing.assert_allclose(o.g[1:], d["g_0"], rtol=0, atol=1e-15)   # same initial formula
+    o.g[1:] = d["g_0"]
+    o.C[1:] = d["c_0"]
+    last = 0
+    for steps in p["steps"]:
+        o.run(steps - last)
+        last = steps
+        assert np.abs(o.g[1:] - d[f"g_{steps}"]).max() < 1e-14
+        assert np.abs(o.C[1:] - d[f"c_{steps}"]).max() < 1e-14
+
+
+def _pulse(oracle_mod, L, w0, D, u, solid=None):
+    solid = np.zeros((L, L, L), np.int32) if solid is None else solid
+    conc = np.zeros((L, L, L))
+    conc[L // 2, L // 2, L // 2] = 1.0
+    o = oracle_mod.STOracle(solid, D, np.tile(np.asarray(u, float), (int(np.count_nonzero(solid == 0)), 1)), conc,
+                            w0=w0)
+    xyz = np.stack(np.unravel_index(o.supinv[1:], solid.shape), 1).astype(float)
+    return o, xyz
+
+
+@pytest.mark.parametrize("w0", [0.0, 0.25, 0.5])
+def test_st_pulse_moments_closed_form(oracle_mod, w0):
+    """A unit pulse in an open box, started at equilibrium: the mean moves exactly u t, and with
+    u = 0 the variance per axis grows by c_s^2 [(2 tau - 1) t - 2 tau (tau - 1)(1 - (1 - 1/tau)^t)]
+    (the moment recursion of BGK streaming), i.e. 2 D t at long times with D = c_s^2 (tau - 1/2),
+    c_s^2 = (1 - w0)/3.  A wrong sound speed, weight or equilibrium flux fails one of these."""
+    L, t, D = 48, 20, 0.05
+    cs2 = (1 - w0) / 3
+    u = (0.02, -0.01, 0.005)
+    o, xyz = _pulse(oracle_mod, L, w0, D, u)
+    x0 = (xyz * o.C[1:, None]).sum(0)
+    o.run(t)
+    assert abs(o.mass() - 1.0) < 1e-14
+    np.testing.assert_allclose((xyz * o.C[1:, None]).sum(0) - x0, np.asarray(u) * t, rtol=0, atol=1e-12)
+    o, xyz = _pulse(oracle_mod, L, w0, D, (0.0, 0.0, 0.0))
+    o.run(t)
+    tau = D / cs2 + 0.5
+    expect = cs2 * ((2 * tau - 1) * t - 2 * tau * (tau - 1) * (1 - (1 - 1 / tau) ** t))
+    var = ((xyz - L // 2) ** 2 * o.C[1:, None]).sum(0)
+    np.testing.assert_allclose(var, expect, rtol=1e-12)
+
+
+def test_st_mass_conserved_with_walls(oracle_mod):
+    """Bounce-back at solid sites reflects solute, so the total is invariant (zero-flux walls)."""
+    solid = (np.random.default_rng(3).uniform(size=(18, 14, 12)) < 0.3).astype(np.int32)
+    rng = np.random.default_rng(4)
+    n = int(np.count_nonzero(solid == 0))
+    o = oracle_mod.STOracle(solid, 0.03, rng.uniform(-0.05, 0.05, (n, 3)), rng.uniform(0, 1, solid.shape), w0=0.25)
+    m0 = o.mass()
+    o.run(200)
+    assert abs(o.mass() - m0) < 1e-12 * m0
