@@ -142,6 +142,51 @@ int lbm_cg_get_state(lbm_handle *h, double *f, double *u, double *rho, double *p
 int lbm_cg_diagnostics(lbm_handle *h, const int32_t *box, lbm_cg_diag *out);
 int lbm_cg_steps_done(const lbm_handle *h, int64_t *steps);
 
+/* ------------------------------------------------------------------ passive solute
+ * Advection-diffusion of a passive solute on the fluid nodes: D3Q7 BGK with rest weight w0,
+ * w_q = (1 - w0)/6 for the six axis directions, c_s^2 = (1 - w0)/3, g_eq_q = w_q C (1 + e_q.u/c_s^2),
+ * tau_s = D/c_s^2 + 1/2, halfway bounce-back (zero flux) at solid sites, pull streaming.  With
+ * w0 = 0 this is the reference's six-velocity program (Single_Phase_3D/
+ * Solute_Transport_MPI_Software_Spars_6V.cpp, ST6 below: g_eq = C/6 (1 + 3 e.u), ST6:1306-1310,
+ * 1466-1535, 1625-1640); w0 = 1/4 is the usual D3Q7.  The velocity is either a frozen field
+ * given per node (the reference's Velocity_for_solute files, ST6:3640-3665) or, coupled, the
+ * velocity a single-phase handle computes in the same step. */
+typedef struct lbm_st_params {
+    int32_t nx, ny, nz;       /* box (ignored when coupled: the flow handle's box is used) */
+    double D;                 /* diffusion coefficient niu_s */
+    double w0;                /* rest weight, 0 <= w0 < 1 */
+    int32_t precision;        /* LBM_FP64 or LBM_FP32 (coupled: must match the flow handle's compute type) */
+    int32_t device;
+} lbm_st_params;
+
+typedef struct lbm_st_diag {
+    double mass;              /* sum of C over the fluid nodes (FP64 mass balance) */
+    double cmin, cmax;
+    double moment1[3];        /* sum C x_a over nodes (voxel coordinates, no periodic unwrapping) */
+    double moment2[3];        /* sum C x_a^2 */
+    double nan_count;
+} lbm_st_diag;
+
+/* standalone: own numbering and neighbour lists, frozen velocity (zero until set) */
+int lbm_st_create(const lbm_st_params *p, const int32_t *solid_ijk, lbm_handle **out);
+/* coupled to a single-phase handle: same nodes and neighbour lists on the device; each
+ * lbm_st_step takes one flow step and one solute step with that step's velocity, on the flow
+ * handle's stream.  The flow handle must outlive this one. */
+int lbm_st_create_coupled(const lbm_st_params *p, lbm_handle *flow, lbm_handle **out);
+int lbm_st_count(const lbm_handle *h, int64_t *count);
+/* frozen velocity u[count][3] in node order -- Backup_input_v (ST6:3640-3665) */
+int lbm_st_set_velocity(lbm_handle *h, const double *u);
+/* concentration per voxel, double [i][j][k]; g = g_eq(C, u) -- init (ST6:1249-1320) */
+int lbm_st_init(lbm_handle *h, const double *conc_ijk);
+/* post-streaming state g[count][7] (index 0 = rest) -- Backup_init (ST6:3716-3770) */
+int lbm_st_load_state(lbm_handle *h, const double *g);
+/* nsteps x (collision + streaming + C), the loop of ST6:525-546 (coupled: a flow step each too) */
+int lbm_st_step(lbm_handle *h, int32_t nsteps);
+int lbm_st_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms);
+/* post-streaming g[count][7] and C[count] (either may be NULL) -- Backup (ST6:3681-3714) */
+int lbm_st_get_state(lbm_handle *h, double *g, double *conc);
+int lbm_st_diagnostics(lbm_handle *h, lbm_st_diag *out);
+
 /* ------------------------------------------------------------------ permeability serving
  * A persistent request loop for many independent single-phase permeability solves -- the
  * GPU counterpart of the reference's practice of cutting a rock into sub-samples and running

@@ -48,6 +48,18 @@ class CgDiag(ctypes.Structure):
                 ("sd_g", ctypes.c_double)]
 
 
+class StParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("D", ctypes.c_double), ("w0", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+class StDiag(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double), ("cmin", ctypes.c_double), ("cmax", ctypes.c_double),
+                ("moment1", ctypes.c_double * 3), ("moment2", ctypes.c_double * 3),
+                ("nan_count", ctypes.c_double)]
+
+
 class SrvParams(ctypes.Structure):
     _fields_ = [("capacity_nodes", ctypes.c_int64), ("mem_fraction", ctypes.c_double),
                 ("check_interval", ctypes.c_int32), ("tol", ctypes.c_double), ("min_steps", ctypes.c_int32),
@@ -127,6 +139,16 @@ def lib():
         L.lbm_cg_get_state.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.lbm_cg_diagnostics.argtypes = [vp, ip, ctypes.POINTER(CgDiag)]
         L.lbm_cg_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_st_create.argtypes = [ctypes.POINTER(StParams), ip, ctypes.POINTER(vp)]
+        L.lbm_st_create_coupled.argtypes = [ctypes.POINTER(StParams), vp, ctypes.POINTER(vp)]
+        L.lbm_st_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.lbm_st_set_velocity.argtypes = [vp, dp]
+        L.lbm_st_init.argtypes = [vp, dp]
+        L.lbm_st_load_state.argtypes = [vp, dp]
+        L.lbm_st_step.argtypes = [vp, ctypes.c_int32]
+        L.lbm_st_step_timed.argtypes = [vp, ctypes.c_int32, dp]
+        L.lbm_st_get_state.argtypes = [vp, dp, dp]
+        L.lbm_st_diagnostics.argtypes = [vp, ctypes.POINTER(StDiag)]
         L.lbm_srv_create.argtypes = [ctypes.POINTER(SrvParams), ctypes.POINTER(vp)]
         L.lbm_srv_submit.argtypes = [vp, ctypes.POINTER(SrvRequest), ctypes.POINTER(ctypes.c_int64)]
         L.lbm_srv_poll.argtypes = [vp, ctypes.POINTER(SrvResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]

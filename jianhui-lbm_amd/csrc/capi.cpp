@@ -17,6 +17,11 @@ const lbm::CgSolver *as_cg(const lbm_handle *h)
 {
     return (h && h->kind == lbm_handle::CG) ? static_cast<const lbm::CgSolver *>(h) : nullptr;
 }
+lbm::StSolver *as_st(lbm_handle *h) { return (h && h->kind == lbm_handle::ST) ? static_cast<lbm::StSolver *>(h) : nullptr; }
+const lbm::StSolver *as_st(const lbm_handle *h)
+{
+    return (h && h->kind == lbm_handle::ST) ? static_cast<const lbm::StSolver *>(h) : nullptr;
+}
 }  // namespace
 
 void lbm::set_create_error(const std::string &msg) { g_create_error = msg; }
@@ -188,6 +193,102 @@ int lbm_cg_steps_done(const lbm_handle *h, int64_t *steps)
     if (!s || !steps) return LBM_ERR_ARG;
     *steps = s->steps_done();
     return LBM_OK;
+}
+
+int lbm_st_create(const lbm_st_params *p, const int32_t *solid_ijk, lbm_handle **out)
+{
+    if (!p || !solid_ijk || !out) {
+        g_create_error = "lbm_st_create: null argument";
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    auto *s = new (std::nothrow) lbm::StSolver();
+    if (!s) {
+        g_create_error = "lbm_st_create: out of host memory";
+        return LBM_ERR_NOMEM;
+    }
+    int rc = s->create(*p, solid_ijk);
+    if (rc != LBM_OK) {
+        g_create_error = s->last_error;
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_st_create_coupled(const lbm_st_params *p, lbm_handle *flow, lbm_handle **out)
+{
+    auto *f = as_sp(flow);
+    if (!p || !f || !out) {
+        g_create_error = "lbm_st_create_coupled: null argument or flow handle is not single-phase";
+        return LBM_ERR_ARG;
+    }
+    *out = nullptr;
+    auto *s = new (std::nothrow) lbm::StSolver();
+    if (!s) {
+        g_create_error = "lbm_st_create_coupled: out of host memory";
+        return LBM_ERR_NOMEM;
+    }
+    int rc = s->create_coupled(*p, f);
+    if (rc != LBM_OK) {
+        g_create_error = s->last_error;
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LBM_OK;
+}
+
+int lbm_st_count(const lbm_handle *h, int64_t *count)
+{
+    auto *s = as_st(h);
+    if (!s || !count) return LBM_ERR_ARG;
+    *count = s->count();
+    return LBM_OK;
+}
+
+int lbm_st_set_velocity(lbm_handle *h, const double *u)
+{
+    auto *s = as_st(h);
+    return s ? s->set_velocity(u) : LBM_ERR_ARG;
+}
+
+int lbm_st_init(lbm_handle *h, const double *conc_ijk)
+{
+    auto *s = as_st(h);
+    return s ? s->init(conc_ijk) : LBM_ERR_ARG;
+}
+
+int lbm_st_load_state(lbm_handle *h, const double *g)
+{
+    auto *s = as_st(h);
+    return s ? s->load_state(g) : LBM_ERR_ARG;
+}
+
+int lbm_st_step(lbm_handle *h, int32_t nsteps)
+{
+    auto *s = as_st(h);
+    return s ? s->step(nsteps, nullptr) : LBM_ERR_ARG;
+}
+
+int lbm_st_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms)
+{
+    auto *s = as_st(h);
+    if (!s || !elapsed_ms) return LBM_ERR_ARG;
+    return s->step(nsteps, elapsed_ms);
+}
+
+int lbm_st_get_state(lbm_handle *h, double *g, double *conc)
+{
+    auto *s = as_st(h);
+    return s ? s->get_state(g, conc) : LBM_ERR_ARG;
+}
+
+int lbm_st_diagnostics(lbm_handle *h, lbm_st_diag *out)
+{
+    auto *s = as_st(h);
+    return s ? s->diagnostics(out) : LBM_ERR_ARG;
 }
 
 }  // extern "C"

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double *__
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         is_max[k] = (max_mask >> k) & 1u;
-        v[k] = 0.0;
+        v[k] = is_max[k] ? -INFINITY : 0.0;
     }
     for (int64_t b = threadIdx.x; b < nblk; b += BLOCK) {
 #pragma unroll

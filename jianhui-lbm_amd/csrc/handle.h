@@ -9,7 +9,7 @@
 #include "lbm_mi355x.h"
 
 struct lbm_handle {
-    enum Kind { SP = 1, CG = 2, SERVER = 3 };
+    enum Kind { SP = 1, CG = 2, SERVER = 3, ST = 4 };
     explicit lbm_handle(Kind k) : kind(k) {}
     virtual ~lbm_handle() = default;
     Kind kind;
@@ -98,6 +98,18 @@ public:
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
 
+    // for a coupled passive solute: one step that also stores its velocity, and the shared
+    // device geometry
+    int step_storing_velocity();
+    const void *device_velocity() const { return d_u_; }
+    NbrDev nbr_view() const { return nbr_.view(); }
+    const SparseGeometry &geometry() const { return geo_; }
+    const int32_t *device_supinv() const { return d_supinv_; }
+    hipStream_t stream() const { return stream_; }
+    int device() const { return device_; }
+    size_t compute_bytes() const { return ct_bytes_; }
+    bool initialised() const { return mode_ != STATE_EMPTY; }
+
 private:
     int launch_step(bool store_u);
     int release();
@@ -168,6 +180,52 @@ private:
     int64_t grid_ = 0;
     StateMode mode_ = STATE_EMPTY;
     double u_i_[3] = {0, 0, 0};
+    int64_t steps_ = 0;
+};
+
+class StSolver : public lbm_handle {
+public:
+    StSolver() : lbm_handle(ST) {}
+    ~StSolver() override;
+
+    int create(const lbm_st_params &p, const int32_t *solid);
+    int create_coupled(const lbm_st_params &p, SpSolver *flow);
+    int set_velocity(const double *u);
+    int init(const double *conc_ijk);
+    int load_state(const double *g);
+    int step(int nsteps, double *elapsed_ms);
+    int get_state(double *g, double *conc);
+    int diagnostics(lbm_st_diag *out);
+    int64_t count() const { return N_; }
+
+private:
+    int setup(const lbm_st_params &p);
+    int upload_velocity(const double *u);
+    int launch_step();
+    // the velocity a launch reads: the frozen (or, coupled, initial) field until the first
+    // coupled step, then the one the flow step just stored
+    const void *velocity() const { return (flow_ && mode_ != STATE_INIT) ? flow_->device_velocity() : d_u_own_; }
+    void release();
+
+    lbm_st_params prm_{};
+    SpSolver *flow_ = nullptr;       // coupled: the flow handle (not owned)
+    SparseGeometry geo_;             // standalone geometry (host); coupled: the flow's
+    const SparseGeometry *g_ = nullptr;
+    DeviceNbr own_nbr_;
+    NbrDev nbr_{};
+    int32_t *own_supinv_ = nullptr;
+    const int32_t *d_supinv_ = nullptr;
+    int64_t N_ = 0, P_ = 0, grid_ = 0;
+    int device_ = 0;
+    hipStream_t own_stream_ = nullptr, stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    size_t st_bytes_ = 8;
+    void *pops_[2] = {nullptr, nullptr};
+    int cur_ = 0;
+    void *d_u_own_ = nullptr;        // frozen velocity (coupled: the flow's at init), 3 x P compute type
+    void *d_cinit_ = nullptr;        // P compute type: concentration for the equilibrium start
+    double *d_partial_ = nullptr, *d_result_ = nullptr, *h_result_ = nullptr;
+    StateMode mode_ = STATE_EMPTY;
     int64_t steps_ = 0;
 };
 

@@ -51,4 +51,30 @@ __device__ __forceinline__ void load_nbr(const NbrDev &d, int64_t P, int64_t n, 
     }
 }
 
+// The first K directions only (K = 6: the axis directions q = 1..6, all a D3Q7 pull needs).
+template <int K>
+__device__ __forceinline__ void load_nbr_first(const NbrDev &d, int64_t P, int64_t n, uint32_t (&nb)[K])
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane((uint32_t)(n >> 6));
+    const uint32_t *__restrict__ h = d.hdr + (int64_t)w * HDR_WORDS;
+    const uint32_t slot = h[HDR_WIDE];
+    if (slot == HDR_COMPACT) {
+        uint32_t base[K], r[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) base[q] = h[q];
+#pragma unroll
+        for (int q = 0; q < K; ++q) r[q] = d.res[(int64_t)q * P + n];
+        const uint32_t self_bb = (uint32_t)n | BB_FLAG;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const uint32_t up = base[q] + r[q];
+            nb[q] = r[q] == RES_BB ? self_bb : up;
+        }
+    } else {
+        const uint32_t *__restrict__ wp = d.wide + (int64_t)slot * (18 * 64) + (n & 63);
+#pragma unroll
+        for (int q = 0; q < K; ++q) nb[q] = wp[q * 64];
+    }
+}
+
 }  // namespace lbm

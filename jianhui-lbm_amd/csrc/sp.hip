@@ -428,6 +428,13 @@ int SpSolver::launch_step(bool store_u)
     return LBM_OK;
 }
 
+int SpSolver::step_storing_velocity()
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    return launch_step(true);
+}
+
 int SpSolver::step(int nsteps, double *elapsed_ms)
 {
     if (nsteps < 0) return fail(LBM_ERR_ARG, "negative step count");

@@ -1,0 +1,506 @@
+// st.hip -- passive solute: D3Q7 BGK advection-diffusion on the sparse fluid nodes.
+//
+// The reference's six-velocity solute program (Single_Phase_3D/
+// Solute_Transport_MPI_Software_Spars_6V.cpp, ST6) relaxes g_q = C/6 (1 + 3 e_q.u) with one rate
+// 1/tau_s, tau_s = D/c_s^2 + 1/2 (ST6:1274, 1466-1474), pushes to the downstream node or bounces
+// back into the node's opposite population at a solid site (ST6:1481-1535, zero flux), and
+// sums the populations into C (ST6:1625-1640).  Here the lattice carries a rest population of
+// weight w0: w_q = (1 - w0)/6 on the six axis directions, c_s^2 = (1 - w0)/3,
+// g_eq_q = w_q C (1 + e_q.u / c_s^2); w0 = 0 is the six-velocity scheme itself.
+//
+// One launch per step: every thread owns one node, pulls its 7 populations (the axis rows of
+// the packed D3Q19 pull table are the D3Q7 links), relaxes and writes the post-collision
+// populations.  As in the flow kernels the stored state is post-collision; get_state pulls.
+//
+// Data layout (row stride P): pops[2] 7 x P, u 3 x P (the flow handle's when coupled),
+// c_init P (the concentration of an equilibrium start).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "handle.h"
+#include "lattice.h"
+#include "nbrpack.h"
+#include "pull.h"
+#include "storage.h"
+
+namespace lbm {
+
+constexpr int Q7 = 7;
+
+template <typename C>
+struct StArgs {
+    const C *__restrict__ src;
+    C *__restrict__ dst;
+    NbrDev nbr;
+    const C *__restrict__ u;
+    const C *__restrict__ cinit;
+    int64_t N, P;
+    C omega;     // 1/tau_s
+    C w0, wq;    // rest and axis weights
+    C inv_cs2;
+};
+
+// e_q.u for the axis direction q = 1..6 (+x, -x, +y, -y, +z, -z: the first six D3Q19 rows)
+template <typename C>
+__device__ __forceinline__ C axis_dot(int q, C ux, C uy, C uz)
+{
+    return q == 1 ? ux : q == 2 ? -ux : q == 3 ? uy : q == 4 ? -uy : q == 5 ? uz : -uz;
+}
+
+// the 7 post-streaming populations of node n
+template <typename C, int MODE>
+__device__ __forceinline__ void st_gather(const StArgs<C> &a, int64_t n, C (&g)[Q7])
+{
+    g[0] = a.src[n];
+    if (MODE == K_DIRECT) {
+#pragma unroll
+        for (int q = 1; q < Q7; ++q) g[q] = a.src[(int64_t)q * a.P + n];
+        return;
+    }
+    uint32_t nb[6];
+    load_nbr_first<6>(a.nbr, a.P, n, nb);
+#pragma unroll
+    for (int q = 1; q < Q7; ++q) {
+        const uint32_t v = nb[q - 1];
+        const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+        g[q] = a.src[row * a.P + (int64_t)(v & ~BB_FLAG)];
+    }
+}
+
+template <typename C, int MODE>
+__global__ void __launch_bounds__(BLOCK) st_step_kernel(StArgs<C> a)
+{
+    const int64_t n = stream_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    const C ux = a.u[n], uy = a.u[a.P + n], uz = a.u[2 * a.P + n];
+    C g[Q7], c;
+    if (MODE == K_INIT) {
+        c = a.cinit[n];   // g = g_eq(C, u): its collision leaves it unchanged
+    } else {
+        st_gather<C, MODE>(a, n, g);
+        c = g[0];
+#pragma unroll
+        for (int q = 1; q < Q7; ++q) c += g[q];
+    }
+    const C wc = a.wq * c;
+    {
+        const C eq = a.w0 * c;
+        const C v = MODE == K_INIT ? eq : g[0] - a.omega * (g[0] - eq);
+        __builtin_nontemporal_store(v, &a.dst[n]);
+    }
+#pragma unroll
+    for (int q = 1; q < Q7; ++q) {
+        const C eq = wc * (C(1) + axis_dot(q, ux, uy, uz) * a.inv_cs2);
+        const C v = MODE == K_INIT ? eq : g[q] - a.omega * (g[q] - eq);
+        __builtin_nontemporal_store(v, &a.dst[(int64_t)q * a.P + n]);
+    }
+}
+
+// post-streaming populations and C in the reference's layout (g[count][7], C[count])
+template <typename C, int MODE>
+__global__ void __launch_bounds__(BLOCK) st_get_state_kernel(StArgs<C> a, double *__restrict__ g_out,
+                                                             double *__restrict__ c_out)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+    C g[Q7];
+    if (MODE == K_INIT) {
+        const C c = a.cinit[n], wc = a.wq * c;
+        const C ux = a.u[n], uy = a.u[a.P + n], uz = a.u[2 * a.P + n];
+        g[0] = a.w0 * c;
+#pragma unroll
+        for (int q = 1; q < Q7; ++q) g[q] = wc * (C(1) + axis_dot(q, ux, uy, uz) * a.inv_cs2);
+    } else {
+        st_gather<C, MODE>(a, n, g);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q7; ++q) {
+        if (g_out) g_out[n * Q7 + q] = (double)g[q];
+        s += (double)g[q];
+    }
+    if (c_out) c_out[n] = s;
+}
+
+template <typename C>
+__global__ void __launch_bounds__(BLOCK) st_load_state_kernel(StArgs<C> a, const double *__restrict__ g_in)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= a.N) return;
+#pragma unroll
+    for (int q = 0; q < Q7; ++q) a.dst[(int64_t)q * a.P + n] = (C)g_in[n * Q7 + q];
+}
+
+// mass, min, max, first and second moments, non-finite count (FP64, fixed reduction order)
+constexpr int ST_NDIAG = 10;   // mass -cmin cmax m1[3] m2[3] nan
+constexpr uint32_t ST_MAX_MASK = (1u << 1) | (1u << 2);
+
+template <typename C, int MODE>
+__global__ void __launch_bounds__(BLOCK) st_diag_kernel(StArgs<C> a, const int32_t *__restrict__ supinv, int32_t ny,
+                                                        int32_t nz, double *__restrict__ partial)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    double v[ST_NDIAG];
+#pragma unroll
+    for (int k = 0; k < ST_NDIAG; ++k) v[k] = 0.0;
+    v[1] = v[2] = -INFINITY;
+    if (n < a.N) {
+        double c;
+        if (MODE == K_INIT) {
+            c = (double)a.cinit[n];
+        } else {
+            C g[Q7];
+            st_gather<C, MODE>(a, n, g);
+            c = 0.0;
+#pragma unroll
+            for (int q = 0; q < Q7; ++q) c += (double)g[q];
+        }
+        const int32_t s = supinv[n];
+        const int32_t nyz = ny * nz;
+        const double x[3] = {(double)(s / nyz), (double)((s % nyz) / nz), (double)(s % nz)};
+        if (isfinite(c)) {
+            v[0] = c;
+            v[1] = -c;
+            v[2] = c;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                v[3 + d] = c * x[d];
+                v[6 + d] = c * x[d] * x[d];
+            }
+        } else {
+            v[9] = 1.0;
+        }
+    }
+    const bool is_max[ST_NDIAG] = {false, true, true, false, false, false, false, false, false, false};
+    block_reduce_store<ST_NDIAG>(v, is_max, partial + (int64_t)blockIdx.x * ST_NDIAG);
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+template <typename C>
+StArgs<C> make_st_args(const void *src, void *dst, NbrDev nbr, const void *u, const void *cinit, int64_t N, int64_t P,
+                       const lbm_st_params &p)
+{
+    StArgs<C> a;
+    a.src = (const C *)src;
+    a.dst = (C *)dst;
+    a.nbr = nbr;
+    a.u = (const C *)u;
+    a.cinit = (const C *)cinit;
+    a.N = N;
+    a.P = P;
+    const double cs2 = (1.0 - p.w0) / 3.0;
+    a.omega = (C)(1.0 / (p.D / cs2 + 0.5));
+    a.w0 = (C)p.w0;
+    a.wq = (C)((1.0 - p.w0) / 6.0);
+    a.inv_cs2 = (C)(1.0 / cs2);
+    return a;
+}
+
+template <typename C>
+hipError_t launch_st_t(const StArgs<C> &a, int mode, int64_t grid, hipStream_t s)
+{
+    dim3 g((unsigned)grid), b(BLOCK);
+    if (mode == K_INIT)
+        st_step_kernel<C, K_INIT><<<g, b, 0, s>>>(a);
+    else if (mode == K_DIRECT)
+        st_step_kernel<C, K_DIRECT><<<g, b, 0, s>>>(a);
+    else
+        st_step_kernel<C, K_PULL><<<g, b, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+int st_kmode(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT ? K_DIRECT : K_PULL); }
+
+}  // namespace
+
+StSolver::~StSolver() { release(); }
+
+void StSolver::release()
+{
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *&b : pops_)
+        if (b) (void)hipFree(b), b = nullptr;
+    own_nbr_.release();
+    void **bufs[] = {(void **)&own_supinv_, &d_u_own_, &d_cinit_, (void **)&d_partial_, (void **)&d_result_};
+    for (void **b : bufs)
+        if (*b) (void)hipFree(*b), *b = nullptr;
+    if (h_result_) (void)hipHostFree(h_result_), h_result_ = nullptr;
+    if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
+    if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
+    if (own_stream_) (void)hipStreamDestroy(own_stream_), own_stream_ = nullptr;
+    stream_ = nullptr;
+}
+
+int StSolver::setup(const lbm_st_params &p)
+{
+    if (p.precision != LBM_FP64 && p.precision != LBM_FP32) return fail(LBM_ERR_ARG, "solute supports LBM_FP64 and LBM_FP32");
+    if (!(p.D > 0.0)) return fail(LBM_ERR_ARG, "diffusion coefficient must be positive");
+    if (!(p.w0 >= 0.0 && p.w0 < 1.0)) return fail(LBM_ERR_ARG, "rest weight must be in [0, 1)");
+    st_bytes_ = p.precision == LBM_FP64 ? 8 : 4;
+    grid_ = (N_ + BLOCK - 1) / BLOCK;
+    LBM_HIP_TRY(hipEventCreate(&ev0_));
+    LBM_HIP_TRY(hipEventCreate(&ev1_));
+    for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q7 * P_ * st_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_cinit_, (size_t)P_ * st_bytes_));
+    LBM_HIP_TRY(hipMalloc(&d_partial_, (size_t)grid_ * ST_NDIAG * sizeof(double)));
+    LBM_HIP_TRY(hipMalloc(&d_result_, 16 * sizeof(double)));
+    LBM_HIP_TRY(hipHostMalloc((void **)&h_result_, 16 * sizeof(double), hipHostMallocDefault));
+    return LBM_OK;
+}
+
+int StSolver::create(const lbm_st_params &p, const int32_t *solid)
+{
+    prm_ = p;
+    std::string err;
+    int rc = build_sparse_geometry(p.nx, p.ny, p.nz, solid, geo_, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    g_ = &geo_;
+    N_ = geo_.N;
+    P_ = geo_.P;
+    device_ = p.device;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking));
+    stream_ = own_stream_;
+    rc = setup(p);
+    if (rc != LBM_OK) return rc;
+    PackedNbr packed;
+    rc = pack_neighbours(geo_, packed, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    LBM_HIP_TRY(own_nbr_.upload(packed, P_, stream_));
+    nbr_ = own_nbr_.view();
+    LBM_HIP_TRY(hipMalloc(&own_supinv_, (size_t)P_ * sizeof(int32_t)));
+    LBM_HIP_TRY(hipMemcpyAsync(own_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream_));
+    d_supinv_ = own_supinv_;
+    LBM_HIP_TRY(hipMalloc(&d_u_own_, (size_t)3 * P_ * st_bytes_));
+    LBM_HIP_TRY(hipMemsetAsync(d_u_own_, 0, (size_t)3 * P_ * st_bytes_, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    std::vector<uint32_t>().swap(geo_.nbr);
+    return LBM_OK;
+}
+
+int StSolver::create_coupled(const lbm_st_params &p, SpSolver *flow)
+{
+    prm_ = p;
+    flow_ = flow;
+    g_ = &flow->geometry();
+    N_ = g_->N;
+    P_ = g_->P;
+    prm_.nx = g_->nx;
+    prm_.ny = g_->ny;
+    prm_.nz = g_->nz;
+    device_ = flow->device();
+    if (p.device != device_) return fail(LBM_ERR_ARG, "coupled solute must live on the flow handle's device");
+    const size_t want = p.precision == LBM_FP64 ? 8 : 4;
+    if (flow->compute_bytes() != want)
+        return fail(LBM_ERR_ARG, "coupled solute precision must match the flow handle's compute type");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    stream_ = flow->stream();
+    const int rc = setup(p);
+    if (rc != LBM_OK) return rc;
+    nbr_ = flow->nbr_view();
+    d_supinv_ = flow->device_supinv();
+    LBM_HIP_TRY(hipMalloc(&d_u_own_, (size_t)3 * P_ * st_bytes_));
+    LBM_HIP_TRY(hipMemsetAsync(d_u_own_, 0, (size_t)3 * P_ * st_bytes_, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    return LBM_OK;
+}
+
+int StSolver::set_velocity(const double *u)
+{
+    if (flow_) return fail(LBM_ERR_STATE, "a coupled solute takes the flow handle's velocity");
+    if (!u) return fail(LBM_ERR_ARG, "null velocity");
+    return upload_velocity(u);
+}
+
+// node-major u[count][3] (double) to the 3 x P compute-type buffer d_u_own_
+int StSolver::upload_velocity(const double *u)
+{
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const size_t cnt = (size_t)3 * P_;
+    std::vector<unsigned char> t(cnt * st_bytes_, 0);
+    for (int64_t n = 0; n < N_; ++n)
+        for (int c = 0; c < 3; ++c) {
+            const size_t i = (size_t)c * P_ + n;
+            if (st_bytes_ == 8)
+                ((double *)t.data())[i] = u[n * 3 + c];
+            else
+                ((float *)t.data())[i] = (float)u[n * 3 + c];
+        }
+    LBM_HIP_TRY(hipMemcpyAsync(d_u_own_, t.data(), t.size(), hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    return LBM_OK;
+}
+
+int StSolver::init(const double *conc_ijk)
+{
+    if (!conc_ijk) return fail(LBM_ERR_ARG, "null concentration field");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    if (flow_) {
+        // g = g_eq(C, u) with the velocity the flow's next step collides with; a copy of it
+        // serves every read of the initial state (the flow's own buffer is written only by
+        // its next step)
+        if (!flow_->initialised()) return fail(LBM_ERR_STATE, "flow handle not initialised");
+        std::vector<double> u((size_t)N_ * 3);
+        const int rc = flow_->get_state(nullptr, u.data(), nullptr);
+        if (rc != LBM_OK) return fail(rc, "flow velocity: " + flow_->last_error);
+        const int rc2 = upload_velocity(u.data());
+        if (rc2 != LBM_OK) return rc2;
+    }
+    if (st_bytes_ == 8) {
+        std::vector<double> c((size_t)P_, 0.0);
+        for (int64_t n = 0; n < N_; ++n) c[(size_t)n] = conc_ijk[g_->supinv[(size_t)n]];
+        LBM_HIP_TRY(hipMemcpyAsync(d_cinit_, c.data(), c.size() * 8, hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<float> c((size_t)P_, 0.0f);
+        for (int64_t n = 0; n < N_; ++n) c[(size_t)n] = (float)conc_ijk[g_->supinv[(size_t)n]];
+        LBM_HIP_TRY(hipMemcpyAsync(d_cinit_, c.data(), c.size() * 4, hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    }
+    mode_ = STATE_INIT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+int StSolver::load_state(const double *g)
+{
+    if (!g) return fail(LBM_ERR_ARG, "null state");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    double *dg = nullptr;
+    LBM_HIP_TRY(hipMalloc(&dg, (size_t)N_ * Q7 * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(dg, g, (size_t)N_ * Q7 * sizeof(double), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) {
+        dim3 gr((unsigned)grid_), b(BLOCK);
+        if (st_bytes_ == 8)
+            st_load_state_kernel<double><<<gr, b, 0, stream_>>>(
+                make_st_args<double>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), dg);
+        else
+            st_load_state_kernel<float><<<gr, b, 0, stream_>>>(
+                make_st_args<float>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), dg);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    (void)hipFree(dg);
+    LBM_HIP_TRY(e);
+    mode_ = STATE_DIRECT;
+    steps_ = 0;
+    return LBM_OK;
+}
+
+int StSolver::launch_step()
+{
+    if (flow_) {
+        const int rc = flow_->step_storing_velocity();
+        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
+    }
+    const void *u = velocity();
+    const int nxt = cur_ ^ 1;
+    hipError_t e;
+    if (st_bytes_ == 8)
+        e = launch_st_t(make_st_args<double>(pops_[cur_], pops_[nxt], nbr_, u, d_cinit_, N_, P_, prm_), st_kmode(mode_),
+                        grid_, stream_);
+    else
+        e = launch_st_t(make_st_args<float>(pops_[cur_], pops_[nxt], nbr_, u, d_cinit_, N_, P_, prm_), st_kmode(mode_),
+                        grid_, stream_);
+    LBM_HIP_TRY(e);
+    cur_ = nxt;
+    mode_ = STATE_PULL;
+    ++steps_;
+    return LBM_OK;
+}
+
+int StSolver::step(int nsteps, double *elapsed_ms)
+{
+    if (nsteps < 0) return fail(LBM_ERR_ARG, "negative step count");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    if (flow_ && !flow_->initialised()) return fail(LBM_ERR_STATE, "flow handle not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    if (elapsed_ms) LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
+    for (int s = 0; s < nsteps; ++s) {
+        const int rc = launch_step();
+        if (rc != LBM_OK) return rc;
+    }
+    if (elapsed_ms) {
+        LBM_HIP_TRY(hipEventRecord(ev1_, stream_));
+        LBM_HIP_TRY(hipEventSynchronize(ev1_));
+        float ms = 0.f;
+        LBM_HIP_TRY(hipEventElapsedTime(&ms, ev0_, ev1_));
+        *elapsed_ms = ms;
+    }
+    return LBM_OK;
+}
+
+int StSolver::get_state(double *g, double *conc)
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const void *u = velocity();
+    double *dg = nullptr, *dc = nullptr;
+    if (g) LBM_HIP_TRY(hipMalloc(&dg, (size_t)N_ * Q7 * sizeof(double)));
+    if (conc) LBM_HIP_TRY(hipMalloc(&dc, (size_t)N_ * sizeof(double)));
+    dim3 gr((unsigned)grid_), b(BLOCK);
+    const int mode = st_kmode(mode_);
+    if (st_bytes_ == 8) {
+        auto a = make_st_args<double>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+        if (mode == K_INIT) st_get_state_kernel<double, K_INIT><<<gr, b, 0, stream_>>>(a, dg, dc);
+        else if (mode == K_DIRECT) st_get_state_kernel<double, K_DIRECT><<<gr, b, 0, stream_>>>(a, dg, dc);
+        else st_get_state_kernel<double, K_PULL><<<gr, b, 0, stream_>>>(a, dg, dc);
+    } else {
+        auto a = make_st_args<float>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+        if (mode == K_INIT) st_get_state_kernel<float, K_INIT><<<gr, b, 0, stream_>>>(a, dg, dc);
+        else if (mode == K_DIRECT) st_get_state_kernel<float, K_DIRECT><<<gr, b, 0, stream_>>>(a, dg, dc);
+        else st_get_state_kernel<float, K_PULL><<<gr, b, 0, stream_>>>(a, dg, dc);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && g) e = hipMemcpyAsync(g, dg, (size_t)N_ * Q7 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && conc) e = hipMemcpyAsync(conc, dc, (size_t)N_ * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (dg) (void)hipFree(dg);
+    if (dc) (void)hipFree(dc);
+    LBM_HIP_TRY(e);
+    return LBM_OK;
+}
+
+int StSolver::diagnostics(lbm_st_diag *out)
+{
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const void *u = velocity();
+    dim3 gr((unsigned)grid_), b(BLOCK);
+    const int mode = st_kmode(mode_);
+    if (st_bytes_ == 8) {
+        auto a = make_st_args<double>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+        if (mode == K_INIT) st_diag_kernel<double, K_INIT><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+        else if (mode == K_DIRECT) st_diag_kernel<double, K_DIRECT><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+        else st_diag_kernel<double, K_PULL><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+    } else {
+        auto a = make_st_args<float>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+        if (mode == K_INIT) st_diag_kernel<float, K_INIT><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+        else if (mode == K_DIRECT) st_diag_kernel<float, K_DIRECT><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+        else st_diag_kernel<float, K_PULL><<<gr, b, 0, stream_>>>(a, d_supinv_, g_->ny, g_->nz, d_partial_);
+    }
+    LBM_HIP_TRY(hipGetLastError());
+    reduce_partials_kernel<ST_NDIAG><<<1, BLOCK, 0, stream_>>>(d_partial_, grid_, ST_MAX_MASK, d_result_);
+    LBM_HIP_TRY(hipGetLastError());
+    LBM_HIP_TRY(hipMemcpyAsync(h_result_, d_result_, ST_NDIAG * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    const double *r = h_result_;
+    out->mass = r[0];
+    out->cmin = -r[1];
+    out->cmax = r[2];
+    for (int d = 0; d < 3; ++d) {
+        out->moment1[d] = r[3 + d];
+        out->moment2[d] = r[6 + d];
+    }
+    out->nan_count = r[9];
+    return LBM_OK;
+}
+
+}  // namespace lbm

@@ -8,6 +8,7 @@ library (liblbm_mi355x.so).  There is no CPU fallback: without the library this 
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -57,6 +58,9 @@ class SinglePhase:
 
     # -- lifecycle ---------------------------------------------------------------------------
     def close(self):
+        # a coupled solute uses this handle's device arrays: it goes first
+        for st in list(getattr(self, "_solutes", ())):
+            st.close()
         if getattr(self, "_h", None) is not None and self._h.value:
             _capi.lib().lbm_destroy(self._h)
             self._h = None
@@ -276,3 +280,103 @@ class ColorGradient:
         """S_l = #(psi >= 0) / (V * porosity) over the box (CG:6604-6763)."""
         d = self.diagnostics(box)
         return d["wet_count"] / d["count_in_box"]
+
+
+class Solute:
+    """Passive solute on the fluid nodes: D3Q7 BGK advection-diffusion (the reference's
+    six-velocity program, Single_Phase_3D/Solute_Transport_MPI_Software_Spars_6V.cpp, when
+    ``w0`` = 0).
+
+    Standalone (``solid`` given): the velocity is a frozen per-node field set with
+    ``set_velocity`` (the reference reads it from the flow run's Velocity_for_solute files).
+    Coupled (``flow`` given, a ``SinglePhase``): every ``step`` advances the flow one step and
+    the solute one step with that step's velocity, sharing the flow's node list on the device.
+    """
+
+    def __init__(self, D: float, solid: Optional[np.ndarray] = None, flow: Optional[SinglePhase] = None,
+                 w0: float = 0.25, precision: str = "fp64", device: int = 0):
+        if (solid is None) == (flow is None):
+            raise ValueError("give exactly one of solid (standalone) or flow (coupled)")
+        L = _capi.lib()
+        p = _capi.StParams()
+        p.D = float(D)
+        p.w0 = float(w0)
+        p.precision = _capi.PRECISIONS[precision]
+        p.device = int(device)
+        h = ctypes.c_void_p()
+        if flow is None:
+            solid = np.ascontiguousarray(solid, np.int32)
+            if solid.ndim != 3:
+                raise ValueError("solid must be a 3-D [i][j][k] array")
+            self.shape = solid.shape
+            p.nx, p.ny, p.nz = solid.shape
+            _capi.check(L.lbm_st_create(ctypes.byref(p), solid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        ctypes.byref(h)), None)
+        else:
+            self.shape = flow.shape
+            _capi.check(L.lbm_st_create_coupled(ctypes.byref(p), flow._h, ctypes.byref(h)), None)
+            if not hasattr(flow, "_solutes"):
+                flow._solutes = weakref.WeakSet()
+            flow._solutes.add(self)
+        self._h = h
+        self.flow = flow
+        self.D, self.w0, self.precision = float(D), float(w0), precision
+        self.cs2 = (1.0 - self.w0) / 3.0
+        self.tau = self.D / self.cs2 + 0.5
+        n = ctypes.c_int64()
+        _capi.check(L.lbm_st_count(self._h, ctypes.byref(n)), self._h)
+        self.count = int(n.value)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _capi.lib().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_velocity(self, u: np.ndarray):
+        """Frozen velocity u[count,3] in node order -- Backup_input_v (ST6:3640-3665)."""
+        u = np.ascontiguousarray(u, np.float64).reshape(self.count, 3)
+        _capi.check(_capi.lib().lbm_st_set_velocity(self._h, _dp(u)), self._h)
+
+    def init(self, conc_field: np.ndarray):
+        """g = g_eq(C, u) from a per-voxel concentration ``[i][j][k]`` -- init (ST6:1249-1320)."""
+        c = np.ascontiguousarray(conc_field, np.float64)
+        if c.shape != tuple(self.shape):
+            raise ValueError("concentration field must have the box shape")
+        _capi.check(_capi.lib().lbm_st_init(self._h, _dp(c)), self._h)
+
+    def load_state(self, g: np.ndarray):
+        """Post-streaming g[count,7] (index 0 = rest) -- Backup_init (ST6:3716-3770)."""
+        g = np.ascontiguousarray(g, np.float64).reshape(self.count, 7)
+        _capi.check(_capi.lib().lbm_st_load_state(self._h, _dp(g)), self._h)
+
+    def step(self, nsteps: int = 1):
+        _capi.check(_capi.lib().lbm_st_step(self._h, int(nsteps)), self._h)
+
+    def step_timed(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _capi.check(_capi.lib().lbm_st_step_timed(self._h, int(nsteps), ctypes.byref(ms)), self._h)
+        return ms.value
+
+    def state(self):
+        """(g[count,7], C[count]) post-streaming."""
+        g = np.empty((self.count, 7))
+        c = np.empty(self.count)
+        _capi.check(_capi.lib().lbm_st_get_state(self._h, _dp(g), _dp(c)), self._h)
+        return g, c
+
+    def concentration(self) -> np.ndarray:
+        c = np.empty(self.count)
+        _capi.check(_capi.lib().lbm_st_get_state(self._h, None, _dp(c)), self._h)
+        return c
+
+    def diagnostics(self) -> dict:
+        d = _capi.StDiag()
+        _capi.check(_capi.lib().lbm_st_diagnostics(self._h, ctypes.byref(d)), self._h)
+        return {"mass": d.mass, "cmin": d.cmin, "cmax": d.cmax, "moment1": np.array(d.moment1[:]),
+                "moment2": np.array(d.moment2[:]), "nan_count": d.nan_count}
