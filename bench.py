@@ -215,6 +215,105 @@ def cg_bench(pkg, steps, warmup, precision, edge=192):
     return out
 
 
+# D3Q7 solute, per node: 7 populations read + 7 written, 6 int32 upstream indices
+B_ALG_ST = {"fp32": 2 * 7 * 4 + 6 * 4, "fp64": 2 * 7 * 8 + 6 * 4}
+
+
+def solute_cpu_baseline(solid, nu, g, D, w0, budget_s, edge=128):
+    """The FP64 C oracle, single-phase step then solute step, on a crop of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    crop = np.ascontiguousarray(solid[:edge, :edge, :edge])
+    fo = oracle.SPOracle(crop, nu, g, nthreads=threads)
+    conc = np.zeros(crop.shape)
+    conc[: edge // 8] = 1.0
+    so = oracle.STOracle(crop, D, fo.u[1:].copy(), conc, w0=w0, nthreads=threads)
+    times = []
+    for o in (fo, so):
+        t0 = time.perf_counter()
+        o.run(1)
+        per = max(time.perf_counter() - t0, 1e-3)
+        n = max(1, int(0.5 * budget_s / per))
+        t0 = time.perf_counter()
+        o.run(n)
+        times.append(((time.perf_counter() - t0) / n, n))
+    step_s = times[0][0] + times[1][0]
+    return {"value": round(fo.count / step_s / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 C oracle (OpenMP), single-phase step + solute step on the {edge}^3 corner of the same "
+                      f"fibre pack ({fo.count} fluid nodes): {times[0][1]} flow steps and {times[1][1]} solute steps "
+                      f"after 1 warm-up step each"}
+
+
+def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=False):
+    """Coupled flow + passive solute: a 384^3 periodic box of straight fibres (radius 4 lu,
+    random orientations, porosity 0.8, seed 2027), nu = 1/6, g = (1.165e-4, 0, 0), D3Q7 solute
+    with rest weight 1/4 and D = 0.008, started from a C = 1 slab over the first eighth of the
+    box.  The pack's steady permeability is 11.44 lu^2 (single-phase run to a relative change
+    below 1e-4 per 1000 steps), so the steady pore velocity is 0.0100 lu/step and
+    Pe = u d / D = 10 with the fibre diameter d = 8.  A step is one single-phase launch (which
+    also stores the velocity) and one solute launch with that velocity, on one stream."""
+    geo = pkg.geometry
+    solid = geo.fibre_pack((edge, edge, edge), 4.0, 0.8, seed=2027)
+    nu, g, D, w0 = 1.0 / 6.0, (1.165e-4, 0.0, 0.0), 0.008, 0.25
+    conc = np.zeros(solid.shape)
+    conc[: edge // 8] = 1.0
+    flow = pkg.SinglePhase(solid, nu, g, precision=precision)
+    st = pkg.Solute(D, flow=flow, w0=w0, precision=precision)
+    st.init(conc)
+    N = flow.count
+    st.step(warmup)
+    m0 = st.diagnostics()["mass"]            # synchronises the shared stream
+    t0 = time.perf_counter()
+    ms = st.step_timed(steps)                # HIP events on the shared stream around 2K launches
+    wall = time.perf_counter() - t0
+    d = st.diagnostics()
+    fd = flow.diagnostics()
+    if d["nan_count"] or fd["nan_count"]:
+        raise RuntimeError("non-finite state in the solute benchmark")
+    b_step = B_ALG[precision] + B_ALG_ST[precision]
+    achieved = b_step * N / (ms / 1e3 / steps) / 1e9
+    u_pore = fd["Q"][0] / N
+    out = {"metric": "fluid MLUPS (coupled flow + solute)", "value": round(N * steps / wall / 1e6, 1),
+           "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(wall * 1e3 / steps, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": DTYPE[precision], "data": "synthetic",
+           "config": {"workload": f"sp_st_d3q19_d3q7_{edge}cube_fibres_r4_phi0.8_pe10", "fluid_nodes": N,
+                      "porosity": round(N / solid.size, 5), "precision": precision, "nu": nu, "g": list(g), "D": D,
+                      "w0": w0, "parallelism": "single GPU"},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": read_traffic(f"sp_st_step_{precision}_{edge}cube"), "bytes_per_node": b_step,
+                        "kernels": "sp_step_kernel + st_step_kernel per step",
+                        "avg_step_us": round(ms * 1e3 / steps, 2)},
+           "mass_rel_drift": abs(d["mass"] - m0) / m0, "pore_velocity_at_end": u_pore,
+           "pe_at_end": u_pore * 8.0 / D, "cpu_baseline": None}
+    st.close()
+    flow.close()
+    # the solute kernel alone on the same pack with a frozen velocity field: the reference's
+    # own solute program (velocity read from the flow run's files)
+    fs = pkg.Solute(D, solid=solid, w0=w0, precision=precision)
+    rng = np.random.default_rng(2028)
+    fs.set_velocity(rng.uniform(-0.01, 0.01, (fs.count, 3)))
+    fs.init(conc)
+    fs.step(warmup)
+    fs.diagnostics()
+    t0 = time.perf_counter()
+    ms = fs.step_timed(steps)
+    wall = time.perf_counter() - t0
+    avg = ms / 1e3 / steps
+    ach = B_ALG_ST[precision] * fs.count / avg / 1e9
+    out["solute_kernel"] = {"workload": "frozen velocity, same pack", "value": round(fs.count * steps / wall / 1e6, 1),
+                            "unit": "MLUPS", "kernel": "st_step_kernel", "bytes_per_node": B_ALG_ST[precision],
+                            "avg_launch_us": round(avg * 1e6, 2), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                            "frac": round(ach / HBM_PEAK_GBS, 4),
+                            "traffic": read_traffic(f"st_step_{precision}_{edge}cube")}
+    fs.close()
+    if not no_cpu:
+        out["cpu_baseline"] = solute_cpu_baseline(solid, nu, g, D, w0, cpu_s)
+    return out
+
+
 def accuracy_study(pkg, edge=256, max_steps=100000):
     """The single-phase workload run to steady state (|k_n - k_(n-100)| <= 1e-6 |k_n|, the
     reference's relative-change test at freRe = 100) in FP64, FP32 and FP16 storage; each
@@ -263,7 +362,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
-    ap.add_argument("--workload", default="sp", choices=["sp", "cg"])
+    ap.add_argument("--workload", default="sp", choices=["sp", "cg", "solute"])
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
     args = ap.parse_args()
@@ -278,6 +377,10 @@ def main():
         return
     if args.accuracy:
         print(json.dumps({"accuracy": accuracy_study(pkg, args.edge)}))
+        return
+    if args.workload == "solute":
+        prec = args.precision if args.precision != "fp16" else "fp32"
+        print(json.dumps(solute_bench(pkg, args.steps, args.warmup, prec, cpu_s=args.cpu_seconds, no_cpu=args.no_cpu)))
         return
     if args.workload == "cg":
         print(json.dumps(cg_bench(pkg, args.steps, args.warmup, args.precision if args.precision != "fp16" else "fp32")))

@@ -119,21 +119,52 @@ def grf_sample(edge: int, corr_len: float, porosity: float, seed: int) -> np.nda
 
 
 def fibre_pack(shape, radius: float, porosity: float, seed: int) -> np.ndarray:
-    """Periodic box of straight solid cylinders with random centres and orientations."""
+    """Periodic box of straight solid fibres (cylinders of ``radius``) with random centres and
+    isotropic orientations from ``numpy.random.default_rng(seed)``, added until the pore
+    fraction is <= ``porosity``.
+
+    A fibre with centre c and unit direction v spans one box length along the dominant axis a
+    of v: in every plane x_a = k its axis passes through p_k = c + v d_k / v_a, with d_k the
+    periodic minimum image of k - c_a, and a voxel y of that plane is solid when its distance
+    to the axis line, |y - p_k|^2 - ((y - p_k).v)^2, is at most radius^2.  Indices wrap
+    periodically.  The in-plane cross-section is an ellipse with semi-axes radius and
+    radius/|v_a| <= radius*sqrt(3), so each plane needs only a small window of candidates.
+    """
     shape = tuple(int(s) for s in shape)
+    L = np.asarray(shape, np.float64)
     rng = np.random.default_rng(seed)
     solid = np.zeros(shape, bool)
-    grid = np.stack(np.meshgrid(*[np.arange(n) for n in shape], indexing="ij"), -1).astype(np.float64)
-    L = np.asarray(shape, np.float64)
-    while 1.0 - solid.mean() > porosity:
+    flat = solid.reshape(-1)
+    total = solid.size
+    n_solid = 0
+    r2 = float(radius) ** 2
+    strides = np.array([shape[1] * shape[2], shape[2], 1], np.int64)
+    while (total - n_solid) / total > porosity:
         c = rng.uniform(0, 1, 3) * L
         v = rng.standard_normal(3)
         v /= np.linalg.norm(v)
-        d = grid - c
-        d -= L * np.round(d / L)
-        t = d @ v
-        perp2 = np.einsum("...a,...a->...", d, d) - t * t
-        solid |= perp2 <= radius * radius
+        a = int(np.argmax(np.abs(v)))
+        b, e = [x for x in range(3) if x != a]
+        h = int(np.ceil(radius / abs(v[a]))) + 1
+        if 2 * h + 1 > min(shape[b], shape[e]):
+            raise ValueError("fibre radius too large for the box")
+        k = np.arange(shape[a])
+        dk = k - c[a]
+        dk -= L[a] * np.round(dk / L[a])
+        p = c[None, :] + v[None, :] * (dk / v[a])[:, None]               # axis point per plane
+        off = np.arange(-h, h + 1)
+        yb = np.floor(p[:, b])[:, None, None] + off[None, :, None]        # (La, W, 1)
+        ye = np.floor(p[:, e])[:, None, None] + off[None, None, :]        # (La, 1, W)
+        db = yb - p[:, b, None, None]
+        de = ye - p[:, e, None, None]
+        t = db * v[b] + de * v[e]                                          # y - p_k has no a part
+        inside = db * db + de * de - t * t <= r2
+        kk = np.broadcast_to(k[:, None, None], inside.shape)[inside]
+        ib = np.mod(np.broadcast_to(yb, inside.shape)[inside].astype(np.int64), shape[b])
+        ie = np.mod(np.broadcast_to(ye, inside.shape)[inside].astype(np.int64), shape[e])
+        lin = kk * strides[a] + ib * strides[b] + ie * strides[e]
+        n_solid += int(np.count_nonzero(~flat[lin]))
+        flat[lin] = True
     return solid.astype(np.int32)
 
 

@@ -4,7 +4,8 @@
  * TEST INFRASTRUCTURE ONLY (parity oracle and timed CPU baseline; the HIP product never
  * calls it).  Each function restates one reference routine; the reference line it follows
  * is cited beside it.  SP = Single_Phase_3D/SINGLE_PHASE_MPI_Software_Spars.cpp,
- * CG = MP_3D/Multi_Component_CG_MPI_SPARSE_Software.cpp.
+ * CG = MP_3D/Multi_Component_CG_MPI_SPARSE_Software.cpp,
+ * 6V.cpp = Single_Phase_3D/Solute_Transport_MPI_Software_Spars_6V.cpp (passive solute).
  *
  * Arithmetic is kept in the reference's evaluation order where that is cheap (left-to-right
  * 19-term sums, the same divisions by c_s^2), so that the oracle tracks the reference to
@@ -556,11 +557,14 @@ void orc_st_init(int count, double w0, const double *C, const double *u, double 
 /* nsteps x (collision + streaming + macro): BGK relaxation g* = g - (g - g_eq)/tau_s
  * (6V.cpp:1466-1474), push to the downstream node or bounce back into the opposite population
  * of the node itself when it is solid (6V.cpp:1481-1535), then C = sum of the populations
- * (6V.cpp:1625-1640).  The rest population stays on the node.  C is updated in place. */
+ * (6V.cpp:1625-1640).  The rest population stays on the node.  C is updated in place.
+ * Every slot of G receives exactly one value (from its upstream node or by bounce-back from
+ * the node itself), so the node loop runs in parallel without changing the result. */
 void orc_st_step(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv, double w0,
-                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps)
+                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps, int nthreads)
 {
     for (int step = 0; step < nsteps; step++) {
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
         for (int ci = 1; ci <= count; ci++) {
             int s = supinv[ci];
             int i = s / (ny * nz), j = (s % (ny * nz)) / nz, m = s % nz;
@@ -579,6 +583,7 @@ void orc_st_step(int nx, int ny, int nz, int count, const int32_t *idx, const in
                     G[(long)ci * 7 + ST_LR[q]] = v;
             }
         }
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
         for (int n = 1; n <= count; n++) {
             double c = 0;
             for (int q = 0; q < 7; q++) {

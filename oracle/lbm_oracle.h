@@ -90,7 +90,7 @@ double orc_cg_wet_count(int ny, int nz, int count, const int32_t *supinv, const 
  * padding row 0 like the others: g[(count+1)*7] (index 0 = rest), C[count+1], u[(count+1)*3]. */
 void orc_st_init(int count, double w0, const double *C, const double *u, double *g);
 void orc_st_step(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv, double w0,
-                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps);
+                 double tau_s, const double *u, double *g, double *G, double *C, int nsteps, int nthreads);
 double orc_st_mass(int count, const double *C);
 
 #ifdef __cplusplus

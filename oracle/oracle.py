@@ -224,8 +224,9 @@ class STOracle:
     """Passive solute (D3Q7 BGK, rest weight w0; w0 = 0 is the reference's six-velocity scheme)
     in a frozen velocity field, stepped like the six-velocity program's time loop."""
 
-    def __init__(self, solid, D, u_nodes, conc_field, w0=0.25, u_init=None):
+    def __init__(self, solid, D, u_nodes, conc_field, w0=0.25, u_init=None, nthreads=1):
         self.solid = np.ascontiguousarray(solid, np.int32)
+        self.nthreads = int(nthreads)
         self.nx, self.ny, self.nz = self.solid.shape
         self.count, self.idx, self.supinv = number_fluid(self.solid)
         self.w0 = float(w0)
@@ -245,7 +246,7 @@ class STOracle:
     def run(self, nsteps: int):
         lib().orc_st_step(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv),
                           ctypes.c_double(self.w0), ctypes.c_double(self.tau), _d(self.u), _d(self.g), _d(self.G),
-                          _d(self.C), int(nsteps))
+                          _d(self.C), int(nsteps), self.nthreads)
         self.steps += nsteps
 
     def mass(self) -> float:

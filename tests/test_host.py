@@ -85,6 +85,41 @@ def test_sphere_pack_hits_porosity_and_is_seeded(pkg):
     assert not np.array_equal(a, geo.sphere_pack((32, 32, 32), 4.0, 0.6, seed=6))
 
 
+def _fibres_brute_force(shape, radius, porosity, seed):
+    """Every voxel against every fibre, in-plane displacements by the periodic minimum image
+    (no windows): the same fibre set as geometry.fibre_pack, built the slow way."""
+    L = np.asarray(shape, float)
+    rng = np.random.default_rng(seed)
+    solid = np.zeros(shape, bool)
+    idx = np.stack(np.meshgrid(*[np.arange(n) for n in shape], indexing="ij"), -1).astype(float)
+    while 1.0 - solid.mean() > porosity:
+        c = rng.uniform(0, 1, 3) * L
+        v = rng.standard_normal(3)
+        v /= np.linalg.norm(v)
+        a = int(np.argmax(np.abs(v)))
+        dk = idx[..., a] - c[a]
+        dk -= L[a] * np.round(dk / L[a])
+        d = np.zeros(shape + (3,))
+        for x in range(3):
+            if x == a:
+                continue
+            px = c[x] + v[x] * dk / v[a]
+            d[..., x] = idx[..., x] - px
+            d[..., x] -= L[x] * np.round(d[..., x] / L[x])
+        t = d @ v
+        solid |= (d * d).sum(-1) - t * t <= radius ** 2
+    return solid.astype(np.int32)
+
+
+def test_fibre_pack_matches_brute_force(pkg):
+    for shape, r, phi, seed in (((40, 36, 32), 3.0, 0.85, 5), ((24, 30, 28), 2.0, 0.7, 9)):
+        fast = pkg.geometry.fibre_pack(shape, r, phi, seed=seed)
+        slow = _fibres_brute_force(shape, r, phi, seed)
+        assert 1.0 - fast.mean() <= phi
+        # boundary voxels may flip by round-off in the distance test; none in these cases
+        assert np.array_equal(fast, slow)
+
+
 def test_grf_sample_porosity(pkg):
     s = pkg.geometry.grf_sample(40, 6.0, 0.3, seed=3)
     assert abs(pkg.geometry.porosity(s) - 0.3) < 1e-3
