@@ -279,6 +279,142 @@ void orc_sp_run(int nx, int ny, int nz, int count, const int32_t *idx, const int
     }
 }
 
+/* ------------------------------------------------------------------ SP boundaries */
+/* Pressure and velocity boundaries on the faces of the box, applied to the streamed F between
+ * collision and comput_macro_variables (SP:676-682): boundary_pressure (SP:3076-3620), then
+ * boundary_velocity (SP:2751-3075).  Faces are visited in the reference's order y+, y-, z+,
+ * z-, x+, x-, so the last face that owns a node decides its state.  The inward neighbour nb of
+ * a face node is the next node along the face normal (j = NY-1 for y+, i = 1 for x-, ...), and
+ * f, rho, u at nb are the pre-collision values of this step.  Single-rank semantics: the x
+ * faces are i = 0 and i = nx-1 of the whole box.  The reference's quirks are kept:
+ *   - x- face with Sub_BC 1 and 3: u_ls[2] takes u[nb][1] (SP:3254, 3613);
+ *   - x faces with Sub_BC 2 and a solid nb: the equilibrium is written to the node at k = NZ of
+ *     the same (i, j) column instead of the face node (SP:3451, 3475), which keeps its streamed
+ *     populations; the write to the column's own k = NZ node is overwritten when the loop
+ *     reaches it.
+ * (no_quirks = 1 replaces both by the evident intent; tests use it to show that the reference's
+ * outputs tell the two apart.)
+ * Velocity Sub_BC 2 and 3 are not restated (their x+ face indexes Solid[nx_l-1][k][j],
+ * SP:2953, 3051); the function returns -1 for them. */
+static double feq_u(int q, const int e[19][3], const double w[19], double rho, double ux, double uy, double uz)
+{
+    const double u[3] = {ux, uy, uz};
+    return feq_sp(e[q], w[q], rho, u);
+}
+
+int orc_sp_boundary(int nx, int ny, int nz, const int32_t *idx, const orc_bc *bc, const double *f,
+                    const double *rho, const double *u, double *F)
+{
+    int e[19][3], opp[19];
+    double w[19], M[19][19], Mi[19][19];
+    orc_lattice(0, e, w, opp);
+    orc_moment_matrix(0, M);
+    sp_inverse(M, Mi);
+    const int NZ = nz - 1;
+#define ND(i, j, k) idx[((long)(i) * ny + (j)) * nz + (k)]
+    int any_p = 0, any_v = 0;
+    for (int a = 0; a < 6; a++) {
+        any_p |= bc->pre[a] == 1;
+        any_v |= bc->vel[a] == 1;
+    }
+    if (any_v && (bc->sub_bc == 2 || bc->sub_bc == 3)) return -1;
+    /* faces in visiting order: code, normal axis, face plane, inward neighbour plane */
+    static const int order[6] = {2, 3, 4, 5, 0, 1};   /* yp yn zp zn xp xn in xp,xn,yp,yn,zp,zn numbering */
+    if (any_p) {
+        for (int oi = 0; oi < 6; oi++) {
+            const int fc = order[oi];
+            if (bc->pre[fc] != 1) continue;
+            const double rb = bc->p[fc];
+            const int ax = fc / 2;                   /* 0 x, 1 y, 2 z */
+            const int dim[3] = {nx, ny, nz};
+            const int plane = (fc % 2 == 0) ? dim[ax] - 1 : 0;
+            const int inward = (fc % 2 == 0) ? plane - 1 : 1;
+            const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+            for (int p1 = 0; p1 < dim[a1]; p1++)
+                for (int p2 = 0; p2 < dim[a2]; p2++) {
+                    int c[3], cn[3];
+                    c[ax] = plane;
+                    c[a1] = p1;
+                    c[a2] = p2;
+                    cn[ax] = inward;
+                    cn[a1] = p1;
+                    cn[a2] = p2;
+                    const int b = ND(c[0], c[1], c[2]);
+                    if (b <= 0) continue;
+                    const int nb = ND(cn[0], cn[1], cn[2]);
+                    double *Fb = F + (long)b * 19;
+                    if (bc->sub_bc == 0 || nb <= 0) {
+                        if (bc->sub_bc == 2 && nb <= 0 && ax == 0 && !bc->no_quirks) {
+                            const int b2 = ND(c[0], c[1], NZ);   /* SP:3451, 3475 */
+                            for (int q = 0; q < 19; q++) F[(long)b2 * 19 + q] = feq_u(q, e, w, rb, 0, 0, 0);
+                            continue;
+                        }
+                        for (int q = 0; q < 19; q++) Fb[q] = feq_u(q, e, w, rb, 0, 0, 0);
+                        continue;
+                    }
+                    const double *un = u + (long)nb * 3;
+                    const double ulz = (fc == 1 && !bc->no_quirks) ? un[1] : un[2];   /* x- face: u_ls[2] = u[nb][1] */
+                    if (bc->sub_bc == 1) {
+                        for (int q = 0; q < 19; q++) Fb[q] = feq_u(q, e, w, rb, un[0], un[1], ulz);
+                    } else if (bc->sub_bc == 2) {
+                        const double *fn = f + (long)nb * 19;
+                        double ml[19];
+                        for (int k = 0; k < 19; k++) {
+                            ml[k] = 0;
+                            for (int q = 0; q < 19; q++) ml[k] += M[k][q] * fn[q];
+                        }
+                        ml[0] = rb;
+                        for (int q = 0; q < 19; q++) {
+                            Fb[q] = 0;
+                            for (int k = 0; k < 19; k++) Fb[q] += Mi[q][k] * ml[k];
+                        }
+                    } else {
+                        const double *fn = f + (long)nb * 19;
+                        for (int q = 0; q < 19; q++)
+                            Fb[q] = feq_u(q, e, w, rb, un[0], un[1], ulz) + fn[q] -
+                                    feq_u(q, e, w, rho[nb], un[0], un[1], un[2]);
+                    }
+                }
+        }
+    }
+    if (any_v) {
+        for (int oi = 0; oi < 6; oi++) {
+            const int fc = order[oi];
+            if (bc->vel[fc] != 1) continue;
+            const int ax = fc / 2;
+            double uf[3] = {0, 0, 0};
+            uf[ax] = bc->v[fc];
+            const int dim[3] = {nx, ny, nz};
+            const int plane = (fc % 2 == 0) ? dim[ax] - 1 : 0;
+            const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+            for (int p1 = 0; p1 < dim[a1]; p1++)
+                for (int p2 = 0; p2 < dim[a2]; p2++) {
+                    int c[3];
+                    c[ax] = plane;
+                    c[a1] = p1;
+                    c[a2] = p2;
+                    const int b = ND(c[0], c[1], c[2]);
+                    if (b <= 0) continue;   /* Sub_BC 0 writes the padding record for solid x-face sites */
+                    for (int q = 0; q < 19; q++) F[(long)b * 19 + q] = feq_u(q, e, w, 1.0, uf[0], uf[1], uf[2]);
+                }
+        }
+    }
+#undef ND
+    return 0;
+}
+
+int orc_sp_run_bc(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                  const double S[19], const double g[3], const orc_bc *bc, double *rho, double *u, double *u0,
+                  double *f, double *F, int nsteps, int nthreads)
+{
+    for (int s = 0; s < nsteps; s++) {
+        orc_sp_collide(nx, ny, nz, count, idx, supinv, S, g, rho, u, f, F, nthreads);
+        if (orc_sp_boundary(nx, ny, nz, idx, bc, f, rho, u, F) != 0) return -1;
+        orc_sp_macro(count, g, F, f, rho, u, u0, nthreads);
+    }
+    return 0;
+}
+
 double orc_sp_error(int count, const double *u, const double *u0, double *umax, double *sum_speed)
 {
     /* the loop stops at count-1: the last node is skipped (SP:3651) */

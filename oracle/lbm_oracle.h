@@ -48,6 +48,20 @@ void orc_sp_collide(int nx, int ny, int nz, int count, const int32_t *idx, const
 void orc_sp_macro(int count, const double g[3], const double *F, double *f, double *rho, double *u,
                   double *u0, int nthreads);
 /* n steps of collide + macro (the reference time loop body, SP:662-684) */
+/* face order in the arrays: x+, x-, y+, y-, z+, z- (the reference's input lines) */
+typedef struct orc_bc {
+    int sub_bc;
+    int pre[6];
+    double p[6];
+    int vel[6];
+    double v[6];
+    int no_quirks;   /* 0: the reference's arithmetic; 1: its two quirks corrected (test use) */
+} orc_bc;
+int orc_sp_boundary(int nx, int ny, int nz, const int32_t *idx, const orc_bc *bc, const double *f,
+                    const double *rho, const double *u, double *F);
+int orc_sp_run_bc(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                  const double S[19], const double g[3], const orc_bc *bc, double *rho, double *u, double *u0,
+                  double *f, double *F, int nsteps, int nthreads);
 void orc_sp_run(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
                 const double S[19], const double g[3], double *rho, double *u, double *u0,
                 double *f, double *F, int nsteps, int nthreads);

@@ -44,6 +44,27 @@ def lib():
     return _lib
 
 
+class BC(ctypes.Structure):
+    """Face boundaries in the reference's input order x+, x-, y+, y-, z+, z- (SP:262-267, 280)."""
+    _fields_ = [("sub_bc", ctypes.c_int), ("pre", ctypes.c_int * 6), ("p", ctypes.c_double * 6),
+                ("vel", ctypes.c_int * 6), ("v", ctypes.c_double * 6), ("no_quirks", ctypes.c_int)]
+
+
+def make_bc(sub_bc=0, pre=None, vel=None, no_quirks=False) -> BC:
+    """``pre`` / ``vel``: dicts face -> value, faces named "xp", "xn", "yp", "yn", "zp", "zn"."""
+    faces = ["xp", "xn", "yp", "yn", "zp", "zn"]
+    b = BC()
+    b.sub_bc = int(sub_bc)
+    b.no_quirks = int(bool(no_quirks))
+    for name, val in (pre or {}).items():
+        b.pre[faces.index(name)] = 1
+        b.p[faces.index(name)] = float(val)
+    for name, val in (vel or {}).items():
+        b.vel[faces.index(name)] = 1
+        b.v[faces.index(name)] = float(val)
+    return b
+
+
 def _d(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_dp)
@@ -98,7 +119,8 @@ def sp_relaxation(nu: float):
 class SPOracle:
     """Single-phase state on the CPU, stepped exactly like the reference time loop."""
 
-    def __init__(self, solid, nu, g, u0=(0.0, 0.0, 0.0), rho0=1.0, nthreads=1):
+    def __init__(self, solid, nu, g, u0=(0.0, 0.0, 0.0), rho0=1.0, nthreads=1, bc=None):
+        self.bc = bc
         self.solid = np.ascontiguousarray(solid, np.int32)
         self.nx, self.ny, self.nz = self.solid.shape
         self.count, self.idx, self.supinv = number_fluid(self.solid)
@@ -124,6 +146,14 @@ class SPOracle:
 
     def run(self, nsteps: int):
         L = lib()
+        if getattr(self, "bc", None) is not None:
+            rc = L.orc_sp_run_bc(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv), _d(self.S),
+                                 _d(self.g), ctypes.byref(self.bc), _d(self.rho), _d(self.u), _d(self.u0), _d(self.f),
+                                 _d(self.F), int(nsteps), self.nthreads)
+            if rc != 0:
+                raise ValueError("boundary kind not restated (velocity Sub_BC 2/3)")
+            self.steps += nsteps
+            return
         L.orc_sp_run(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv), _d(self.S),
                      _d(self.g), _d(self.rho), _d(self.u), _d(self.u0), _d(self.f), _d(self.F),
                      int(nsteps), self.nthreads)

@@ -65,11 +65,24 @@ def _text(workdir, fname):
     return open(p).read() if os.path.exists(p) else ""
 
 
-def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None):
+def _bc_fields(inp, bc):
+    """Pressure / velocity boundary lines of the INPUT file (SP:260-267, 280) from
+    {"sub_bc": s, "pre": {face: rho}, "vel": {face: v}}, faces "xp", "xn", "yp", ..."""
+    inp.in_BC = 1
+    inp.Sub_BC = int(bc["sub_bc"])
+    for kind, attr in (("pre", "pre_"), ("vel", "vel_")):
+        vals = bc.get(kind, {})
+        for ax in "xyz":
+            p, n = vals.get(ax + "p"), vals.get(ax + "n")
+            setattr(inp, attr + ax, (1 if p is not None else 0, p if p is not None else (1.0 if kind == "pre" else 0.0),
+                                     1 if n is not None else 0, n if n is not None else (1.0 if kind == "pre" else 0.0)))
+
+
+def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
-                  steps=list(steps_list), radius=radius, porosity_target=por, seed=seed)
+                  steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="sp_fix_")
         try:
@@ -77,6 +90,8 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             inp = inputs.SPInput(filename="geometry.bin", n_xyz=tuple(shape), n_max=steps - 1, reso=1.0,
                                  g=tuple(g), in_vis=nu, ini_v=tuple(u0), wr_per=1, PerDir=1, freRe=freRe,
                                  Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
+            if bc is not None:
+                _bc_fields(inp, bc)
             if box is not None:
                 inp.par_per = (1, 1, 1)
                 inp.per_x = (box[1], box[0])
@@ -227,19 +242,45 @@ def st_case(name, sp_fixture, niu_s, u_scale, c_seed, n_max, labels, freRe=5):
     print("wrote", path, "count =", u.shape[0], "ranks", counts)
 
 
-def main():
+def main(only=()):
+    """Write every fixture, or only those whose names are given on the command line."""
     _build_ref()
-    sp_case("sp_mrt_16x12x8", (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
-            u0=(1e-3, -2e-3, 5e-4), steps_list=[1, 101])
-    sp_case("sp_tau1_box_20x10x12", (20, 10, 12), 2.5, 0.5, 11, nu=1.0 / 6.0, g=(1e-6, 0.0, 0.0),
-            u0=(0.0, 0.0, 0.0), steps_list=[1, 51], box=[4, 15, 1, 8, 2, 10])
-    cg_case("cg_binary_16x12x8", (16, 12, 8), 3.0, 0.6, 5, "binary", 6, nu_l=0.1, nu_g=0.05,
-            cos_theta=0.3, cap_a=0.01, g=(1e-5, 0.0, 0.0), steps_list=[1, 51])
-    cg_case("cg_mixed_gated_14x10x8", (14, 10, 8), 2.5, 0.55, 9, "uniform", 10, nu_l=1.0 / 6.0,
-            nu_g=1.0 / 6.0, cos_theta=0.0, cap_a=0.005, g=(2e-5, 1e-5, 0.0), steps_list=[1, 21],
-            bodyforce_apply=1)
-    st_case("st6_16x12x8", "sp_mrt_16x12x8", niu_s=0.02, u_scale=100.0, c_seed=21, n_max=20, labels=[5, 20])
+    cases = [
+        ("sp_mrt_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
+                                              u0=(1e-3, -2e-3, 5e-4), steps_list=[1, 101])),
+        ("sp_tau1_box_20x10x12", lambda n: sp_case(n, (20, 10, 12), 2.5, 0.5, 11, nu=1.0 / 6.0, g=(1e-6, 0.0, 0.0),
+                                                    u0=(0.0, 0.0, 0.0), steps_list=[1, 51],
+                                                    box=[4, 15, 1, 8, 2, 10])),
+        ("cg_binary_16x12x8", lambda n: cg_case(n, (16, 12, 8), 3.0, 0.6, 5, "binary", 6, nu_l=0.1, nu_g=0.05,
+                                                 cos_theta=0.3, cap_a=0.01, g=(1e-5, 0.0, 0.0), steps_list=[1, 51])),
+        ("cg_mixed_gated_14x10x8", lambda n: cg_case(n, (14, 10, 8), 2.5, 0.55, 9, "uniform", 10, nu_l=1.0 / 6.0,
+                                                      nu_g=1.0 / 6.0, cos_theta=0.0, cap_a=0.005,
+                                                      g=(2e-5, 1e-5, 0.0), steps_list=[1, 21], bodyforce_apply=1)),
+        ("st6_16x12x8", lambda n: st_case(n, "sp_mrt_16x12x8", niu_s=0.02, u_scale=100.0, c_seed=21, n_max=20,
+                                          labels=[5, 20])),
+        # face boundaries (SP:2751-3620), one fixture per rule, u0 != 0 so that the x- face
+        # quirk of Sub_BC 1 and 3 (u_ls[2] = u[nb][1]) shows
+        ("sp_bc_p0_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 31, nu=0.1, g=(0.0, 0.0, 0.0),
+                                                 u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
+                                                 bc={"sub_bc": 0, "pre": {"xp": 1.0, "xn": 1.01}})),
+        ("sp_bc_p1_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 32, nu=0.1, g=(0.0, 1e-5, 0.0),
+                                                 u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
+                                                 bc={"sub_bc": 1, "pre": {"xp": 0.995, "xn": 1.005, "yp": 1.0}})),
+        ("sp_bc_p2_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 33, nu=0.1, g=(0.0, 0.0, 0.0),
+                                                 u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
+                                                 bc={"sub_bc": 2, "pre": {"xp": 1.0, "xn": 1.01, "zn": 1.002}})),
+        ("sp_bc_p3_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 34, nu=0.1, g=(0.0, 0.0, 1e-5),
+                                                 u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
+                                                 bc={"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.01, "yn": 1.0,
+                                                                          "zp": 0.999}})),
+        ("sp_bc_v1_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 35, nu=0.1, g=(0.0, 0.0, 0.0),
+                                                 u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
+                                                 bc={"sub_bc": 1, "pre": {"xp": 1.0}, "vel": {"xn": 0.01, "zp": 0.002}})),
+    ]
+    for name, make in cases:
+        if not only or name in only:
+            make(name)
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))
