@@ -74,6 +74,27 @@ typedef struct lbm_sp_diag {
 int lbm_sp_create(const lbm_sp_params *p, const int32_t *solid_ijk, lbm_handle **out);
 /* number of fluid nodes (Count) */
 int lbm_sp_count(const lbm_handle *h, int64_t *count);
+
+/* Pressure and velocity faces (the INPUT.dat lines 6-11 and Sub_BC, SP:262-267, 280).  Arrays
+ * are indexed x+, x-, y+, y-, z+, z-.  After each streaming step the populations of the fluid
+ * nodes on an active face are replaced as boundary_pressure (SP:3076-3620) and then
+ * boundary_velocity (SP:2751-3075) do, faces in the reference's order y+, y-, z+, z-, x+, x-:
+ *   pressure Sub_BC 0: f = feq(p, 0);  1: f = feq(p, u_nb);  2: f = M^-1 (M f_nb with m0 = p);
+ *            3: f = feq(p, u_nb) + f_nb - feq(rho_nb, u_nb);  (nb = the inward neighbour, its
+ *            pre-collision state of the same step; feq(p, 0) when nb is solid)
+ *   velocity Sub_BC 0, 1: f = feq(1, v n)  (n the face's axis)
+ * with the reference's two quirks (the x- face's u_ls[2] = u_nb[1] in Sub_BC 1 and 3, and no
+ * replacement on the x faces under Sub_BC 2 where nb is solid).  Velocity faces with Sub_BC 2
+ * or 3 are refused (LBM_ERR_ARG).  The streaming itself stays periodic, as in the reference.
+ * Call before stepping (after create, init or load_state); NULL removes the faces. */
+typedef struct lbm_sp_bc {
+    int32_t sub_bc;
+    int32_t pre[6];      /* pressure face on (1) / off (0): pre_xp, pre_xn, pre_yp, ... */
+    double p[6];         /* boundary density p_xp, p_xn, ... */
+    int32_t vel[6];      /* velocity face on / off: vel_xp, vel_xn, ... */
+    double v[6];         /* face-normal velocity v_xp, v_xn, ... */
+} lbm_sp_bc;
+int lbm_sp_set_boundaries(lbm_handle *h, const lbm_sp_bc *bc);
 /* f = feq(rho, u0), u = u0, rho everywhere -- init (SP:1282-1383) */
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3]);
 /* load a post-streaming state, e.g. a reference checkpoint -- Backup_init (SP:4715-4828).

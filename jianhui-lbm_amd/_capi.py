@@ -28,6 +28,11 @@ class SpParams(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("fp16_scale", ctypes.c_double), ("device", ctypes.c_int32)]
 
 
+class SpBC(ctypes.Structure):
+    _fields_ = [("sub_bc", ctypes.c_int32), ("pre", ctypes.c_int32 * 6), ("p", ctypes.c_double * 6),
+                ("vel", ctypes.c_int32 * 6), ("v", ctypes.c_double * 6)]
+
+
 class SpDiag(ctypes.Structure):
     _fields_ = [("Q", ctypes.c_double * 3), ("err", ctypes.c_double), ("umax", ctypes.c_double),
                 ("sum_speed", ctypes.c_double), ("nan_count", ctypes.c_double), ("count_in_box", ctypes.c_double)]
@@ -123,6 +128,7 @@ def lib():
         L.lbm_sp_create.argtypes = [ctypes.POINTER(SpParams), ip, ctypes.POINTER(vp)]
         L.lbm_sp_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.lbm_sp_init.argtypes = [vp, ctypes.c_double, dp]
+        L.lbm_sp_set_boundaries.argtypes = [vp, ctypes.POINTER(SpBC)]
         L.lbm_sp_load_state.argtypes = [vp, dp, dp, dp]
         L.lbm_sp_step.argtypes = [vp, ctypes.c_int32]
         L.lbm_sp_step_timed.argtypes = [vp, ctypes.c_int32, dp]

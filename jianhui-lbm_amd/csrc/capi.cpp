@@ -68,6 +68,12 @@ int lbm_sp_count(const lbm_handle *h, int64_t *count)
     return LBM_OK;
 }
 
+int lbm_sp_set_boundaries(lbm_handle *h, const lbm_sp_bc *bc)
+{
+    auto *s = as_sp(h);
+    return s ? s->set_boundaries(bc) : LBM_ERR_ARG;
+}
+
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3])
 {
     auto *s = as_sp(h);

@@ -81,6 +81,46 @@ struct DeviceNbr {
     NbrDev view() const { return NbrDev{res, hdr, wide}; }
 };
 
+// Face boundaries (lbm_sp_set_boundaries).  Every fluid node on an active face gets one
+// entry with the rule that the reference's face loops leave on it; a rule that reads the
+// inward neighbour's pre-collision state names a slot of the saved-state table, which the
+// step kernel fills for the neighbour nodes (19 deviations, rho - rho0, u) and the next step
+// reads.  Per wave of 64 node ids, two bit masks (boundary lanes, saved lanes) and the index
+// of the wave's first entry and first slot.
+enum BcRule : int32_t { BC_EQ = 0, BC_EQU = 1, BC_MOM = 2, BC_NEQ = 3 };
+struct BcEntry {
+    int32_t rule;
+    int32_t slot;      // saved-state slot of the inward neighbour (BC_EQU / BC_MOM / BC_NEQ)
+    int32_t quirk;     // x- face of Sub_BC 1 / 3: u_z taken from the neighbour's u_y
+    int32_t pad;
+    double val;        // boundary density
+    double uf[3];      // BC_EQ: the equilibrium velocity
+};
+struct BcWave {
+    uint64_t b, s;          // lanes that are boundary nodes / whose state is saved
+    uint32_t bbase, sbase;  // first entry / slot of the wave
+};
+constexpr int BC_ROWS = 23;   // saved state per slot: h[19], rho - rho0, u[3]
+
+struct BoundaryTables {
+    std::vector<BcEntry> entry;    // sorted by node id
+    std::vector<BcWave> wave;      // P / 64
+    std::vector<int32_t> saved;    // node id of each slot, sorted
+};
+
+// Resolve the faces of bc on the geometry (LBM_ERR_ARG for unsupported or malformed input).
+int build_boundaries(const SparseGeometry &g, const lbm_sp_bc &bc, BoundaryTables &out, std::string &err);
+
+// device view; S holds BC_ROWS x sp values of the compute type, one buffer read (the
+// previous step's) and one written per launch
+struct BcDev {
+    const BcWave *__restrict__ wave;   // nullptr: no faces
+    const BcEntry *__restrict__ entry;
+    const void *s_prev;
+    void *s_cur;
+    int64_t sp;
+};
+
 enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
 
 class SpSolver : public lbm_handle {
@@ -94,6 +134,7 @@ public:
     int step(int nsteps, double *elapsed_ms);
     int get_state(double *f, double *u, double *rho);
     int diagnostics(const int32_t *box, lbm_sp_diag *out);
+    int set_boundaries(const lbm_sp_bc *bc);
 
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
@@ -113,6 +154,8 @@ public:
 private:
     int launch_step(bool store_u);
     int release();
+    void release_boundaries();
+    BcDev bc_view(bool for_step) const;
 
     lbm_sp_params prm_{};
     SparseGeometry geo_;
@@ -135,6 +178,12 @@ private:
     StateMode mode_ = STATE_EMPTY;
     double rho_i_ = 1.0, u_i_[3] = {0, 0, 0};
     int64_t steps_ = 0;
+    // face boundaries (none unless set_boundaries was called)
+    BcWave *d_bc_wave_ = nullptr;
+    BcEntry *d_bc_entry_ = nullptr;
+    void *d_bc_s_[2] = {nullptr, nullptr};
+    int64_t bc_sp_ = 0;
+    int bc_cur_ = 0;    // the saved-state buffer the next launch reads
 };
 
 class CgSolver : public lbm_handle {

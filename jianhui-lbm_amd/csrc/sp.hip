@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "boundary.h"
 #include "collide.h"
 #include "common.h"
 #include "handle.h"
@@ -37,6 +38,7 @@ struct SpArgs {
     C gx, gy, gz, rho0, sv, sq;
     C rho_i, ux_i, uy_i, uz_i;
     float scale, inv_scale;
+    BcDev bc;   // face boundaries (bc.wave == nullptr: none)
 };
 
 template <typename ST, int MODE>
@@ -70,7 +72,9 @@ __device__ __forceinline__ Macro<typename Store<ST>::C> node_macro(const SpArgs<
     return mc;
 }
 
-template <typename ST, int MODE, bool STORE_U>
+// BC: face boundaries on.  A pull launch replaces the pulled populations of face nodes by
+// their rule, and every launch saves the pre-collision state of the nodes the rules read.
+template <typename ST, int MODE, bool STORE_U, bool BC>
 __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
 {
     using C = typename Store<ST>::C;
@@ -81,10 +85,17 @@ __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
     if (MODE == K_INIT) {
         mc = node_macro<ST, MODE>(a, n, m);
         sp_eq_moments(mc, m);
+        if (BC && bc_is_saved(a.bc, n)) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) h[q] = feq_dev(q, mc.rho, mc.drho, mc.ux, mc.uy, mc.uz);
+            bc_save(a.bc, n, h, mc.drho, mc.ux, mc.uy, mc.uz);
+        }
     } else {
         gather<ST, MODE>(a, n, h);
+        if (BC && MODE == K_PULL) bc_patch(a.bc, n, a.rho0, h);
         moments_fwd(h, m);
         mc = node_macro<ST, MODE>(a, n, m);
+        if (BC) bc_save(a.bc, n, h, mc.drho, mc.ux, mc.uy, mc.uz);
     }
     if (STORE_U && MODE != K_DIRECT) {
         a.u_aux[n] = mc.ux;
@@ -120,6 +131,7 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
             mc = node_macro<ST, MODE>(a, n, m);
         } else {
             gather<ST, MODE>(a, n, h);
+            if (MODE == K_PULL && a.bc.wave) bc_patch(a.bc, n, a.rho0, h);
             moments_fwd(h, m);
             mc = node_macro<ST, MODE>(a, n, m);
         }
@@ -174,6 +186,7 @@ __global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, doubl
         }
     } else {
         gather<ST, MODE>(a, n, h);
+        if (MODE == K_PULL && a.bc.wave) bc_patch(a.bc, n, a.rho0, h);
         moments_fwd(h, m);
         mc = node_macro<ST, MODE>(a, n, m);
 #pragma unroll
@@ -232,27 +245,41 @@ SpArgs<ST> make_args(const void *src, void *dst, const NbrDev &nbr, void *u_aux,
     a.uz_i = (C)u_i[2];
     a.scale = scale;
     a.inv_scale = inv_scale;
+    a.bc = BcDev{nullptr, nullptr, nullptr, nullptr, 0};
     return a;
 }
 
+template <typename ST, bool BC>
+void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b, hipStream_t s)
+{
+    if (mode == K_INIT)
+        store_u ? sp_step_kernel<ST, K_INIT, true, BC><<<g, b, 0, s>>>(a)
+                : sp_step_kernel<ST, K_INIT, false, BC><<<g, b, 0, s>>>(a);
+    else if (mode == K_DIRECT)
+        sp_step_kernel<ST, K_DIRECT, false, BC><<<g, b, 0, s>>>(a);
+    else
+        store_u ? sp_step_kernel<ST, K_PULL, true, BC><<<g, b, 0, s>>>(a)
+                : sp_step_kernel<ST, K_PULL, false, BC><<<g, b, 0, s>>>(a);
+}
+
 template <typename ST>
-hipError_t launch_step_t(const SpArgs<ST> &a, int mode, bool store_u, int64_t grid, hipStream_t s)
+hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, int mode, bool store_u, int64_t grid, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
-    if (mode == K_INIT)
-        store_u ? sp_step_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : sp_step_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
-    else if (mode == K_DIRECT)
-        sp_step_kernel<ST, K_DIRECT, false><<<g, b, 0, s>>>(a);
+    a.bc = bc;
+    if (bc.wave)
+        launch_step_bc<ST, true>(a, mode, store_u, g, b, s);
     else
-        store_u ? sp_step_kernel<ST, K_PULL, true><<<g, b, 0, s>>>(a) : sp_step_kernel<ST, K_PULL, false><<<g, b, 0, s>>>(a);
+        launch_step_bc<ST, false>(a, mode, store_u, g, b, s);
     return hipGetLastError();
 }
 
 template <typename ST>
-hipError_t launch_diag_t(const SpArgs<ST> &a, int mode, const int32_t *supinv, DiagBox box, double *partial,
+hipError_t launch_diag_t(SpArgs<ST> a, const BcDev &bc, int mode, const int32_t *supinv, DiagBox box, double *partial,
                          int64_t grid, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
+    a.bc = bc;
     if (mode == K_INIT)
         sp_diag_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, supinv, box, partial);
     else if (mode == K_DIRECT)
@@ -263,9 +290,11 @@ hipError_t launch_diag_t(const SpArgs<ST> &a, int mode, const int32_t *supinv, D
 }
 
 template <typename ST>
-hipError_t launch_get_t(const SpArgs<ST> &a, int mode, double *f, double *u, double *rho, int64_t grid, hipStream_t s)
+hipError_t launch_get_t(SpArgs<ST> a, const BcDev &bc, int mode, double *f, double *u, double *rho, int64_t grid,
+                        hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
+    a.bc = bc;
     if (mode == K_INIT)
         sp_get_state_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, f, u, rho);
     else if (mode == K_DIRECT)
@@ -295,6 +324,7 @@ int SpSolver::release()
     if (h_result_) (void)hipHostFree(h_result_), h_result_ = nullptr;
     if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
     if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
+    release_boundaries();
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
     return LBM_OK;
 }
@@ -412,19 +442,64 @@ int SpSolver::launch_step(bool store_u)
     if (prm_.precision == LBM_FP64)
         e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          mode, store_u, grid_, stream_);
+                          bc_view(true), mode, store_u, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
         e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          mode, store_u, grid_, stream_);
+                          bc_view(true), mode, store_u, grid_, stream_);
     else
         e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          mode, store_u, grid_, stream_);
+                          bc_view(true), mode, store_u, grid_, stream_);
     LBM_HIP_TRY(e);
     cur_ = nxt;
+    if (d_bc_wave_) bc_cur_ ^= 1;
     mode_ = STATE_PULL;
     ++steps_;
+    return LBM_OK;
+}
+
+BcDev SpSolver::bc_view(bool for_step) const
+{
+    if (!d_bc_wave_) return BcDev{nullptr, nullptr, nullptr, nullptr, 0};
+    // a launch reads the buffer the previous one wrote; state reads use the same one
+    return BcDev{d_bc_wave_, d_bc_entry_, d_bc_s_[bc_cur_], for_step ? d_bc_s_[bc_cur_ ^ 1] : nullptr, bc_sp_};
+}
+
+void SpSolver::release_boundaries()
+{
+    if (d_bc_wave_) (void)hipFree(d_bc_wave_), d_bc_wave_ = nullptr;
+    if (d_bc_entry_) (void)hipFree(d_bc_entry_), d_bc_entry_ = nullptr;
+    for (void *&b : d_bc_s_)
+        if (b) (void)hipFree(b), b = nullptr;
+    bc_sp_ = 0;
+    bc_cur_ = 0;
+}
+
+int SpSolver::set_boundaries(const lbm_sp_bc *bc)
+{
+    if (mode_ == STATE_PULL)
+        return fail(LBM_ERR_STATE, "set boundaries before stepping: after create, init or load_state");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    release_boundaries();
+    if (!bc) return LBM_OK;
+    BoundaryTables t;
+    std::string err;
+    const int rc = build_boundaries(geo_, *bc, t, err);
+    if (rc != LBM_OK) return fail(rc, err);
+    if (t.entry.empty()) return LBM_OK;   // no fluid node on any active face
+    bc_sp_ = std::max<int64_t>(1, (int64_t)((t.saved.size() + 63) / 64 * 64));
+    LBM_HIP_TRY(hipMalloc((void **)&d_bc_wave_, t.wave.size() * sizeof(BcWave)));
+    LBM_HIP_TRY(hipMalloc((void **)&d_bc_entry_, t.entry.size() * sizeof(BcEntry)));
+    for (void *&b : d_bc_s_) {
+        LBM_HIP_TRY(hipMalloc(&b, (size_t)BC_ROWS * bc_sp_ * ct_bytes_));
+        LBM_HIP_TRY(hipMemsetAsync(b, 0, (size_t)BC_ROWS * bc_sp_ * ct_bytes_, stream_));
+    }
+    LBM_HIP_TRY(hipMemcpyAsync(d_bc_wave_, t.wave.data(), t.wave.size() * sizeof(BcWave), hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemcpyAsync(d_bc_entry_, t.entry.data(), t.entry.size() * sizeof(BcEntry), hipMemcpyHostToDevice,
+                               stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;
 }
 
@@ -471,15 +546,15 @@ int SpSolver::get_state(double *f, double *u, double *rho)
     if (prm_.precision == LBM_FP64)
         e = launch_get_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                            u_i_, scale_, inv_scale_),
-                         mode, df, du, dr, grid_, stream_);
+                         bc_view(false), mode, df, du, dr, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
         e = launch_get_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                           u_i_, scale_, inv_scale_),
-                         mode, df, du, dr, grid_, stream_);
+                         bc_view(false), mode, df, du, dr, grid_, stream_);
     else
         e = launch_get_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
                                            u_i_, scale_, inv_scale_),
-                         mode, df, du, dr, grid_, stream_);
+                         bc_view(false), mode, df, du, dr, grid_, stream_);
     if (e == hipSuccess && f) e = hipMemcpyAsync(f, df, (size_t)N * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, stream_);
@@ -513,15 +588,15 @@ int SpSolver::diagnostics(const int32_t *box, lbm_sp_diag *out)
     if (prm_.precision == LBM_FP64)
         e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                             rho_i_, u_i_, scale_, inv_scale_),
-                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
         e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                            rho_i_, u_i_, scale_, inv_scale_),
-                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
     else
         e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
                                             rho_i_, u_i_, scale_, inv_scale_),
-                          mode, d_supinv_, db, d_partial_, grid_, stream_);
+                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
     LBM_HIP_TRY(e);
     reduce_partials_kernel<NDIAG + 1><<<1, BLOCK, 0, stream_>>>(d_partial_, grid_, 1u << 6, d_result_);
     LBM_HIP_TRY(hipGetLastError());

@@ -77,6 +77,35 @@ class SinglePhase:
     def __exit__(self, *exc):
         self.close()
 
+    # -- faces ---------------------------------------------------------------------------------
+    FACES = ("xp", "xn", "yp", "yn", "zp", "zn")
+
+    def set_boundaries(self, sub_bc: int = 0, pre: Optional[dict] = None, vel: Optional[dict] = None):
+        """Pressure faces ``pre`` {face: density} and velocity faces ``vel`` {face: normal velocity},
+        faces "xp", "xn", "yp", "yn", "zp", "zn", with the reference's Sub_BC (0-3 for pressure,
+        0-1 for velocity) -- boundary_pressure / boundary_velocity (SP:2751-3620).  Call before
+        stepping; ``set_boundaries()`` with no faces removes them."""
+        b = _capi.SpBC()
+        b.sub_bc = int(sub_bc)
+        for name, val in (pre or {}).items():
+            b.pre[self.FACES.index(name)] = 1
+            b.p[self.FACES.index(name)] = float(val)
+        for name, val in (vel or {}).items():
+            b.vel[self.FACES.index(name)] = 1
+            b.v[self.FACES.index(name)] = float(val)
+        _capi.check(_capi.lib().lbm_sp_set_boundaries(self._h, ctypes.byref(b)), self._h)
+        self._pre = dict(pre or {})
+
+    def pressure_gradient(self, PerDir: int = 1) -> float:
+        """dp of Comput_Perm (SP:4600-4614): |p_+ - p_-| c_s^2 / L along PerDir (the INPUT's
+        in_BC = 1 case), 0 without pressure faces."""
+        pre = getattr(self, "_pre", {})
+        ax = {1: 0, 2: 1, 3: 2}.get(PerDir, 0)
+        a = "xyz"[ax]
+        if not pre:
+            return 0.0
+        return abs(pre.get(a + "p", 1.0) - pre.get(a + "n", 1.0)) / 3.0 / self.shape[ax]
+
     # -- state ---------------------------------------------------------------------------------
     def init(self, rho: float = 1.0, u0: Sequence[float] = (0.0, 0.0, 0.0)):
         """f = feq(rho, u0) at every node -- the reference's init (SP:1282-1383)."""
