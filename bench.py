@@ -61,11 +61,12 @@ def read_traffic(tag):
     return None
 
 
-def cpu_baseline(solid, nu, g, budget_s):
+def cpu_baseline(solid, nu, g, budget_s, faces=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    o = oracle.SPOracle(solid, nu, g, nthreads=threads)
+    bc = oracle.make_bc(faces["sub_bc"], faces.get("pre"), faces.get("vel")) if faces else None
+    o = oracle.SPOracle(solid, nu, g, nthreads=threads, bc=bc)
     t0 = time.perf_counter()
     o.run(1)
     t1 = time.perf_counter()
@@ -363,6 +364,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--workload", default="sp", choices=["sp", "cg", "solute"])
+    ap.add_argument("--faces", action="store_true",
+                    help="single-phase workload driven by pressure faces on x (Sub_BC 3) instead of a body force")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
     args = ap.parse_args()
@@ -388,7 +391,15 @@ def main():
     edge = args.edge
     solid = pkg.geometry.sphere_pack((edge, edge, edge), 8.0, 0.35, seed=2024)
     nu, g = 1.0 / 6.0, (1e-6, 0.0, 0.0)
+    faces = None
+    if args.faces:
+        # the same mean driving as g = 1e-6: dp = |p+ - p-| c_s^2 / L = 1e-6 over 256 lu
+        g = (0.0, 0.0, 0.0)
+        faces = {"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.0 + 3e-6 * edge}}
     s = pkg.SinglePhase(solid, nu, g, precision=args.precision, device=0)
+    if faces:
+        s.set_boundaries(faces["sub_bc"], faces["pre"])
+        s.init(1.0, (0.0, 0.0, 0.0))
     N = s.count
     s.step(args.warmup)
     s.diagnostics()                      # synchronises the handle stream
@@ -402,7 +413,8 @@ def main():
     value = N * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = B_ALG[args.precision] * N / avg_launch_s / 1e9
-    traffic = read_traffic(f"sp_step_{args.precision}_256cube") if edge == 256 else None
+    tag = f"sp_step_{args.precision}_256cube" + ("_xfaces" if faces else "")
+    traffic = read_traffic(tag) if edge == 256 else None
     out = {
         "metric": "fluid MLUPS",
         "value": round(value, 1),
@@ -416,9 +428,9 @@ def main():
         "vs_baseline": None,
         "dtype": DTYPE[args.precision],
         "data": "synthetic",
-        "config": {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35", "fluid_nodes": N,
-                   "box": [edge, edge, edge], "precision": args.precision, "tau": 1.0, "g": list(g),
-                   "parallelism": "single GPU"},
+        "config": {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35" + ("_xpressure_faces_subbc3" if faces else ""),
+                   "fluid_nodes": N, "box": [edge, edge, edge], "precision": args.precision, "tau": 1.0, "g": list(g),
+                   "faces": faces, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "sp_step_kernel", "bytes_per_node": B_ALG[args.precision],
@@ -427,7 +439,7 @@ def main():
     }
     s.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds, faces)
     if not args.no_serve:
         out["serving"] = serving_bench(pkg, args.serve_requests, args.serve_rate)
     print(json.dumps(out))

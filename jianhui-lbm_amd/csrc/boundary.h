@@ -21,43 +21,53 @@ __device__ __forceinline__ C feq_dev(int q, C rho, C drho, C ux, C uy, C uz)
     return C(weight(q)) * (drho + rho * (C(3) * eu + C(4.5) * eu * eu - C(1.5) * uu));
 }
 
-// The rule of entry e (deviations h on return).
+// The rule of entry e (deviations h on return).  Every rule fills its own array and one
+// final loop copies it, so that h stays in registers across the divergent rules.
 template <typename C>
 __device__ __forceinline__ void bc_apply(const BcDev &bc, const BcEntry &e, C rho0, C (&h)[Q])
 {
+    const int rule = e.rule;
     const C rb = (C)e.val, drb = (C)(e.val - (double)rho0);
-    if (e.rule == BC_EQ) {
-        const C ux = (C)e.uf[0], uy = (C)e.uf[1], uz = (C)e.uf[2];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) h[q] = feq_dev(q, rb, drb, ux, uy, uz);
-        return;
-    }
     const C *__restrict__ S = (const C *)bc.s_prev;
-    const int64_t sp = bc.sp, s = e.slot;
-    const C nux = S[20 * sp + s], nuy = S[21 * sp + s], nuz = S[22 * sp + s];
-    const C uz_l = e.quirk ? nuy : nuz;   // the x- face's u_ls[2] = u[nb][1] (SP:3254, 3613)
-    if (e.rule == BC_EQU) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q) h[q] = feq_dev(q, rb, drb, nux, nuy, uz_l);
-        return;
-    }
+    const int64_t sp = bc.sp, s = rule == BC_EQ ? 0 : e.slot;
+    C ux, uy, uz, uzl, dn = C(0);
     C hn[Q];
+    if (rule == BC_EQ) {
+        ux = (C)e.uf[0];
+        uy = (C)e.uf[1];
+        uz = uzl = (C)e.uf[2];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) hn[q] = S[q * sp + s];
-    if (e.rule == BC_MOM) {
+        for (int q = 0; q < Q; ++q) hn[q] = C(0);
+    } else {
+        ux = S[20 * sp + s];
+        uy = S[21 * sp + s];
+        uz = S[22 * sp + s];
+        uzl = e.quirk ? uy : uz;   // the x- face's u_ls[2] = u[nb][1] (SP:3254, 3613)
+        dn = S[19 * sp + s];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) hn[q] = S[q * sp + s];
+    }
+    C out[Q];
+    if (rule == BC_MOM) {
         // M^-1 (M f_nb with m0 = rho_b); in deviations only m0 = rho - rho0 changes
         C m[Q];
         moments_fwd(hn, m);
         m[0] = drb;
 #pragma unroll
         for (int k = 0; k < Q; ++k) m[k] = m[k] / C(norm2(k));
-        moments_inv(m, h);
-        return;
-    }
-    // BC_NEQ: feq(rho_b, u_ls) + f_nb - feq(rho_nb, u_nb)
-    const C dn = S[19 * sp + s], rn = rho0 + dn;
+        moments_inv(m, out);
+    } else {
+        // BC_EQ / BC_EQU: feq(rho_b, u_ls); BC_NEQ: + f_nb - feq(rho_nb, u_nb)
+        const bool neq = rule == BC_NEQ;
+        const C rn = rho0 + dn;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) h[q] = feq_dev(q, rb, drb, nux, nuy, uz_l) + hn[q] - feq_dev(q, rn, dn, nux, nuy, nuz);
+        for (int q = 0; q < Q; ++q) {
+            const C b = feq_dev(q, rb, drb, ux, uy, uzl);
+            out[q] = neq ? b + hn[q] - feq_dev(q, rn, dn, ux, uy, uz) : b;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) h[q] = out[q];
 }
 
 // Replace h of node n if it is a face node (post-streaming state of a pull launch).

@@ -6,9 +6,10 @@
 // GPU through the C ABI, and writes the reference's result files under the output prefix:
 // Results.txt, Permeability_error_local_Perm.txt (wr_per = 1), bodyforce.txt,
 // Velocity_ave_max.txt, General_Results.txt, and the checkpoint files of Backup (SP:4507).
-// Features outside the hot path (pressure/velocity boundaries, non-Newtonian collision, VTK
-// output, lattice velocity scaling, group/local permeability) are reported and refused or
-// skipped, never silently approximated.
+// Pressure and velocity faces (INPUT lines 7-12, Sub_BC) run on the device
+// (lbm_sp_set_boundaries); the library refuses the velocity faces of Sub_BC 2 and 3.  Features
+// outside the hot path (non-Newtonian collision, VTK output, lattice velocity scaling,
+// group/local permeability) are reported and refused or skipped, never silently approximated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -71,7 +72,6 @@ int main(int argc, char **argv)
     SpInput in;
     std::string err;
     if (!lbmhost::read_sp_input(input, in, err)) return die(err);
-    if (in.in_BC != 0) return die("pressure/velocity boundaries (in_BC = 1) are not part of this build");
     if (in.nnf_module != 0) return die("the non-Newtonian module is not part of this build");
     if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
         return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
@@ -98,6 +98,30 @@ int main(int argc, char **argv)
     p.device = device;
     lbm_handle *h = nullptr;
     if (check(lbm_sp_create(&p, solid.data(), &h), nullptr, "lbm_sp_create")) return 1;
+    // pressure / velocity faces (INPUT lines 7-12 and Sub_BC; applied whenever a face flag is on,
+    // SP:676-682): pre[axis] = {on+, p+, on-, p-}
+    lbm_sp_bc bc{};
+    bc.sub_bc = in.Sub_BC;
+    bool faces = false;
+    for (int a = 0; a < 3; ++a) {
+        bc.pre[2 * a] = (int32_t)in.pre[a][0];
+        bc.p[2 * a] = in.pre[a][1];
+        bc.pre[2 * a + 1] = (int32_t)in.pre[a][2];
+        bc.p[2 * a + 1] = in.pre[a][3];
+        bc.vel[2 * a] = (int32_t)in.vel[a][0];
+        bc.v[2 * a] = in.vel[a][1];
+        bc.vel[2 * a + 1] = (int32_t)in.vel[a][2];
+        bc.v[2 * a + 1] = in.vel[a][3];
+        faces |= bc.pre[2 * a] == 1 || bc.pre[2 * a + 1] == 1 || bc.vel[2 * a] == 1 || bc.vel[2 * a + 1] == 1;
+    }
+    if (faces && check(lbm_sp_set_boundaries(h, &bc), h, "lbm_sp_set_boundaries")) return 1;
+    // dp of Comput_Perm (SP:4600-4614): the pressure drop along PerDir when in_BC = 1
+    double dp = 0.0;
+    if (in.in_BC != 0) {
+        const int ax = (in.PerDir >= 1 && in.PerDir <= 3) ? in.PerDir - 1 : 0;
+        const int len[3] = {nx, ny, nz};
+        dp = std::fabs(in.pre[ax][1] - in.pre[ax][3]) * (1.0 / 3.0) / len[ax] / 1.0;
+    }
     int64_t count = 0;
     lbm_sp_count(h, &count);
     const auto t_start = std::chrono::steady_clock::now();
@@ -177,7 +201,7 @@ int main(int argc, char **argv)
         u_ave = dfull.sum_speed / (vfull * porosity);
         if (u_max >= 10.0) U_max_ref += 1;
         double Perm[3];
-        for (int a = 0; a < 3; ++a) Perm[a] = dbox.Q[a] / vbox * in.in_vis / (in.g[a] + 0.0);
+        for (int a = 0; a < 3; ++a) Perm[a] = dbox.Q[a] / vbox * in.in_vis / (in.g[a] + dp);
         error_perm = (Perm[dir] - Permia[dir]) / Permia[dir];
         for (int a = 0; a < 3; ++a) Permia[a] = Perm[a];
         {

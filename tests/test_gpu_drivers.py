@@ -141,8 +141,39 @@ def test_lbm_cg_outputs_match_reference(pkg, tmp_path, name):
     assert np.abs(psi - d[f"psi_{last}"]).max() < 1e-12
 
 
+def _bc_over(bc):
+    """INPUT fields of a fixture's faces (SP:260-267, 280)."""
+    over = {"in_BC": 1, "Sub_BC": bc["sub_bc"]}
+    for kind, attr, off in (("pre", "pre_", 1.0), ("vel", "vel_", 0.0)):
+        vals = bc.get(kind, {})
+        for ax in "xyz":
+            pv, nv = vals.get(ax + "p"), vals.get(ax + "n")
+            over[attr + ax] = (int(pv is not None), off if pv is None else pv, int(nv is not None),
+                               off if nv is None else nv)
+    return over
+
+
+@pytest.mark.parametrize("name", ["sp_bc_p0_16x12x10", "sp_bc_p3_16x12x10", "sp_bc_v1_16x12x10"])
+def test_lbm_sp_faces_match_reference(pkg, tmp_path, name):
+    """The driver on an INPUT file with pressure / velocity faces: the reference's printed
+    permeability (with its dp term) and its final checkpoint."""
+    _log(f"[gpu] lbm_sp with faces on {name}")
+    d0, p0 = load_fixture(name)
+    d, p = _sp_case(pkg, tmp_path, name, n_max=p0["steps"][-1] - 1, **_bc_over(p0["bc"]))
+    _run("lbm_sp", str(tmp_path))
+    last = p["steps"][-1]
+    got = _rows(open(tmp_path / "Permeability_error_local_Perm.txt").read())
+    ref = _rows(str(d[f"perm_txt_{last}"]))
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        if np.isfinite(r[0]):
+            assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0])
+    f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
+    assert rel_err(f, d[f"f_{last}"]) < 1e-13
+
+
 def test_driver_refuses_out_of_path_features(pkg, tmp_path):
-    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, in_BC=1)
+    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, nnf_module=1)
     proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
                           text=True, timeout=60)
     assert proc.returncode != 0 and "not part of this build" in proc.stderr
