@@ -197,9 +197,15 @@ static double guo(const int *e, double w, const double *u, const double *g)
 
 static inline int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
-void orc_sp_collide(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
-                    const double S[19], const double g[3], const double *rho, const double *u,
-                    const double *f, double *F, int nthreads)
+/* pl = {nnf_m, nnf_n} switches on the power-law collision (collision_nnf, SP:2017-2700): from
+ * reference step n > 10 on (SP:2397), each node relaxes with its own rates from niu_vn, which it
+ * first updates from its strain rate (SP:2399-2491, the "SRT" estimate of SP:2430-2449):
+ *   s = 1/(niu_vn/c_s^2 + 1/2);  S_ab = -s/(2 rho c_s^2) sum_q e_a e_b (f_q - feq_q)
+ *   niu_vn = m exp((n-1) log sqrt(Sxx^2+Syy^2+Szz^2+2Sxy^2+2Syz^2+2Sxz^2))
+ *   s_v = 1/(niu_vn/c_s^2 + 1/2), s_q = 8(2-s_v)/(8-s_v) in place of S (SP:2466-2491). */
+static void sp_collide_impl(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                            const double S0[19], const double g[3], const double *rho, const double *u,
+                            const double *f, double *F, int nthreads, const double *pl, double *niu_vn, int step)
 {
     int e[19][3], opp[19];
     double w[19], M[19][19], Mi[19][19];
@@ -213,10 +219,37 @@ void orc_sp_collide(int nx, int ny, int nz, int count, const int32_t *idx, const
         int i = s / (ny * nz), j = (s % (ny * nz)) / nz, m = s % nz;
         const double *fc = f + (long)ci * 19;
         const double *uc = u + (long)ci * 3;
-        double gf[19], fe[19], ml[19];
+        double gf[19], fe[19], ml[19], S[19];
         for (int q = 0; q < 19; q++) {
             gf[q] = guo(e[q], w[q], uc, g);
             fe[q] = feq_sp(e[q], w[q], rho[ci], uc);
+            S[q] = S0[q];
+        }
+        if (pl && step > 10) {
+            const double sv0 = 1.0 / (niu_vn[ci] / (C_S2 * 1.0) + 0.5);
+            double sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0;
+            for (int q = 0; q < 19; q++) {
+                const double d = fc[q] - fe[q];
+                sxx += e[q][0] * e[q][0] * d;
+                syy += e[q][1] * e[q][1] * d;
+                szz += e[q][2] * e[q][2] * d;
+                sxy += e[q][0] * e[q][1] * d;
+                syz += e[q][1] * e[q][2] * d;
+                sxz += e[q][0] * e[q][2] * d;
+            }
+            const double c = -sv0 / (2 * rho[ci] * C_S2 * 1.0);
+            sxx *= c;
+            syy *= c;
+            szz *= c;
+            sxy *= c;
+            syz *= c;
+            sxz *= c;
+            niu_vn[ci] = pl[0] * exp((pl[1] - 1) * log(sqrt(sxx * sxx + syy * syy + szz * szz + 2 * sxy * sxy +
+                                                            2 * syz * syz + 2 * sxz * sxz)));
+            const double sv = 1 / (niu_vn[ci] / (C_S2 * 1.0) + 0.5);
+            const double so = 8 * (2 - sv) / (8 - sv);
+            static const int kind[19] = {0, 1, 1, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2};
+            for (int k = 0; k < 19; k++) S[k] = kind[k] == 0 ? 0.0 : (kind[k] == 1 ? sv : so);
         }
         /* m = M f, F_hat = (I - S/2) M GuoF, meq = M feq, relax (SP:1642-1829) */
         for (int k = 0; k < 19; k++) {
@@ -241,6 +274,13 @@ void orc_sp_collide(int nx, int ny, int nz, int count, const int32_t *idx, const
                 F[(long)ci * 19 + opp[q]] = v;
         }
     }
+}
+
+void orc_sp_collide(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                    const double S[19], const double g[3], const double *rho, const double *u,
+                    const double *f, double *F, int nthreads)
+{
+    sp_collide_impl(nx, ny, nz, count, idx, supinv, S, g, rho, u, f, F, nthreads, NULL, NULL, 0);
 }
 
 void orc_sp_macro(int count, const double g[3], const double *F, double *f, double *rho, double *u,
@@ -410,6 +450,20 @@ int orc_sp_run_bc(int nx, int ny, int nz, int count, const int32_t *idx, const i
     for (int s = 0; s < nsteps; s++) {
         orc_sp_collide(nx, ny, nz, count, idx, supinv, S, g, rho, u, f, F, nthreads);
         if (orc_sp_boundary(nx, ny, nz, idx, bc, f, rho, u, F) != 0) return -1;
+        orc_sp_macro(count, g, F, f, rho, u, u0, nthreads);
+    }
+    return 0;
+}
+
+/* The time loop of SP:662-684 with either option: the power-law collision (pl, niu_vn; step0
+ * is the reference's loop counter n of the first step) and the faces (bc).  Either may be NULL. */
+int orc_sp_run_ex(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                  const double S[19], const double g[3], const orc_bc *bc, const double *pl, double *niu_vn, int step0,
+                  double *rho, double *u, double *u0, double *f, double *F, int nsteps, int nthreads)
+{
+    for (int s = 0; s < nsteps; s++) {
+        sp_collide_impl(nx, ny, nz, count, idx, supinv, S, g, rho, u, f, F, nthreads, pl, niu_vn, step0 + s);
+        if (bc && orc_sp_boundary(nx, ny, nz, idx, bc, f, rho, u, F) != 0) return -1;
         orc_sp_macro(count, g, F, f, rho, u, u0, nthreads);
     }
     return 0;

@@ -62,6 +62,9 @@ int orc_sp_boundary(int nx, int ny, int nz, const int32_t *idx, const orc_bc *bc
 int orc_sp_run_bc(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
                   const double S[19], const double g[3], const orc_bc *bc, double *rho, double *u, double *u0,
                   double *f, double *F, int nsteps, int nthreads);
+int orc_sp_run_ex(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
+                  const double S[19], const double g[3], const orc_bc *bc, const double *pl, double *niu_vn, int step0,
+                  double *rho, double *u, double *u0, double *f, double *F, int nsteps, int nthreads);
 void orc_sp_run(int nx, int ny, int nz, int count, const int32_t *idx, const int32_t *supinv,
                 const double S[19], const double g[3], double *rho, double *u, double *u0,
                 double *f, double *F, int nsteps, int nthreads);

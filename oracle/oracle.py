@@ -119,14 +119,16 @@ def sp_relaxation(nu: float):
 class SPOracle:
     """Single-phase state on the CPU, stepped exactly like the reference time loop."""
 
-    def __init__(self, solid, nu, g, u0=(0.0, 0.0, 0.0), rho0=1.0, nthreads=1, bc=None):
+    def __init__(self, solid, nu, g, u0=(0.0, 0.0, 0.0), rho0=1.0, nthreads=1, bc=None, power_law=None):
         self.bc = bc
+        self.power_law = None if power_law is None else np.ascontiguousarray(power_law, np.float64)
         self.solid = np.ascontiguousarray(solid, np.int32)
         self.nx, self.ny, self.nz = self.solid.shape
         self.count, self.idx, self.supinv = number_fluid(self.solid)
         self.nu = float(nu)
         self.g = np.ascontiguousarray(g, np.float64)
         self.S, self.s_v = sp_relaxation(self.nu)
+        self.niu_vn = np.full(self.count + 1, self.nu)   # per-node viscosity of the power-law collision
         n1 = self.count + 1
         self.rho = np.zeros(n1)
         self.u = np.zeros((n1, 3))
@@ -146,6 +148,15 @@ class SPOracle:
 
     def run(self, nsteps: int):
         L = lib()
+        if getattr(self, "power_law", None) is not None:
+            bc = ctypes.byref(self.bc) if self.bc is not None else None
+            rc = L.orc_sp_run_ex(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv), _d(self.S),
+                                 _d(self.g), bc, _d(self.power_law), _d(self.niu_vn), int(self.steps), _d(self.rho),
+                                 _d(self.u), _d(self.u0), _d(self.f), _d(self.F), int(nsteps), self.nthreads)
+            if rc != 0:
+                raise ValueError("boundary kind not restated (velocity Sub_BC 2/3)")
+            self.steps += nsteps
+            return
         if getattr(self, "bc", None) is not None:
             rc = L.orc_sp_run_bc(self.nx, self.ny, self.nz, self.count, _i(self.idx), _i(self.supinv), _d(self.S),
                                  _d(self.g), ctypes.byref(self.bc), _d(self.rho), _d(self.u), _d(self.u0), _d(self.f),

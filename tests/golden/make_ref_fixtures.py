@@ -78,11 +78,12 @@ def _bc_fields(inp, bc):
                                      1 if n is not None else 0, n if n is not None else (1.0 if kind == "pre" else 0.0)))
 
 
-def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None):
+def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
-                  steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc)
+                  steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc,
+                  power_law=power_law)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="sp_fix_")
         try:
@@ -92,6 +93,9 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
                                  Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
             if bc is not None:
                 _bc_fields(inp, bc)
+            if power_law is not None:                 # nnf_module = 1, nnf_m nnf_n (SP:307-308)
+                inp.nnf_module = 1
+                inp.nnf_mn = tuple(power_law)
             if box is not None:
                 inp.par_per = (1, 1, 1)
                 inp.per_x = (box[1], box[0])
@@ -276,6 +280,15 @@ def main(only=()):
         ("sp_bc_v1_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 35, nu=0.1, g=(0.0, 0.0, 0.0),
                                                  u0=(2e-3, 1e-3, -1e-3), steps_list=[1, 2, 21],
                                                  bc={"sub_bc": 1, "pre": {"xp": 1.0}, "vel": {"xn": 0.01, "zp": 0.002}})),
+        # power-law (non-Newtonian) collision, collision_nnf (SP:2017-2700): switched on from
+        # loop step n = 11, so the 12-step state is the first that shows it
+        ("sp_nnf_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-4, 2e-5, 0.0),
+                                              u0=(1e-3, -2e-3, 5e-4), steps_list=[1, 11, 12, 13, 40],
+                                              power_law=(0.02, 0.8))),
+        ("sp_nnf_bc_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 36, nu=0.1, g=(0.0, 0.0, 0.0),
+                                                  u0=(2e-3, 1e-3, -1e-3), steps_list=[12, 30],
+                                                  bc={"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.02}},
+                                                  power_law=(0.05, 1.3))),
     ]
     for name, make in cases:
         if not only or name in only:

@@ -333,3 +333,26 @@ def test_sp_boundary_quirks_are_visible_in_reference_output(oracle_mod, name):
     o = _bc_oracle(oracle_mod, d, p, no_quirks=True)
     o.run(p["steps"][-1])
     assert np.abs(o.f[1:] - d[f"f_{p['steps'][-1]}"]).max() > 1e-9
+
+
+@pytest.mark.parametrize("name", ["sp_nnf_16x12x8", "sp_nnf_bc_16x12x10"])
+def test_sp_power_law_matches_reference(oracle_mod, name):
+    """collision_nnf (SP:2017-2700): Newtonian up to loop step 10, then per-node viscosity from
+    the strain rate.  The oracle follows the reference's checkpoints; the Newtonian oracle
+    follows them up to 11 steps and not after."""
+    d, p = load_fixture(name)
+    bc = p.get("bc")
+    b = oracle_mod.make_bc(bc["sub_bc"], bc.get("pre"), bc.get("vel")) if bc else None
+    solid = d["solid"].astype(np.int32)
+    o = oracle_mod.SPOracle(solid, p["nu"], p["g"], p["u0"], bc=b, power_law=p["power_law"])
+    newt = oracle_mod.SPOracle(solid, p["nu"], p["g"], p["u0"], bc=b)
+    for steps in p["steps"]:
+        o.run(steps - o.steps)
+        newt.run(steps - newt.steps)
+        ref = d[f"f_{steps}"]
+        assert np.abs(o.f[1:] - ref).max() < 1e-14
+        assert np.abs(o.u[1:] - d[f"u_{steps}"]).max() < 1e-14
+        if steps <= 11:
+            assert np.abs(newt.f[1:] - ref).max() < 1e-14
+        else:
+            assert np.abs(newt.f[1:] - ref).max() > 1e-6
