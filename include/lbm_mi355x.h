@@ -95,6 +95,16 @@ typedef struct lbm_sp_bc {
     double v[6];         /* face-normal velocity v_xp, v_xn, ... */
 } lbm_sp_bc;
 int lbm_sp_set_boundaries(lbm_handle *h, const lbm_sp_bc *bc);
+
+/* Power-law fluid (the INPUT's nnf_module = 1 with nnf_m, nnf_n; collision_nnf, SP:2017-2700).
+ * From the 12th step after init / load_state (the reference's loop step n > 10, SP:2397) each
+ * node relaxes with its own viscosity nu = m |S|^(n-1), |S| the norm of the strain rate that
+ * the non-equilibrium populations give with the node's viscosity of the previous step
+ * (SP:2399-2491); init and load_state set every node's viscosity to the handle's nu.  m <= 0
+ * switches it off.  Call before stepping (after create, init or load_state). */
+int lbm_sp_set_power_law(lbm_handle *h, double m, double n);
+/* per-node viscosity niu_vn[count] (the handle's nu everywhere without the power law) */
+int lbm_sp_get_viscosity(lbm_handle *h, double *nu);
 /* f = feq(rho, u0), u = u0, rho everywhere -- init (SP:1282-1383) */
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3]);
 /* load a post-streaming state, e.g. a reference checkpoint -- Backup_init (SP:4715-4828).

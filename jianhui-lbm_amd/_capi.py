@@ -129,6 +129,8 @@ def lib():
         L.lbm_sp_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.lbm_sp_init.argtypes = [vp, ctypes.c_double, dp]
         L.lbm_sp_set_boundaries.argtypes = [vp, ctypes.POINTER(SpBC)]
+        L.lbm_sp_set_power_law.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.lbm_sp_get_viscosity.argtypes = [vp, dp]
         L.lbm_sp_load_state.argtypes = [vp, dp, dp, dp]
         L.lbm_sp_step.argtypes = [vp, ctypes.c_int32]
         L.lbm_sp_step_timed.argtypes = [vp, ctypes.c_int32, dp]

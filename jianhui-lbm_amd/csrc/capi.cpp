@@ -74,6 +74,18 @@ int lbm_sp_set_boundaries(lbm_handle *h, const lbm_sp_bc *bc)
     return s ? s->set_boundaries(bc) : LBM_ERR_ARG;
 }
 
+int lbm_sp_set_power_law(lbm_handle *h, double m, double n)
+{
+    auto *s = as_sp(h);
+    return s ? s->set_power_law(m, n) : LBM_ERR_ARG;
+}
+
+int lbm_sp_get_viscosity(lbm_handle *h, double *nu)
+{
+    auto *s = as_sp(h);
+    return s ? s->get_viscosity(nu) : LBM_ERR_ARG;
+}
+
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3])
 {
     auto *s = as_sp(h);

@@ -135,6 +135,8 @@ public:
     int get_state(double *f, double *u, double *rho);
     int diagnostics(const int32_t *box, lbm_sp_diag *out);
     int set_boundaries(const lbm_sp_bc *bc);
+    int set_power_law(double m, double n);
+    int get_viscosity(double *nu);
 
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
@@ -156,6 +158,7 @@ private:
     int release();
     void release_boundaries();
     BcDev bc_view(bool for_step) const;
+    int reset_viscosity();
 
     lbm_sp_params prm_{};
     SparseGeometry geo_;
@@ -184,6 +187,9 @@ private:
     void *d_bc_s_[2] = {nullptr, nullptr};
     int64_t bc_sp_ = 0;
     int bc_cur_ = 0;    // the saved-state buffer the next launch reads
+    // power-law collision (none unless set_power_law was called)
+    void *d_nu_vn_ = nullptr;   // P compute type
+    double pl_m_ = 0.0, pl_n_ = 0.0;
 };
 
 class CgSolver : public lbm_handle {

@@ -96,6 +96,7 @@ bool read_sp_input(const std::string &path, SpInput &in, std::string &err)
          get_int(L, 33, 0, in.fre_backup, err) && get_int(L, 34, 0, in.mode_backup_ini, err) &&
          get_int(L, 35, 0, in.vel_sol, err) && get_int(L, 37, 0, in.gperm, err) && get_int(L, 45, 0, in.decbin, err) &&
          get_int(L, 47, 0, in.nnf_module, err);
+    if (ok && in.nnf_module == 1) ok = get_dbl(L, 48, 0, in.nnf_m, err) && get_dbl(L, 48, 1, in.nnf_n, err);
     return ok;
 }
 

@@ -30,6 +30,7 @@ struct SpInput {
     int gperm = 0;
     int decbin = 0;
     int nnf_module = 0;
+    double nnf_m = 0.0, nnf_n = 1.0;   // line 48, read when nnf_module = 1 (SP:307-308)
 };
 
 // Colour-gradient INPUT_CG_SPARSE.dat, the line map of CG:371-451

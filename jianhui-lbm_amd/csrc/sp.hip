@@ -39,7 +39,43 @@ struct SpArgs {
     C rho_i, ux_i, uy_i, uz_i;
     float scale, inv_scale;
     BcDev bc;   // face boundaries (bc.wave == nullptr: none)
+    C *__restrict__ nu_vn;   // power-law collision: per-node viscosity (nullptr: Newtonian)
+    C pl_m, pl_n;            // nu = m |S|^(n-1)
 };
+
+// Power-law collision (collision_nnf, SP:2399-2491): update the node's viscosity from its
+// strain rate, estimated from the non-equilibrium populations with the viscosity of the
+// previous step, and return the rates s_v, s_q of the new viscosity.
+template <typename C>
+__device__ __forceinline__ void power_law_rates(const C (&h)[Q], const Macro<C> &mc, C *__restrict__ nu_vn, int64_t n,
+                                                C pl_m, C pl_n, C &sv, C &sq)
+{
+    const C cs2 = C(1.0 / 3.0);
+    const C sv0 = C(1) / (nu_vn[n] / cs2 + C(0.5));
+    C sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const C d = h[q] - feq_dev(q, mc.rho, mc.drho, mc.ux, mc.uy, mc.uz);   // f - feq
+        if (EX[q] != 0) sxx += d;
+        if (EY[q] != 0) syy += d;
+        if (EZ[q] != 0) szz += d;
+        if (EX[q] * EY[q] != 0) sxy += C(EX[q] * EY[q]) * d;
+        if (EY[q] * EZ[q] != 0) syz += C(EY[q] * EZ[q]) * d;
+        if (EX[q] * EZ[q] != 0) sxz += C(EX[q] * EZ[q]) * d;
+    }
+    const C c = -sv0 / (C(2) * mc.rho * cs2);
+    sxx *= c;
+    syy *= c;
+    szz *= c;
+    sxy *= c;
+    syz *= c;
+    sxz *= c;
+    const C s2 = sxx * sxx + syy * syy + szz * szz + C(2) * sxy * sxy + C(2) * syz * syz + C(2) * sxz * sxz;
+    const C nu = pl_m * exp((pl_n - C(1)) * log(sqrt(s2)));
+    nu_vn[n] = nu;
+    sv = C(1) / (nu / cs2 + C(0.5));
+    sq = C(8) * (C(2) - sv) / (C(8) - sv);
+}
 
 template <typename ST, int MODE>
 __device__ __forceinline__ void gather(const SpArgs<ST> &a, int64_t n, typename Store<ST>::C (&h)[Q])
@@ -74,7 +110,8 @@ __device__ __forceinline__ Macro<typename Store<ST>::C> node_macro(const SpArgs<
 
 // BC: face boundaries on.  A pull launch replaces the pulled populations of face nodes by
 // their rule, and every launch saves the pre-collision state of the nodes the rules read.
-template <typename ST, int MODE, bool STORE_U, bool BC>
+// PL: power-law collision (pull launches of reference steps n > 10, SP:2397).
+template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false>
 __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
 {
     using C = typename Store<ST>::C;
@@ -102,9 +139,18 @@ __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
         a.u_aux[a.P + n] = mc.uy;
         a.u_aux[2 * a.P + n] = mc.uz;
     }
-    sp_relax(m, mc, a.gx, a.gy, a.gz, a.sv, a.sq, h);
+    C sv = a.sv, sq = a.sq;
+    if (PL && MODE == K_PULL) power_law_rates(h, mc, a.nu_vn, n, a.pl_m, a.pl_n, sv, sq);
+    sp_relax(m, mc, a.gx, a.gy, a.gz, sv, sq, h);
 #pragma unroll
     for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&a.dst[(int64_t)q * a.P + n], h[q], a.scale);
+}
+
+template <typename C>
+__global__ void __launch_bounds__(BLOCK) fill_kernel(C *__restrict__ p, C v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -246,13 +292,18 @@ SpArgs<ST> make_args(const void *src, void *dst, const NbrDev &nbr, void *u_aux,
     a.scale = scale;
     a.inv_scale = inv_scale;
     a.bc = BcDev{nullptr, nullptr, nullptr, nullptr, 0};
+    a.nu_vn = nullptr;
+    a.pl_m = a.pl_n = C(0);
     return a;
 }
 
 template <typename ST, bool BC>
 void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b, hipStream_t s)
 {
-    if (mode == K_INIT)
+    if (mode == K_PULL && a.nu_vn)
+        store_u ? sp_step_kernel<ST, K_PULL, true, BC, true><<<g, b, 0, s>>>(a)
+                : sp_step_kernel<ST, K_PULL, false, BC, true><<<g, b, 0, s>>>(a);
+    else if (mode == K_INIT)
         store_u ? sp_step_kernel<ST, K_INIT, true, BC><<<g, b, 0, s>>>(a)
                 : sp_step_kernel<ST, K_INIT, false, BC><<<g, b, 0, s>>>(a);
     else if (mode == K_DIRECT)
@@ -263,10 +314,15 @@ void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b,
 }
 
 template <typename ST>
-hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, int mode, bool store_u, int64_t grid, hipStream_t s)
+hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m, double pl_n, int mode, bool store_u,
+                         int64_t grid, hipStream_t s)
 {
+    using C = typename Store<ST>::C;
     dim3 g((unsigned)grid), b(BLOCK);
     a.bc = bc;
+    a.nu_vn = (C *)nu_vn;
+    a.pl_m = (C)pl_m;
+    a.pl_n = (C)pl_n;
     if (bc.wave)
         launch_step_bc<ST, true>(a, mode, store_u, g, b, s);
     else
@@ -325,6 +381,7 @@ int SpSolver::release()
     if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
     if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
     release_boundaries();
+    if (d_nu_vn_) (void)hipFree(d_nu_vn_), d_nu_vn_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
     return LBM_OK;
 }
@@ -387,7 +444,7 @@ int SpSolver::init(double rho, const double u0[3])
     for (int c = 0; c < 3; ++c) u_i_[c] = u0[c];
     mode_ = STATE_INIT;
     steps_ = 0;
-    return LBM_OK;
+    return reset_viscosity();
 }
 
 #define SP_DISPATCH(EXPR_DOUBLE, EXPR_FLOAT, EXPR_HALF)                          \
@@ -431,26 +488,31 @@ int SpSolver::load_state(const double *f, const double *u, const double *rho)
     LBM_HIP_TRY(e);
     mode_ = STATE_DIRECT;
     steps_ = 0;
-    return LBM_OK;
+    // the reference's restart leaves niu_vn unset (it is set in init, SP:1355); here it starts
+    // from the Newtonian viscosity as after init
+    return reset_viscosity();
 }
 
 int SpSolver::launch_step(bool store_u)
 {
     const int mode = kmode_of(mode_);
+    // the power-law collision acts from the reference's loop step n = 11 on (SP:2397); the
+    // launches since init or load_state are that loop's n
+    void *pl_nu = (d_nu_vn_ && steps_ > 10) ? d_nu_vn_ : nullptr;
     const int nxt = cur_ ^ 1;
     hipError_t e;
     if (prm_.precision == LBM_FP64)
         e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
     else if (prm_.precision == LBM_FP32)
         e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
     else
         e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
     LBM_HIP_TRY(e);
     cur_ = nxt;
     if (d_bc_wave_) bc_cur_ ^= 1;
@@ -500,6 +562,60 @@ int SpSolver::set_boundaries(const lbm_sp_bc *bc)
     LBM_HIP_TRY(hipMemcpyAsync(d_bc_entry_, t.entry.data(), t.entry.size() * sizeof(BcEntry), hipMemcpyHostToDevice,
                                stream_));
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    return LBM_OK;
+}
+
+int SpSolver::reset_viscosity()
+{
+    if (!d_nu_vn_) return LBM_OK;
+    const dim3 g((unsigned)grid_), b(BLOCK);
+    if (ct_bytes_ == 8)
+        fill_kernel<double><<<g, b, 0, stream_>>>((double *)d_nu_vn_, prm_.nu, geo_.P);
+    else
+        fill_kernel<float><<<g, b, 0, stream_>>>((float *)d_nu_vn_, (float)prm_.nu, geo_.P);
+    LBM_HIP_TRY(hipGetLastError());
+    return LBM_OK;
+}
+
+int SpSolver::set_power_law(double m, double n)
+{
+    if (mode_ == STATE_PULL)
+        return fail(LBM_ERR_STATE, "set the power law before stepping: after create, init or load_state");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    if (!(m > 0.0)) {   // back to the Newtonian collision
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+        if (d_nu_vn_) (void)hipFree(d_nu_vn_), d_nu_vn_ = nullptr;
+        pl_m_ = pl_n_ = 0.0;
+        return LBM_OK;
+    }
+    if (!std::isfinite(n)) return fail(LBM_ERR_ARG, "power-law index must be finite");
+    if (!d_nu_vn_) LBM_HIP_TRY(hipMalloc(&d_nu_vn_, (size_t)geo_.P * ct_bytes_));
+    pl_m_ = m;
+    pl_n_ = n;
+    const int rc = reset_viscosity();   // niu_vn = niu (SP:1355)
+    if (rc != LBM_OK) return rc;
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    return LBM_OK;
+}
+
+int SpSolver::get_viscosity(double *nu)
+{
+    if (!nu) return fail(LBM_ERR_ARG, "null output");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N;
+    if (!d_nu_vn_) {
+        for (int64_t i = 0; i < N; ++i) nu[i] = prm_.nu;
+        return LBM_OK;
+    }
+    if (ct_bytes_ == 8) {
+        LBM_HIP_TRY(hipMemcpyAsync(nu, d_nu_vn_, (size_t)N * 8, hipMemcpyDeviceToHost, stream_));
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<float> t((size_t)N);
+        LBM_HIP_TRY(hipMemcpyAsync(t.data(), d_nu_vn_, (size_t)N * 4, hipMemcpyDeviceToHost, stream_));
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+        for (int64_t i = 0; i < N; ++i) nu[i] = t[(size_t)i];
+    }
     return LBM_OK;
 }
 

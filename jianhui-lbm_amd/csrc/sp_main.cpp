@@ -7,9 +7,10 @@
 // Results.txt, Permeability_error_local_Perm.txt (wr_per = 1), bodyforce.txt,
 // Velocity_ave_max.txt, General_Results.txt, and the checkpoint files of Backup (SP:4507).
 // Pressure and velocity faces (INPUT lines 7-12, Sub_BC) run on the device
-// (lbm_sp_set_boundaries); the library refuses the velocity faces of Sub_BC 2 and 3.  Features
-// outside the hot path (non-Newtonian collision, VTK output, lattice velocity scaling,
-// group/local permeability) are reported and refused or skipped, never silently approximated.
+// (lbm_sp_set_boundaries); the library refuses the velocity faces of Sub_BC 2 and 3.  The
+// power-law collision (nnf_module = 1) runs on the device too (lbm_sp_set_power_law).  Features
+// outside the hot path (VTK output, lattice velocity scaling, group/local permeability) are
+// reported and refused or skipped, never silently approximated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -72,7 +73,7 @@ int main(int argc, char **argv)
     SpInput in;
     std::string err;
     if (!lbmhost::read_sp_input(input, in, err)) return die(err);
-    if (in.nnf_module != 0) return die("the non-Newtonian module is not part of this build");
+    if (in.nnf_module != 0 && in.nnf_module != 1) return die("nnf_module must be 0 or 1");
     if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
         return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
     if (in.freVe > 0 || in.freDe > 0 || in.vel_sol > 0 || in.gperm > 0 || in.loc_perm == 1)
@@ -115,6 +116,8 @@ int main(int argc, char **argv)
         faces |= bc.pre[2 * a] == 1 || bc.pre[2 * a + 1] == 1 || bc.vel[2 * a] == 1 || bc.vel[2 * a + 1] == 1;
     }
     if (faces && check(lbm_sp_set_boundaries(h, &bc), h, "lbm_sp_set_boundaries")) return 1;
+    // power-law fluid: collision_nnf in place of collision (SP:665-668)
+    if (in.nnf_module == 1 && check(lbm_sp_set_power_law(h, in.nnf_m, in.nnf_n), h, "lbm_sp_set_power_law")) return 1;
     // dp of Comput_Perm (SP:4600-4614): the pressure drop along PerDir when in_BC = 1
     double dp = 0.0;
     if (in.in_BC != 0) {

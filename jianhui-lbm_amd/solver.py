@@ -96,6 +96,17 @@ class SinglePhase:
         _capi.check(_capi.lib().lbm_sp_set_boundaries(self._h, ctypes.byref(b)), self._h)
         self._pre = dict(pre or {})
 
+    def set_power_law(self, m: float, n: float):
+        """Power-law fluid nu = m |S|^(n-1) from the 12th step on -- collision_nnf (SP:2017-2700);
+        m <= 0 switches it off.  Call before stepping."""
+        _capi.check(_capi.lib().lbm_sp_set_power_law(self._h, float(m), float(n)), self._h)
+
+    def viscosity(self) -> np.ndarray:
+        """Per-node viscosity niu_vn[count]."""
+        nu = np.empty(self.count)
+        _capi.check(_capi.lib().lbm_sp_get_viscosity(self._h, _dp(nu)), self._h)
+        return nu
+
     def pressure_gradient(self, PerDir: int = 1) -> float:
         """dp of Comput_Perm (SP:4600-4614): |p_+ - p_-| c_s^2 / L along PerDir (the INPUT's
         in_BC = 1 case), 0 without pressure faces."""

@@ -172,8 +172,29 @@ def test_lbm_sp_faces_match_reference(pkg, tmp_path, name):
     assert rel_err(f, d[f"f_{last}"]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["sp_nnf_16x12x8", "sp_nnf_bc_16x12x10"])
+def test_lbm_sp_power_law_matches_reference(pkg, tmp_path, name):
+    """nnf_module = 1 in the INPUT file: the reference's permeability output and checkpoint."""
+    _log(f"[gpu] lbm_sp power law on {name}")
+    d0, p0 = load_fixture(name)
+    over = {"nnf_module": 1, "nnf_mn": tuple(p0["power_law"])}
+    if p0.get("bc"):
+        over.update(_bc_over(p0["bc"]))
+    d, p = _sp_case(pkg, tmp_path, name, n_max=p0["steps"][-1] - 1, **over)
+    _run("lbm_sp", str(tmp_path))
+    last = p["steps"][-1]
+    got = _rows(open(tmp_path / "Permeability_error_local_Perm.txt").read())
+    ref = _rows(str(d[f"perm_txt_{last}"]))
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        if np.isfinite(r[0]):
+            assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0])
+    f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
+    assert rel_err(f, d[f"f_{last}"]) < 1e-13
+
+
 def test_driver_refuses_out_of_path_features(pkg, tmp_path):
-    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, nnf_module=1)
+    _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, lattice=(1, 1.0, 2.0))
     proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
                           text=True, timeout=60)
     assert proc.returncode != 0 and "not part of this build" in proc.stderr
