@@ -61,12 +61,14 @@ def read_traffic(tag):
     return None
 
 
-def cpu_baseline(solid, nu, g, budget_s, faces=None):
+def cpu_baseline(solid, nu, g, budget_s, faces=None, power_law=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
     bc = oracle.make_bc(faces["sub_bc"], faces.get("pre"), faces.get("vel")) if faces else None
-    o = oracle.SPOracle(solid, nu, g, nthreads=threads, bc=bc)
+    o = oracle.SPOracle(solid, nu, g, nthreads=threads, bc=bc, power_law=power_law)
+    if power_law:
+        o.run(11)    # the power law acts from the 12th step on
     t0 = time.perf_counter()
     o.run(1)
     t1 = time.perf_counter()
@@ -366,6 +368,8 @@ def main():
     ap.add_argument("--workload", default="sp", choices=["sp", "cg", "solute"])
     ap.add_argument("--faces", action="store_true",
                     help="single-phase workload driven by pressure faces on x (Sub_BC 3) instead of a body force")
+    ap.add_argument("--power-law", action="store_true",
+                    help="single-phase workload as a shear-thinning power-law fluid (nu = m |S|^(n-1), n = 0.8)")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
     args = ap.parse_args()
@@ -400,6 +404,12 @@ def main():
     if faces:
         s.set_boundaries(faces["sub_bc"], faces["pre"])
         s.init(1.0, (0.0, 0.0, 0.0))
+    power_law = None
+    if args.power_law:
+        # m chosen so that nu = 1/6 at |S| = 1e-7, the strain rate scale of this flow
+        power_law = (1.0 / 6.0 * 1e-7 ** 0.2, 0.8)
+        s.set_power_law(*power_law)
+        s.init(1.0, (0.0, 0.0, 0.0))
     N = s.count
     s.step(args.warmup)
     s.diagnostics()                      # synchronises the handle stream
@@ -412,8 +422,10 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = N * args.steps / wall / 1e6
     avg_launch_s = kernel_ms / 1e3 / args.steps
-    achieved = B_ALG[args.precision] * N / avg_launch_s / 1e9
-    tag = f"sp_step_{args.precision}_256cube" + ("_xfaces" if faces else "")
+    # the power-law step also reads and writes the node's viscosity
+    b_node = B_ALG[args.precision] + (2 * (8 if args.precision == "fp64" else 4) if power_law else 0)
+    achieved = b_node * N / avg_launch_s / 1e9
+    tag = f"sp_step_{args.precision}_256cube" + ("_xfaces" if faces else "") + ("_powerlaw" if power_law else "")
     traffic = read_traffic(tag) if edge == 256 else None
     out = {
         "metric": "fluid MLUPS",
@@ -428,18 +440,19 @@ def main():
         "vs_baseline": None,
         "dtype": DTYPE[args.precision],
         "data": "synthetic",
-        "config": {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35" + ("_xpressure_faces_subbc3" if faces else ""),
+        "config": {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35" + ("_xpressure_faces_subbc3" if faces else "")
+                   + ("_power_law_n0.8" if power_law else ""), "power_law": power_law,
                    "fluid_nodes": N, "box": [edge, edge, edge], "precision": args.precision, "tau": 1.0, "g": list(g),
                    "faces": faces, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "sp_step_kernel", "bytes_per_node": B_ALG[args.precision],
+                     "kernel": "sp_step_kernel", "bytes_per_node": b_node,
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
         "cpu_baseline": None,
     }
     s.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds, faces)
+        out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
     if not args.no_serve:
         out["serving"] = serving_bench(pkg, args.serve_requests, args.serve_rate)
     print(json.dumps(out))
