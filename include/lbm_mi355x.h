@@ -105,6 +105,11 @@ int lbm_sp_set_boundaries(lbm_handle *h, const lbm_sp_bc *bc);
 int lbm_sp_set_power_law(lbm_handle *h, double m, double n);
 /* per-node viscosity niu_vn[count] (the handle's nu everywhere without the power law) */
 int lbm_sp_get_viscosity(lbm_handle *h, double *nu);
+/* Post-streaming u and rho on the dense box, [i][j][k] (u: [i][j][k][3]), solid voxels u = 0,
+ * rho = 1: the arrays the reference gathers on rank 0 for its VTK and profile writers
+ * (output_velocity_b SP:4265-4390, output_density_b SP:4405-4505), scattered on the device and
+ * copied back in one transfer each.  Either pointer may be NULL. */
+int lbm_sp_get_dense(lbm_handle *h, double *u_ijk, double *rho_ijk);
 /* f = feq(rho, u0), u = u0, rho everywhere -- init (SP:1282-1383) */
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3]);
 /* load a post-streaming state, e.g. a reference checkpoint -- Backup_init (SP:4715-4828).

@@ -86,6 +86,12 @@ int lbm_sp_get_viscosity(lbm_handle *h, double *nu)
     return s ? s->get_viscosity(nu) : LBM_ERR_ARG;
 }
 
+int lbm_sp_get_dense(lbm_handle *h, double *u_ijk, double *rho_ijk)
+{
+    auto *s = as_sp(h);
+    return s ? s->get_dense(u_ijk, rho_ijk) : LBM_ERR_ARG;
+}
+
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3])
 {
     auto *s = as_sp(h);

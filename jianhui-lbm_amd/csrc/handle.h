@@ -137,6 +137,7 @@ public:
     int set_boundaries(const lbm_sp_bc *bc);
     int set_power_law(double m, double n);
     int get_viscosity(double *nu);
+    int get_dense(double *u_ijk, double *rho_ijk);
 
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
@@ -159,6 +160,7 @@ private:
     void release_boundaries();
     BcDev bc_view(bool for_step) const;
     int reset_viscosity();
+    hipError_t state_to_device(double *df, double *du, double *dr);
 
     lbm_sp_params prm_{};
     SparseGeometry geo_;

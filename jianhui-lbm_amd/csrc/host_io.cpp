@@ -263,3 +263,112 @@ double box_porosity(const std::vector<int32_t> &solid, int nx, int ny, int nz, c
 }
 
 }  // namespace lbmhost
+
+namespace lbmhost {
+
+namespace {
+
+const char *VTK_HEAD = "# vtk DataFile Version 2.0\n";
+
+std::string vtk_header(const char *title, int nx, int ny, int nz, const char *data_line, bool lookup)
+{
+    std::ostringstream h;
+    h << VTK_HEAD << title << "\nASCII\nDATASET STRUCTURED_POINTS\n";
+    h << "DIMENSIONS         " << nx << "         " << ny << "         " << nz << "\n";
+    h << "ORIGIN 0 0 0\nSPACING 1 1 1\n";
+    h << "POINT_DATA     " << (long long)nx * ny * nz << "\n" << data_line << "\n";
+    h << (lookup ? "LOOKUP_TABLE default\n" : "\n");
+    return h.str();
+}
+
+// format the points of planes k in [k0, k1) (x fastest within a plane) with emit(i, j, k, buf)
+template <typename Emit>
+bool write_planes(const std::string &path, const std::string &head, int nx, int ny, int nz, Emit emit, std::string &err)
+{
+    std::FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) {
+        err = "cannot write " + path;
+        return false;
+    }
+    std::fwrite(head.data(), 1, head.size(), fp);
+    const int block = 8;   // planes per task
+    std::vector<std::string> out((size_t)block);
+    bool ok = true;
+    for (int k0 = 0; k0 < nz && ok; k0 += block * 16) {
+        const int k1 = std::min(nz, k0 + block * 16);
+        const int ntask = (k1 - k0 + block - 1) / block;
+        std::vector<std::string> parts((size_t)ntask);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int t = 0; t < ntask; ++t) {
+            std::string &b = parts[(size_t)t];
+            b.reserve((size_t)block * nx * ny * 16);
+            char tmp[128];
+            for (int k = k0 + t * block; k < std::min(k1, k0 + (t + 1) * block); ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        const int len = emit(i, j, k, tmp);
+                        b.append(tmp, (size_t)len);
+                    }
+        }
+        for (const auto &b : parts) ok = ok && std::fwrite(b.data(), 1, b.size(), fp) == b.size();
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) err = "write failed: " + path;
+    return ok;
+}
+
+}  // namespace
+
+bool write_vtk_geometry(const std::string &path, int nx, int ny, int nz, const int32_t *solid, std::string &err)
+{
+    const std::string head =
+        vtk_header("J.Yang Lattice Boltzmann Simulation 3D Single Phase-Solid-Geometry", nx, ny, nz,
+                   "SCALARS sample_scalars float", true);
+    return write_planes(path, head, nx, ny, nz, [&](int i, int j, int k, char *buf) {
+        const int64_t v = ((int64_t)i * ny + j) * nz + k;
+        return std::snprintf(buf, 128, "%d\n", solid[v] != 0 ? 1 : 0);   // 1 = solid (SP:4189-4192)
+    }, err);
+}
+
+bool write_vtk_velocity(const std::string &path, int nx, int ny, int nz, const double *u, std::string &err)
+{
+    const std::string head = vtk_header("J.Yang Lattice Boltzmann Simulation 3D Single Phase-Velocity", nx, ny, nz,
+                                        "VECTORS sample_vectors double", false);
+    return write_planes(path, head, nx, ny, nz, [&](int i, int j, int k, char *buf) {
+        const double *p = u + (((int64_t)i * ny + j) * nz + k) * 3;
+        return std::snprintf(buf, 128, "%g %g %g \n", p[0], p[1], p[2]);
+    }, err);
+}
+
+bool write_vtk_density(const std::string &path, int nx, int ny, int nz, const double *rho, std::string &err)
+{
+    const std::string head =
+        vtk_header("J.Yang Lattice Boltzmann Simulation 3D Single Phase-Solid-Density", nx, ny, nz,
+                   "SCALARS sample_scalars float", true);
+    return write_planes(path, head, nx, ny, nz, [&](int i, int j, int k, char *buf) {
+        return std::snprintf(buf, 128, "%g\n", rho[((int64_t)i * ny + j) * nz + k]);
+    }, err);
+}
+
+bool write_velocity_profile(const std::string &path, int nx, int ny, int nz, const int32_t *solid, const double *u,
+                            std::string &err)
+{
+    std::FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) {
+        err = "cannot write " + path;
+        return false;
+    }
+    for (int j = 0; j < ny; ++j) {
+        double v = 0.0;
+        if (nx > 2 && nz > 1 && solid[((int64_t)1 * ny + j) * nz + 1] == 0)
+            v = u[(((int64_t)2 * ny + j) * nz + 1) * 3];
+        std::fprintf(fp, "%g\n", v);
+    }
+    if (std::fclose(fp) != 0) {
+        err = "write failed: " + path;
+        return false;
+    }
+    return true;
+}
+
+}  // namespace lbmhost

@@ -75,6 +75,18 @@ std::string checkpoint_name(const std::string &prefix, const char *what, long la
 bool write_checkpoint(const std::string &path, const double *data, int64_t count, int width, std::string &err);
 bool read_checkpoint(const std::string &path, double *data, int64_t count, int width, std::string &err);
 
+// The reference's legacy-VTK text files (Geometry_b SP:4168-4263, output_velocity_b SP:4265-4390,
+// output_density_b SP:4405-4505): x fastest, values as its ostreams print them (six
+// significant digits, "%g").  The text is formatted in parallel, one block of planes per
+// thread, and written in order.  Dense arrays are [i][j][k] (u: [i][j][k][3]).
+bool write_vtk_geometry(const std::string &path, int nx, int ny, int nz, const int32_t *solid_ijk, std::string &err);
+bool write_vtk_velocity(const std::string &path, int nx, int ny, int nz, const double *u_ijk, std::string &err);
+bool write_vtk_density(const std::string &path, int nx, int ny, int nz, const double *rho_ijk, std::string &err);
+// LBM_velocity_<m>.out: u_x at (2, j, 1) for each j where (1, j, 1) is fluid, else 0
+// (SP:4366-4375, solid (2, j, 1) reads the reference's padding record, written here as 0)
+bool write_velocity_profile(const std::string &path, int nx, int ny, int nz, const int32_t *solid_ijk,
+                            const double *u_ijk, std::string &err);
+
 // porosity over the partial-permeability box (SP:1088) and that box as {xn,xp,yn,yp,zn,zp}
 void perm_box(const int par_per[3], const int per_x[2], const int per_y[2], const int per_z[2], int nx, int ny, int nz,
               int box[6], bool &partial);

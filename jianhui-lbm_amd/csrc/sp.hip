@@ -153,6 +153,22 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(C *__restrict__ p, C v, int
     if (i < n) p[i] = v;
 }
 
+// node-ordered u [N][3] and rho [N] into the dense [i][j][k] grid (voxel supinv[n])
+__global__ void __launch_bounds__(BLOCK) scatter_dense_kernel(const int32_t *__restrict__ supinv, int64_t N,
+                                                              const double *__restrict__ u, const double *__restrict__ rho,
+                                                              double *__restrict__ gu, double *__restrict__ gr)
+{
+    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    const int64_t s = supinv[n];
+    if (gu) {
+        gu[s * 3 + 0] = u[n * 3 + 0];
+        gu[s * 3 + 1] = u[n * 3 + 1];
+        gu[s * 3 + 2] = u[n * 3 + 2];
+    }
+    if (gr) gr[s] = rho[n];
+}
+
 // ---------------------------------------------------------------- diagnostics
 constexpr int NDIAG = 8;   // Q0 Q1 Q2 err_num err_den speed_sum umax2 nan  (+ box count below)
 
@@ -648,6 +664,24 @@ int SpSolver::step(int nsteps, double *elapsed_ms)
     return LBM_OK;
 }
 
+// post-streaming f (AoS [N][19]), u ([N][3]) and rho into device buffers (any may be null)
+hipError_t SpSolver::state_to_device(double *df, double *du, double *dr)
+{
+    const int64_t N = geo_.N;
+    const int mode = kmode_of(mode_);
+    if (prm_.precision == LBM_FP64)
+        return launch_get_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
+                                              rho_i_, u_i_, scale_, inv_scale_),
+                            bc_view(false), mode, df, du, dr, grid_, stream_);
+    if (prm_.precision == LBM_FP32)
+        return launch_get_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
+                                             rho_i_, u_i_, scale_, inv_scale_),
+                            bc_view(false), mode, df, du, dr, grid_, stream_);
+    return launch_get_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
+                                          rho_i_, u_i_, scale_, inv_scale_),
+                        bc_view(false), mode, df, du, dr, grid_, stream_);
+}
+
 int SpSolver::get_state(double *f, double *u, double *rho)
 {
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
@@ -657,20 +691,7 @@ int SpSolver::get_state(double *f, double *u, double *rho)
     if (f) LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
     if (u) LBM_HIP_TRY(hipMalloc(&du, (size_t)N * 3 * sizeof(double)));
     if (rho) LBM_HIP_TRY(hipMalloc(&dr, (size_t)N * sizeof(double)));
-    const int mode = kmode_of(mode_);
-    hipError_t e;
-    if (prm_.precision == LBM_FP64)
-        e = launch_get_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
-                                           u_i_, scale_, inv_scale_),
-                         bc_view(false), mode, df, du, dr, grid_, stream_);
-    else if (prm_.precision == LBM_FP32)
-        e = launch_get_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
-                                          u_i_, scale_, inv_scale_),
-                         bc_view(false), mode, df, du, dr, grid_, stream_);
-    else
-        e = launch_get_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, rho_i_,
-                                           u_i_, scale_, inv_scale_),
-                         bc_view(false), mode, df, du, dr, grid_, stream_);
+    hipError_t e = state_to_device(df, du, dr);
     if (e == hipSuccess && f) e = hipMemcpyAsync(f, df, (size_t)N * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, stream_);
@@ -678,6 +699,39 @@ int SpSolver::get_state(double *f, double *u, double *rho)
     if (df) (void)hipFree(df);
     if (du) (void)hipFree(du);
     if (dr) (void)hipFree(dr);
+    LBM_HIP_TRY(e);
+    return LBM_OK;
+}
+
+int SpSolver::get_dense(double *u_ijk, double *rho_ijk)
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    if (!u_ijk && !rho_ijk) return LBM_OK;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int64_t N = geo_.N, V = (int64_t)geo_.nx * geo_.ny * geo_.nz;
+    double *du = nullptr, *dr = nullptr, *gu = nullptr, *gr = nullptr;
+    hipError_t e = hipSuccess;
+    auto alloc = [&](double **p, size_t n) {
+        if (e == hipSuccess) e = hipMalloc((void **)p, n * sizeof(double));
+    };
+    if (u_ijk) alloc(&du, (size_t)N * 3), alloc(&gu, (size_t)V * 3);
+    if (rho_ijk) alloc(&dr, (size_t)N), alloc(&gr, (size_t)V);
+    // solid voxels keep the values the reference's writers put there: u = 0, rho = 1
+    if (e == hipSuccess && gu) e = hipMemsetAsync(gu, 0, (size_t)V * 3 * sizeof(double), stream_);
+    if (e == hipSuccess && gr) {
+        fill_kernel<double><<<dim3((unsigned)((V + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream_>>>(gr, 1.0, V);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = state_to_device(nullptr, du, dr);
+    if (e == hipSuccess) {
+        scatter_dense_kernel<<<dim3((unsigned)grid_), dim3(BLOCK), 0, stream_>>>(d_supinv_, N, du, dr, gu, gr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && gu) e = hipMemcpyAsync(u_ijk, gu, (size_t)V * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && gr) e = hipMemcpyAsync(rho_ijk, gr, (size_t)V * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    for (double *p : {du, dr, gu, gr})
+        if (p) (void)hipFree(p);
     LBM_HIP_TRY(e);
     return LBM_OK;
 }

@@ -76,8 +76,8 @@ int main(int argc, char **argv)
     if (in.nnf_module != 0 && in.nnf_module != 1) return die("nnf_module must be 0 or 1");
     if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
         return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
-    if (in.freVe > 0 || in.freDe > 0 || in.vel_sol > 0 || in.gperm > 0 || in.loc_perm == 1)
-        std::cerr << "lbm_sp: note: VTK, solute-velocity, group and local permeability outputs are skipped\n";
+    if (in.gperm > 0 || in.loc_perm == 1)
+        std::cerr << "lbm_sp: note: group and local permeability outputs are skipped\n";
     const int nx = in.nx, ny = in.ny, nz = in.nz;
     std::vector<int32_t> solid;
     if (!lbmhost::read_geometry(in.filename, nx, ny, nz, in.decbin, solid, err)) return die(err);
@@ -140,6 +140,34 @@ int main(int argc, char **argv)
         if (check(lbm_sp_load_state(h, f.data(), u.data(), rho.data()), h, "lbm_sp_load_state")) return 1;
     }
     std::cout << "\nINITIALIZATION COMPLETED  (" << count << " fluid nodes, porosity " << porosity << ")\n\n";
+    if ((in.freVe >= 0 || in.freDe >= 0) &&
+        !lbmhost::write_vtk_geometry(in.outputfile + "LBM_Geometry.vtk", nx, ny, nz, solid.data(), err))   // SP:538-543
+        return die(err);
+    // field writers: the device scatters u / rho onto the box (lbm_sp_get_dense), the host
+    // formats the reference's text in parallel
+    auto write_velocity = [&](long label) -> bool {
+        std::vector<double> ud((size_t)nx * ny * nz * 3);
+        if (check(lbm_sp_get_dense(h, ud.data(), nullptr), h, "lbm_sp_get_dense")) return false;
+        return lbmhost::write_vtk_velocity(in.outputfile + "LBM_velocity_Vector_" + std::to_string(label) + ".vtk", nx,
+                                           ny, nz, ud.data(), err) &&
+               lbmhost::write_velocity_profile("LBM_velocity_" + std::to_string(label) + ".out", nx, ny, nz,
+                                               solid.data(), ud.data(), err);
+    };
+    auto write_density = [&](long label) -> bool {
+        std::vector<double> rd((size_t)nx * ny * nz);
+        if (check(lbm_sp_get_dense(h, nullptr, rd.data()), h, "lbm_sp_get_dense")) return false;
+        return lbmhost::write_vtk_density(in.outputfile + "LBM_Density_" + std::to_string(label) + ".vtk", nx, ny, nz,
+                                          rd.data(), err);
+    };
+    auto write_solute_velocity = [&](long label) -> bool {   // output_velocity_for_solute (SP:5093-5118)
+        std::vector<double> u((size_t)count * 3);
+        if (check(lbm_sp_get_state(h, nullptr, u.data(), nullptr), h, "lbm_sp_get_state")) return false;
+        const std::string path = in.outputfile + "Velocity_for_solute_" + std::to_string(label) + ".0.bin";
+        std::ofstream o(path, std::ios::binary);
+        o.write((const char *)u.data(), (std::streamsize)(u.size() * sizeof(double)));
+        if (!o) err = "write failed: " + path;
+        return (bool)o;
+    };
 
     const std::string out = in.outputfile;
     const std::string fResults = out + "Results.txt", fPerm = out + "Permeability_error_local_Perm.txt",
@@ -240,8 +268,16 @@ int main(int argc, char **argv)
                   << "The Maximum velocity is: " << u_max << "   Re=" << Re << "\n"
                   << "The max relative error of uv is: " << error << "\n\n";
         bool stop = false;
-        if (in.Out_Mode == 1 && std::fabs((u_ave2 - u_ave) / u_ave2) < 1e-7) stop = true;   // SP:812-816
-        if (in.fre_backup > 0 && n % in.fre_backup == 0 && n > 0 && !backup(n)) return die(err);
+        long nl = n;   // the label the outputs below see (n_max + 1 after the converged stop)
+        if (in.Out_Mode == 1 && std::fabs((u_ave2 - u_ave) / u_ave2) < 1e-7) {   // SP:812-816
+            if (!write_velocity(n)) return die(err);
+            stop = true;
+            nl = in.n_max + 1;
+        }
+        if (in.freDe > 0 && nl % in.freDe == 0 && !write_density(nl)) return die(err);   // SP:818-822
+        if (in.freVe > 0 && nl % in.freVe == 0 && !write_velocity(nl)) return die(err);  // SP:824-828
+        if (in.fre_backup > 0 && nl % in.fre_backup == 0 && nl > 0 && !backup(nl)) return die(err);
+        if (in.vel_sol > 0 && nl % in.vel_sol == 0 && nl > 0 && !write_solute_velocity(nl)) return die(err);
         if (error != error) {
             std::cout << "PROGRAM STOP" << std::endl;
             stop = true;

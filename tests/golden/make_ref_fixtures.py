@@ -78,7 +78,21 @@ def _bc_fields(inp, bc):
                                      1 if n is not None else 0, n if n is not None else (1.0 if kind == "pre" else 0.0)))
 
 
-def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None):
+def _field_outputs(wd, freRe, n_max, out):
+    """The reference's VTK, profile and solute-velocity files of one run (SP:538-543, 818-832)."""
+    out["vtk_geometry"] = np.array(_text(wd, "LBM_Geometry.vtk"))
+    for n in range(0, n_max + 1, freRe):
+        out[f"vtk_velocity_{n}"] = np.array(_text(wd, f"LBM_velocity_Vector_{n}.vtk"))
+        out[f"vtk_density_{n}"] = np.array(_text(wd, f"LBM_Density_{n}.vtk"))
+        out[f"profile_{n}"] = np.array(_text(wd, f"LBM_velocity_{n}.out"))
+        if n > 0:
+            parts = [np.fromfile(os.path.join(wd, f"Velocity_for_solute_{n}.{r}.bin"), dtype="<f8").reshape(-1, 3)
+                     for r in range(2)]
+            out[f"solute_velocity_{n}"] = np.concatenate(parts, axis=0)
+
+
+def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None,
+            field_outputs=False):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
@@ -96,6 +110,8 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             if power_law is not None:                 # nnf_module = 1, nnf_m nnf_n (SP:307-308)
                 inp.nnf_module = 1
                 inp.nnf_mn = tuple(power_law)
+            if field_outputs:                         # freVe, freDe, vel_sol = freRe
+                inp.freVe = inp.freDe = inp.vel_sol = freRe
             if box is not None:
                 inp.par_per = (1, 1, 1)
                 inp.per_x = (box[1], box[0])
@@ -110,6 +126,8 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             out[f"perm_txt_{steps}"] = np.array(_text(wd, "Permeability_error_local_Perm.txt"))
             out[f"vel_txt_{steps}"] = np.array(_text(wd, "Velocity_ave_max.txt"))
             out[f"general_txt_{steps}"] = np.array(_text(wd, "General_Results.txt"))
+            if field_outputs:
+                _field_outputs(wd, freRe, steps - 1, out)
         finally:
             shutil.rmtree(wd, ignore_errors=True)
     out["params"] = np.array(json.dumps(params))
@@ -289,6 +307,10 @@ def main(only=()):
                                                   u0=(2e-3, 1e-3, -1e-3), steps_list=[12, 30],
                                                   bc={"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.02}},
                                                   power_law=(0.05, 1.3))),
+        # the field writers: legacy VTK (velocity, density, geometry), the velocity profile and
+        # the solute-velocity files at n = 0, 10, 20
+        ("sp_out_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
+                                              u0=(1e-3, -2e-3, 5e-4), steps_list=[21], field_outputs=True)),
     ]
     for name, make in cases:
         if not only or name in only:

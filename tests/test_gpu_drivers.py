@@ -198,3 +198,26 @@ def test_driver_refuses_out_of_path_features(pkg, tmp_path):
     proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
                           text=True, timeout=60)
     assert proc.returncode != 0 and "not part of this build" in proc.stderr
+
+
+def test_lbm_sp_field_writers_match_reference_files(pkg, tmp_path):
+    """freVe / freDe / vel_sol: the geometry, velocity and density VTK files, the velocity
+    profile and the solute-velocity files the reference wrote in the same run, text for text."""
+    _log("[gpu] lbm_sp field writers")
+    d, p = _sp_case(pkg, tmp_path, "sp_out_16x12x8", n_max=20, freVe=10, freDe=10, vel_sol=10)
+    _run("lbm_sp", str(tmp_path))
+    assert open(tmp_path / "LBM_Geometry.vtk").read() == str(d["vtk_geometry"])
+    for n in (0, 10, 20):
+        for ours, key in ((f"LBM_velocity_Vector_{n}.vtk", f"vtk_velocity_{n}"), (f"LBM_Density_{n}.vtk", f"vtk_density_{n}"),
+                          (f"LBM_velocity_{n}.out", f"profile_{n}")):
+            got, ref = open(tmp_path / ours).read(), str(d[key])
+            if got != ref:   # six printed digits: only a value within round-off of a rounding boundary may differ
+                gl, rl = got.splitlines(), ref.splitlines()
+                assert len(gl) == len(rl)
+                diff = [(a, b) for a, b in zip(gl, rl) if a != b]
+                assert len(diff) <= 2, diff[:4]
+                for a, b in diff:
+                    np.testing.assert_allclose(np.array(a.split(), float), np.array(b.split(), float), rtol=2e-6)
+        if n > 0:
+            u = np.fromfile(tmp_path / f"Velocity_for_solute_{n}.0.bin", dtype="<f8").reshape(-1, 3)
+            assert np.abs(u - d[f"solute_velocity_{n}"]).max() < 1e-14
