@@ -317,6 +317,39 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
     return out
 
 
+def vtk_bench(pkg, edge=256):
+    """The field writers on the single-phase workload: lbm_sp writes the velocity and density
+    VTK files of its first step (the reference's legacy ASCII format), once with the host's
+    threads and once with one thread (the reference formats on one rank)."""
+    import subprocess
+    import tempfile
+    solid = pkg.geometry.sphere_pack((edge, edge, edge), 8.0, 0.35, seed=2024)
+    out = {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35 velocity + density VTK", "points": edge ** 3}
+    with tempfile.TemporaryDirectory() as wd:
+        pkg.geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+        inp = pkg.inputs.SPInput(filename="geometry.bin", n_xyz=(edge, edge, edge), n_max=0, reso=1.0, g=(1e-6, 0, 0),
+                                 in_vis=1 / 6, freRe=1, freVe=1, freDe=1, outputfile="./", fre_backup=-1, decbin=1)
+        pkg.inputs.write_sp_input(inp, os.path.join(wd, "INPUT.dat"))
+        for label, threads in (("threads", max(1, min(16, os.cpu_count() or 1))), ("one_thread", 1)):
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat", "--precision", "fp32"], cwd=wd, env=env,
+                                  capture_output=True, text=True, timeout=900)
+            if proc.returncode != 0:
+                raise RuntimeError("lbm_sp failed: " + proc.stderr[-2000:])
+            rec = {"threads": threads}
+            for line in proc.stdout.splitlines():
+                if line.startswith("[lbm_sp] velocity VTK") or line.startswith("[lbm_sp] density VTK"):
+                    kind = line.split()[1]
+                    parts = line.replace(",", "").split()
+                    rec[f"{kind}_gather_s"] = float(parts[parts.index("gather") + 1])
+                    rec[f"{kind}_text_s"] = float(parts[parts.index("text") + 1])
+            rec["velocity_MB"] = round(os.path.getsize(os.path.join(wd, "LBM_velocity_Vector_0.vtk")) / 1e6, 1)
+            rec["density_MB"] = round(os.path.getsize(os.path.join(wd, "LBM_Density_0.vtk")) / 1e6, 1)
+            out[label] = rec
+            print(f"[vtk] {label}: {rec}", file=sys.stderr, flush=True)
+    return out
+
+
 def accuracy_study(pkg, edge=256, max_steps=100000):
     """The single-phase workload run to steady state (|k_n - k_(n-100)| <= 1e-6 |k_n|, the
     reference's relative-change test at freRe = 100) in FP64, FP32 and FP16 storage; each
@@ -370,6 +403,7 @@ def main():
                     help="single-phase workload driven by pressure faces on x (Sub_BC 3) instead of a body force")
     ap.add_argument("--power-law", action="store_true",
                     help="single-phase workload as a shear-thinning power-law fluid (nu = m |S|^(n-1), n = 0.8)")
+    ap.add_argument("--vtk", action="store_true", help="time the VTK field writers of the lbm_sp driver")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
     args = ap.parse_args()
@@ -384,6 +418,9 @@ def main():
         return
     if args.accuracy:
         print(json.dumps({"accuracy": accuracy_study(pkg, args.edge)}))
+        return
+    if args.vtk:
+        print(json.dumps({"vtk": vtk_bench(pkg, args.edge)}))
         return
     if args.workload == "solute":
         prec = args.precision if args.precision != "fp16" else "fp32"

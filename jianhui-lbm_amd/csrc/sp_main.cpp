@@ -145,19 +145,32 @@ int main(int argc, char **argv)
         return die(err);
     // field writers: the device scatters u / rho onto the box (lbm_sp_get_dense), the host
     // formats the reference's text in parallel
+    auto secs = [](std::chrono::steady_clock::time_point t0) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
     auto write_velocity = [&](long label) -> bool {
+        const auto t0 = std::chrono::steady_clock::now();
         std::vector<double> ud((size_t)nx * ny * nz * 3);
         if (check(lbm_sp_get_dense(h, ud.data(), nullptr), h, "lbm_sp_get_dense")) return false;
-        return lbmhost::write_vtk_velocity(in.outputfile + "LBM_velocity_Vector_" + std::to_string(label) + ".vtk", nx,
-                                           ny, nz, ud.data(), err) &&
-               lbmhost::write_velocity_profile("LBM_velocity_" + std::to_string(label) + ".out", nx, ny, nz,
-                                               solid.data(), ud.data(), err);
+        const double tg = secs(t0);
+        const auto t1 = std::chrono::steady_clock::now();
+        const std::string path = in.outputfile + "LBM_velocity_Vector_" + std::to_string(label) + ".vtk";
+        const bool ok = lbmhost::write_vtk_velocity(path, nx, ny, nz, ud.data(), err) &&
+                        lbmhost::write_velocity_profile("LBM_velocity_" + std::to_string(label) + ".out", nx, ny, nz,
+                                                        solid.data(), ud.data(), err);
+        std::cout << "[lbm_sp] velocity VTK " << label << ": device gather " << tg << " s, text " << secs(t1) << " s\n";
+        return ok;
     };
     auto write_density = [&](long label) -> bool {
+        const auto t0 = std::chrono::steady_clock::now();
         std::vector<double> rd((size_t)nx * ny * nz);
         if (check(lbm_sp_get_dense(h, nullptr, rd.data()), h, "lbm_sp_get_dense")) return false;
-        return lbmhost::write_vtk_density(in.outputfile + "LBM_Density_" + std::to_string(label) + ".vtk", nx, ny, nz,
-                                          rd.data(), err);
+        const double tg = secs(t0);
+        const auto t1 = std::chrono::steady_clock::now();
+        const bool ok = lbmhost::write_vtk_density(in.outputfile + "LBM_Density_" + std::to_string(label) + ".vtk", nx,
+                                                   ny, nz, rd.data(), err);
+        std::cout << "[lbm_sp] density VTK " << label << ": device gather " << tg << " s, text " << secs(t1) << " s\n";
+        return ok;
     };
     auto write_solute_velocity = [&](long label) -> bool {   // output_velocity_for_solute (SP:5093-5118)
         std::vector<double> u((size_t)count * 3);
