@@ -13,10 +13,11 @@ The same line carries the serving figures ("serving"): a seeded Poisson stream o
 permeability requests on thresholded Gaussian-random-field subsamples (edge 48..160 lu,
 correlation length 6 lu, porosity 0.15..0.45, FP32 populations, FP64 reductions), arriving
 faster than they can be served (200/s), through the continuous-batching server with Anderson
-acceleration, and the same stream through sequential plain FP32 solves stopped by the
-reference's relative-change test (1e-6 over 100 steps, and 1e-7, which is what it takes to
-hold every k within 1e-4); permeabilities per hour, p50/p99 latency, and the largest error of
-each against FP64 reference values.
+acceleration, and the same stream through sequential plain solves, one sample at a time:
+stopped at the first check where k is within 1e-4 of the FP64 reference value (the least work
+plain stepping can do at the server's accuracy), and by the reference's relative-change test
+(1e-6 over 100 steps, and 1e-7), which miss that accuracy; permeabilities per hour, p50/p99
+latency, and the largest error of each against FP64 reference values.
 """
 import argparse
 import importlib.util
@@ -112,7 +113,7 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000):
 
     def serve(accel):
         """The stream through the continuous-batching server (FP32 arena)."""
-        kw = dict(accel_depth=8, accel_period=100, accel_warmup=300, escalate_k=1e-3) if accel else {}
+        kw = dict(accel_depth=8, accel_period=100, accel_warmup=200, escalate_k=1e-3) if accel else {}
         with pkg.PermeabilityServer(check_interval=interval, tol=1e-6, max_steps=max_steps, block=0, **kw) as srv:
             t0 = srv.clock()
             tickets = []
@@ -169,18 +170,57 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000):
                 "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
                 "max_rel_err_vs_fp64": errors(ks), "node_steps": int(steps_nodes)}, span
 
+    def sequential_oracle():
+        """FIFO plain solves on the same arrival schedule, one sample at a time, each stopped at
+        the first check (every 100 steps) at which its k is within 1e-4 of the FP64 reference
+        value: the least work any stop rule for plain time stepping could do at that accuracy
+        (plain k(t) approaches its limit monotonically on every sample of this stream, so the
+        first entry into the band is final).  Samples whose reference k is below 1e-3 are
+        solved in FP64: FP32 deviations cannot resolve their flux."""
+        lat, ks, steps_nodes = [], [], 0
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            now = time.perf_counter() - t0
+            if now < arrivals[i]:
+                time.sleep(arrivals[i] - now)
+            prec = "fp64" if abs(truth[i]) < 1e-3 else "fp32"
+            s = pkg.SinglePhase(samples[i], nu, g, precision=prec)
+            steps, kd = 0, float("nan")
+            while steps < 400000:
+                s.step(interval)
+                steps += interval
+                kd = s.Comput_Perm(1)[0][0]
+                if abs(kd - truth[i]) <= 1e-4 * abs(truth[i]):
+                    break
+            s.close()
+            ks.append(kd)
+            steps_nodes += steps * nodes[i]
+            lat.append(time.perf_counter() - t0 - arrivals[i])
+        span = time.perf_counter() - t0
+        lat = np.array(lat)
+        return {"stop": "first check with |k - k_ref| <= 1e-4 k_ref (k_ref known)", "perms_per_hour": round(nreq / span * 3600.0, 1),
+                "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                "max_rel_err_vs_fp64": errors(ks), "node_steps": int(steps_nodes),
+                "fp64_samples": int(sum(1 for t in truth if abs(t) < 1e-3))}, span
+
     served, span_srv = serve(True)
     batched, span_bat = serve(False)
     seq_ref, span_ref = sequential(1e-6)
     seq_acc, span_acc = sequential(1e-7)
+    seq_orc, span_orc = sequential_oracle() if truth is not None else (None, None)
     out = {"requests": nreq, "arrival_rate_per_s": rate, "seed": seed, "fluid_nodes_total": int(sum(nodes)),
            "accuracy_target": "k within 1e-4 of the FP64 reference (tests/golden/serving_stream_k.json)"}
     out.update(served)
-    out["server"] = ("FP32 arena, Anderson acceleration m=8 every 100 steps after 300, retire at 2 x |dk/k| <= 1e-6, "
+    out["server"] = ("FP32 arena, Anderson acceleration m=8 every 100 steps after 200, retire at 2 x |dk/k| <= 1e-6, "
                      "samples with k < 1e-3 continued in FP64")
-    out["speedup_vs_sequential_same_accuracy"] = round(span_acc / span_srv, 2)
+    # the same accuracy: every k within 1e-4 of the FP64 reference, which only the oracle-stopped
+    # sequential solves reach (the 1e-7 and 1e-6 relative-change rules miss it)
+    out["speedup_vs_sequential_same_accuracy"] = round(span_orc / span_srv, 2) if span_orc else None
+    out["speedup_vs_sequential_tol_1e-7"] = round(span_acc / span_srv, 2)
     out["speedup_vs_sequential_reference_rule"] = round(span_ref / span_srv, 2)
     out["server_plain_stepping"] = batched
+    out["sequential_plain_oracle_stop"] = seq_orc
     out["sequential_plain_tol_1e-7"] = seq_acc
     out["sequential_plain_tol_1e-6"] = seq_ref
     return out

@@ -427,6 +427,7 @@ private:
     void shutdown();
     void loop();
     bool escalate_locked(Active &a);
+    bool make_fallback(int64_t min_nodes);
     int run_chunk(std::unique_lock<std::mutex> &lk);
     bool try_admit_locked();
     int rebuild_tables();
@@ -499,21 +500,27 @@ void Server::enqueue(Prepared &&pr)
 
 // Read the sample's neighbour deltas back from the arena and hand the sample to the FP64
 // server; false (the FP32 result stands) if that server cannot be made or cannot hold it.
+// The internal FP64 server that escalated samples continue on.  It is made when the server is
+// created, so that an escalation costs a state copy and not an arena allocation on the loop.
+bool Server::make_fallback(int64_t min_nodes)
+{
+    lbm_srv_params fp = prm_;
+    fp.precision = LBM_FP64;
+    fp.escalate_k = 0.0;
+    fp.block = blk_;
+    fp.capacity_nodes = std::max<int64_t>({int64_t(4) << 20, cap_ / 8, min_nodes});
+    fp.mem_fraction = 0.0;
+    auto f = std::unique_ptr<Server>(new (std::nothrow) Server());
+    if (!f) return false;
+    f->t0_ = t0_;   // one clock for submit, admit and done times
+    if (f->create(fp) != LBM_OK) return false;
+    fallback_ = std::move(f);
+    return true;
+}
+
 bool Server::escalate_locked(Active &a)
 {
-    if (!fallback_) {
-        lbm_srv_params fp = prm_;
-        fp.precision = LBM_FP64;
-        fp.escalate_k = 0.0;
-        fp.block = blk_;
-        fp.capacity_nodes = std::max<int64_t>(int64_t(4) << 20, a.req.npad);
-        fp.mem_fraction = 0.0;
-        auto f = std::unique_ptr<Server>(new (std::nothrow) Server());
-        if (!f) return false;
-        f->t0_ = t0_;   // one clock for submit, admit and done times
-        if (f->create(fp) != LBM_OK) return false;
-        fallback_ = std::move(f);
-    }
+    if (!fallback_ && !make_fallback(a.req.npad)) return false;
     if (!fallback_->fits(a.req.npad)) return false;
     a.req.nbr.resize((size_t)18 * a.req.npad);
     hipError_t e = hipMemcpy2DAsync(a.req.nbr.data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
@@ -624,6 +631,8 @@ int Server::create(const lbm_srv_params &p)
         int rc = autotune();
         if (rc != LBM_OK) return rc;
     }
+    if (prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && !make_fallback(0))
+        return fail(LBM_ERR_NOMEM, "could not create the FP64 server for escalated samples");
     thr_ = std::thread([this] { loop(); });
     // geometry preparation off the caller's thread: each worker numbers a sample and builds
     // its neighbour deltas while the device steps the admitted ones
