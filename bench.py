@@ -294,8 +294,8 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
     with rest weight 1/4 and D = 0.008, started from a C = 1 slab over the first eighth of the
     box.  The pack's steady permeability is 11.44 lu^2 (single-phase run to a relative change
     below 1e-4 per 1000 steps), so the steady pore velocity is 0.0100 lu/step and
-    Pe = u d / D = 10 with the fibre diameter d = 8.  A step is one single-phase launch (which
-    also stores the velocity) and one solute launch with that velocity, on one stream."""
+    Pe = u d / D = 10 with the fibre diameter d = 8.  A step is one launch that advances the
+    flow and, with the velocity it has just computed, the solute."""
     geo = pkg.geometry
     solid = geo.fibre_pack((edge, edge, edge), 4.0, 0.8, seed=2027)
     nu, g, D, w0 = 1.0 / 6.0, (1.165e-4, 0.0, 0.0), 0.008, 0.25
@@ -327,7 +327,7 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": read_traffic(f"sp_st_step_{precision}_{edge}cube"), "bytes_per_node": b_step,
-                        "kernels": "sp_step_kernel + st_step_kernel per step",
+                        "kernels": "one sp_step_kernel launch per step with the solute step fused in",
                         "avg_step_us": round(ms * 1e3 / steps, 2)},
            "mass_rel_drift": abs(d["mass"] - m0) / m0, "pore_velocity_at_end": u_pore,
            "pe_at_end": u_pore * 8.0 / D, "cpu_baseline": None}

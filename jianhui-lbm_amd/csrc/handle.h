@@ -123,6 +123,14 @@ struct BcDev {
 
 enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
 
+// a coupled solute's buffers and rates for the fused flow + solute launch (compute type of the
+// flow handle)
+struct FusedSolute {
+    const void *src;
+    void *dst;
+    double omega, w0, wq, inv_cs2;
+};
+
 class SpSolver : public lbm_handle {
 public:
     SpSolver() : lbm_handle(SP) {}
@@ -145,6 +153,9 @@ public:
     // for a coupled passive solute: one step that also stores its velocity, and the shared
     // device geometry
     int step_storing_velocity();
+    // one flow step that also advances a coupled solute in the same launch (pull state only)
+    int step_fused(const FusedSolute &fs, bool store_u);
+    bool pull_state() const { return mode_ == STATE_PULL; }
     const void *device_velocity() const { return d_u_; }
     NbrDev nbr_view() const { return nbr_.view(); }
     const SparseGeometry &geometry() const { return geo_; }
@@ -155,7 +166,7 @@ public:
     bool initialised() const { return mode_ != STATE_EMPTY; }
 
 private:
-    int launch_step(bool store_u);
+    int launch_step(bool store_u, const FusedSolute *fs = nullptr);
     int release();
     void release_boundaries();
     BcDev bc_view(bool for_step) const;
@@ -258,7 +269,7 @@ public:
 private:
     int setup(const lbm_st_params &p);
     int upload_velocity(const double *u);
-    int launch_step();
+    int launch_step(bool last);
     // the velocity a launch reads: the frozen (or, coupled, initial) field until the first
     // coupled step, then the one the flow step just stored
     const void *velocity() const { return (flow_ && mode_ != STATE_INIT) ? flow_->device_velocity() : d_u_own_; }

@@ -26,6 +26,16 @@
 
 namespace lbm {
 
+// A coupled passive solute advanced in the same launch (D3Q7 BGK, the solute step of st.hip):
+// its six links are the first six rows of the pull table the flow step has just decoded, and its
+// velocity is the one the flow step has just computed, so neither is read from memory again.
+template <typename C>
+struct StFused {
+    const C *__restrict__ src;   // 7 x P post-collision populations of the previous step
+    C *__restrict__ dst;
+    C omega, w0, wq, inv_cs2;    // 1/tau_s, rest and axis weights, 1/c_s^2
+};
+
 template <typename ST>
 struct SpArgs {
     using C = typename Store<ST>::C;
@@ -41,6 +51,7 @@ struct SpArgs {
     BcDev bc;   // face boundaries (bc.wave == nullptr: none)
     C *__restrict__ nu_vn;   // power-law collision: per-node viscosity (nullptr: Newtonian)
     C pl_m, pl_n;            // nu = m |S|^(n-1)
+    StFused<C> st;           // coupled solute (pull launches of the fused variant only)
 };
 
 // Power-law collision (collision_nnf, SP:2399-2491): update the node's viscosity from its
@@ -111,7 +122,8 @@ __device__ __forceinline__ Macro<typename Store<ST>::C> node_macro(const SpArgs<
 // BC: face boundaries on.  A pull launch replaces the pulled populations of face nodes by
 // their rule, and every launch saves the pre-collision state of the nodes the rules read.
 // PL: power-law collision (pull launches of reference steps n > 10, SP:2397).
-template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false>
+// FUSED: also the step of a coupled solute (pull launches).
+template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false, bool FUSED = false>
 __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
 {
     using C = typename Store<ST>::C;
@@ -128,11 +140,33 @@ __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
             bc_save(a.bc, n, h, mc.drho, mc.ux, mc.uy, mc.uz);
         }
     } else {
-        gather<ST, MODE>(a, n, h);
+        uint32_t nb[Q - 1];
+        gather_pops<ST, MODE>(a.src, a.nbr, a.P, n, a.inv_scale, h, nb);
         if (BC && MODE == K_PULL) bc_patch(a.bc, n, a.rho0, h);
         moments_fwd(h, m);
         mc = node_macro<ST, MODE>(a, n, m);
         if (BC) bc_save(a.bc, n, h, mc.drho, mc.ux, mc.uy, mc.uz);
+        if (FUSED && MODE == K_PULL) {
+            C g[7];
+            g[0] = a.st.src[n];
+#pragma unroll
+            for (int q = 1; q < 7; ++q) {
+                const uint32_t v = nb[q - 1];
+                const int64_t row = (v & BB_FLAG) ? (int64_t)opp(q) : (int64_t)q;
+                g[q] = a.st.src[row * a.P + (int64_t)(v & ~BB_FLAG)];
+            }
+            C c = g[0];
+#pragma unroll
+            for (int q = 1; q < 7; ++q) c += g[q];
+            const C wc = a.st.wq * c;
+            __builtin_nontemporal_store(g[0] - a.st.omega * (g[0] - a.st.w0 * c), &a.st.dst[n]);
+#pragma unroll
+            for (int q = 1; q < 7; ++q) {
+                const C eu = q == 1 ? mc.ux : q == 2 ? -mc.ux : q == 3 ? mc.uy : q == 4 ? -mc.uy : q == 5 ? mc.uz : -mc.uz;
+                const C eq = wc * (C(1) + eu * a.st.inv_cs2);
+                __builtin_nontemporal_store(g[q] - a.st.omega * (g[q] - eq), &a.st.dst[(int64_t)q * a.P + n]);
+            }
+        }
     }
     if (STORE_U && MODE != K_DIRECT) {
         a.u_aux[n] = mc.ux;
@@ -310,13 +344,21 @@ SpArgs<ST> make_args(const void *src, void *dst, const NbrDev &nbr, void *u_aux,
     a.bc = BcDev{nullptr, nullptr, nullptr, nullptr, 0};
     a.nu_vn = nullptr;
     a.pl_m = a.pl_n = C(0);
+    a.st = StFused<C>{nullptr, nullptr, C(0), C(0), C(0), C(0)};
     return a;
 }
 
 template <typename ST, bool BC>
 void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b, hipStream_t s)
 {
-    if (mode == K_PULL && a.nu_vn)
+    if (mode == K_PULL && a.st.src) {
+        if (a.nu_vn)
+            store_u ? sp_step_kernel<ST, K_PULL, true, BC, true, true><<<g, b, 0, s>>>(a)
+                    : sp_step_kernel<ST, K_PULL, false, BC, true, true><<<g, b, 0, s>>>(a);
+        else
+            store_u ? sp_step_kernel<ST, K_PULL, true, BC, false, true><<<g, b, 0, s>>>(a)
+                    : sp_step_kernel<ST, K_PULL, false, BC, false, true><<<g, b, 0, s>>>(a);
+    } else if (mode == K_PULL && a.nu_vn)
         store_u ? sp_step_kernel<ST, K_PULL, true, BC, true><<<g, b, 0, s>>>(a)
                 : sp_step_kernel<ST, K_PULL, false, BC, true><<<g, b, 0, s>>>(a);
     else if (mode == K_INIT)
@@ -331,7 +373,7 @@ void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b,
 
 template <typename ST>
 hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m, double pl_n, int mode, bool store_u,
-                         int64_t grid, hipStream_t s)
+                         int64_t grid, hipStream_t s, const FusedSolute *fs = nullptr)
 {
     using C = typename Store<ST>::C;
     dim3 g((unsigned)grid), b(BLOCK);
@@ -339,6 +381,14 @@ hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m
     a.nu_vn = (C *)nu_vn;
     a.pl_m = (C)pl_m;
     a.pl_n = (C)pl_n;
+    if (fs) {
+        a.st.src = (const C *)fs->src;
+        a.st.dst = (C *)fs->dst;
+        a.st.omega = (C)fs->omega;
+        a.st.w0 = (C)fs->w0;
+        a.st.wq = (C)fs->wq;
+        a.st.inv_cs2 = (C)fs->inv_cs2;
+    }
     if (bc.wave)
         launch_step_bc<ST, true>(a, mode, store_u, g, b, s);
     else
@@ -509,7 +559,7 @@ int SpSolver::load_state(const double *f, const double *u, const double *rho)
     return reset_viscosity();
 }
 
-int SpSolver::launch_step(bool store_u)
+int SpSolver::launch_step(bool store_u, const FusedSolute *fs)
 {
     const int mode = kmode_of(mode_);
     // the power-law collision acts from the reference's loop step n = 11 on (SP:2397); the
@@ -520,15 +570,15 @@ int SpSolver::launch_step(bool store_u)
     if (prm_.precision == LBM_FP64)
         e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
     else if (prm_.precision == LBM_FP32)
         e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
     else
         e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
                                             sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_);
+                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
     LBM_HIP_TRY(e);
     cur_ = nxt;
     if (d_bc_wave_) bc_cur_ ^= 1;
@@ -640,6 +690,13 @@ int SpSolver::step_storing_velocity()
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
     LBM_HIP_TRY(hipSetDevice(device_));
     return launch_step(true);
+}
+
+int SpSolver::step_fused(const FusedSolute &fs, bool store_u)
+{
+    if (mode_ != STATE_PULL) return fail(LBM_ERR_STATE, "the fused step needs a flow state in pull form");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    return launch_step(store_u, &fs);
 }
 
 int SpSolver::step(int nsteps, double *elapsed_ms)

@@ -393,14 +393,25 @@ int StSolver::load_state(const double *g)
     return LBM_OK;
 }
 
-int StSolver::launch_step()
+int StSolver::launch_step(bool last)
 {
+    const int nxt = cur_ ^ 1;
+    if (flow_ && mode_ == STATE_PULL && flow_->pull_state()) {
+        // both states in pull form: one launch advances the flow and the solute; the flow keeps
+        // its velocity for its Error diagnostic on the last step of a call, as its own step does
+        const double cs2 = (1.0 - prm_.w0) / 3.0;
+        FusedSolute fs{pops_[cur_], pops_[nxt], 1.0 / (prm_.D / cs2 + 0.5), prm_.w0, (1.0 - prm_.w0) / 6.0, 1.0 / cs2};
+        const int rc = flow_->step_fused(fs, last);
+        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
+        cur_ = nxt;
+        ++steps_;
+        return LBM_OK;
+    }
     if (flow_) {
         const int rc = flow_->step_storing_velocity();
         if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
     }
     const void *u = velocity();
-    const int nxt = cur_ ^ 1;
     hipError_t e;
     if (st_bytes_ == 8)
         e = launch_st_t(make_st_args<double>(pops_[cur_], pops_[nxt], nbr_, u, d_cinit_, N_, P_, prm_), st_kmode(mode_),
@@ -423,7 +434,7 @@ int StSolver::step(int nsteps, double *elapsed_ms)
     LBM_HIP_TRY(hipSetDevice(device_));
     if (elapsed_ms) LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
     for (int s = 0; s < nsteps; ++s) {
-        const int rc = launch_step();
+        const int rc = launch_step(s == nsteps - 1);
         if (rc != LBM_OK) return rc;
     }
     if (elapsed_ms) {
