@@ -82,6 +82,51 @@ def cpu_baseline(solid, nu, g, budget_s, faces=None, power_law=None):
             "sample": f"FP64 C oracle (OpenMP) on the same 256^3 geometry, {n} steps after 1 warm-up step, {dt:.1f} s"}
 
 
+def reference_cpu_baseline(solid, nu, g, budget_s, faces=None, power_law=None):
+    """The reference program itself (oracle/_ref/sp_ref, compiled from its own source by
+    `make -C oracle ref`), under mpirun on the host's cores, on the same geometry: the time per
+    step is the difference of two runs of different length, so start-up and geometry reading
+    drop out.  None if the binary or mpirun is not there (the C port is used instead)."""
+    import subprocess
+    import tempfile
+    binary = os.path.join(ROOT, "oracle", "_ref", "sp_ref")
+    mpirun = "/opt/conda/bin/mpirun"
+    if not (os.path.exists(binary) and os.path.exists(mpirun)) or faces or power_law:
+        return None
+    pkg = load_package()
+    ranks = max(2, min(16, os.cpu_count() or 2))
+    nx, ny, nz = solid.shape
+    env = dict(os.environ, PATH=os.path.dirname(mpirun) + ":" + os.environ.get("PATH", ""))
+
+    def run(n_max, wd):
+        inp = pkg.inputs.SPInput(filename="geometry.bin", n_xyz=(nx, ny, nz), n_max=n_max, reso=1.0, g=tuple(g),
+                                 in_vis=nu, freRe=1 << 30, outputfile="./", fre_backup=-1, decbin=1)
+        pkg.inputs.write_sp_input(inp, os.path.join(wd, "INPUT.dat"))
+        t0 = time.perf_counter()
+        proc = subprocess.run([mpirun, "-np", str(ranks), binary, "INPUT.dat"], cwd=wd, env=env, capture_output=True,
+                              text=True, timeout=900)
+        if proc.returncode != 0:
+            raise RuntimeError("reference run failed: " + proc.stdout[-500:] + proc.stderr[-500:])
+        return time.perf_counter() - t0
+
+    try:
+        with tempfile.TemporaryDirectory() as wd:
+            pkg.geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+            t_short = run(1, wd)                       # 2 steps (n = 0, 1)
+            t_probe = run(3, wd)                       # 4 steps
+            per = max((t_probe - t_short) / 2.0, 1e-3)
+            extra = max(2, int(budget_s / per))
+            t_long = run(1 + extra, wd)
+    except (RuntimeError, subprocess.TimeoutExpired, OSError) as e:
+        print(f"[bench] reference CPU run unavailable: {e}", file=sys.stderr, flush=True)
+        return None
+    per = (t_long - t_short) / extra
+    count = int(np.count_nonzero(solid == 0))
+    return {"value": round(count / per / 1e6, 3), "unit": "MLUPS", "cores": ranks, "kind": "reference",
+            "sample": f"the reference program (SINGLE_PHASE_MPI_Software_Spars, FP64) under mpirun -np {ranks} on the "
+                      f"same geometry: ({t_long:.1f} s for {2 + extra} steps - {t_short:.1f} s for 2 steps) / {extra}"}
+
+
 def load_serving_truth(seed, nreq):
     """FP64 reference permeabilities of the serving stream (tests/golden/make_serving_truth.py)."""
     path = os.path.join(ROOT, "tests", "golden", "serving_stream_k.json")
@@ -529,7 +574,11 @@ def main():
     }
     s.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
+        ref = reference_cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
+        port = cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
+        out["cpu_baseline"] = ref if ref is not None else port
+        if ref is not None:
+            out["cpu_port"] = port   # the FP64 C restatement on the same cores, for comparison
     if not args.no_serve:
         out["serving"] = serving_bench(pkg, args.serve_requests, args.serve_rate)
     print(json.dumps(out))
