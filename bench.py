@@ -435,6 +435,42 @@ def vtk_bench(pkg, edge=256):
     return out
 
 
+def cg_accuracy(pkg, steps=20000, edge=192, every=1000):
+    """The colour-gradient workload for its full 20,000 steps in FP32 and in FP64: saturation,
+    phase fluxes and relative permeabilities of the FP32 run against the FP64 run every 1000
+    steps (interfaces move, so pointwise fields drift apart; these integral quantities are the
+    ones the reference reports)."""
+    geo = pkg.geometry
+    solid = geo.sphere_pack((edge, edge, edge), 6.0, 0.45, seed=2025)
+    psi = np.where(np.random.default_rng(2026).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
+    runs = {}
+    for precision in ("fp64", "fp32"):
+        s = pkg.ColorGradient(solid, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), precision=precision)
+        s.init(psi)
+        rows = []
+        t0 = time.perf_counter()
+        for n in range(every, steps + 1, every):
+            s.step(every)
+            kl, kg, _ = s.Comput_Perm(1)
+            d = s.diagnostics()
+            rows.append((n, s.Comput_Saturation(), kl[0], kg[0], d["nan_count"]))
+        runs[precision] = {"rows": rows, "seconds": round(time.perf_counter() - t0, 2)}
+        s.close()
+        print(f"[cg accuracy] {precision}: {steps} steps in {runs[precision]['seconds']} s, S_w {rows[-1][1]:.6f}",
+              file=sys.stderr, flush=True)
+    a, b = np.array(runs["fp32"]["rows"]), np.array(runs["fp64"]["rows"])
+    return {"workload": f"cg_d3q19_mrt_{edge}cube_spheres_r6_phi0.45_sw0.5", "steps": steps,
+            "fp64_final": {"saturation": b[-1, 1], "k_wetting": b[-1, 2], "k_nonwetting": b[-1, 3]},
+            "fp32_final": {"saturation": a[-1, 1], "k_wetting": a[-1, 2], "k_nonwetting": a[-1, 3]},
+            "max_abs_saturation_diff": float(np.abs(a[:, 1] - b[:, 1]).max()),
+            "max_rel_k_wetting_diff": float((np.abs(a[:, 2] - b[:, 2]) / np.abs(b[:, 2])).max()),
+            "max_rel_k_nonwetting_diff": float((np.abs(a[:, 3] - b[:, 3]) / np.abs(b[:, 3])).max()),
+            "final_rel_k_wetting_diff": float(abs(a[-1, 2] - b[-1, 2]) / abs(b[-1, 2])),
+            "final_rel_k_nonwetting_diff": float(abs(a[-1, 3] - b[-1, 3]) / abs(b[-1, 3])),
+            "nan_count": float(a[:, 4].max() + b[:, 4].max()),
+            "seconds": {k: v["seconds"] for k, v in runs.items()}}
+
+
 def accuracy_study(pkg, edge=256, max_steps=100000):
     """The single-phase workload run to steady state (|k_n - k_(n-100)| <= 1e-6 |k_n|, the
     reference's relative-change test at freRe = 100) in FP64, FP32 and FP16 storage; each
@@ -501,7 +537,7 @@ def main():
     if args.serve_only:
         print(json.dumps({"serving": serving_bench(pkg, args.serve_requests, args.serve_rate)}))
         return
-    if args.accuracy:
+    if args.accuracy and args.workload == "sp":
         print(json.dumps({"accuracy": accuracy_study(pkg, args.edge)}))
         return
     if args.vtk:
@@ -510,6 +546,9 @@ def main():
     if args.workload == "solute":
         prec = args.precision if args.precision != "fp16" else "fp32"
         print(json.dumps(solute_bench(pkg, args.steps, args.warmup, prec, cpu_s=args.cpu_seconds, no_cpu=args.no_cpu)))
+        return
+    if args.workload == "cg" and args.accuracy:
+        print(json.dumps({"cg_accuracy": cg_accuracy(pkg)}))
         return
     if args.workload == "cg":
         print(json.dumps(cg_bench(pkg, args.steps, args.warmup, args.precision if args.precision != "fp16" else "fp32")))

@@ -40,19 +40,19 @@ def _build_ref():
     subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
 
 
-def _run(binary, workdir, input_name="INPUT.dat"):
+def _run(binary, workdir, input_name="INPUT.dat", ranks=2):
     env = dict(os.environ)
     env["PATH"] = MPI_BIN + ":" + env.get("PATH", "")
-    proc = subprocess.run(["mpirun", "-np", "2", binary, input_name], cwd=workdir, env=env,
+    proc = subprocess.run(["mpirun", "-np", str(ranks), binary, input_name], cwd=workdir, env=env,
                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
     if proc.returncode != 0:
         raise RuntimeError(f"reference run failed in {workdir}:\n{proc.stdout[-3000:]}")
     return proc.stdout
 
 
-def _read_ckpt(workdir, name, label, width):
+def _read_ckpt(workdir, name, label, width, ranks=2):
     parts = []
-    for rank in range(2):
+    for rank in range(ranks):
         path = os.path.join(workdir, f"LBM_checkpoint_{name}_{label}.{rank}.bin_input")
         a = np.fromfile(path, dtype="<f8")
         a = a.reshape(-1, width) if width > 1 else a
@@ -264,6 +264,37 @@ def st_case(name, sp_fixture, niu_s, u_scale, c_seed, n_max, labels, freRe=5):
     print("wrote", path, "count =", u.shape[0], "ranks", counts)
 
 
+def c1_case(name, steps=2000, ranks=8):
+    """The reference's own CPU case at full size: 64^3 spheres r = 6 to porosity 0.6 (seed
+    12345, 157,141 fluid nodes), tau = 1, g = (1e-6, 0, 0), FP64, 2000 steps.  The state is too
+    large to keep whole, so the fixture holds every 64th node's f, u and rho, the per-direction
+    sums of f over all nodes, the sums of u and rho, and the printed permeability file."""
+    solid = geometry.sphere_pack((64, 64, 64), 6.0, 0.6, 12345)
+    wd = tempfile.mkdtemp(prefix="sp_c1_")
+    try:
+        geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+        inp = inputs.SPInput(filename="geometry.bin", n_xyz=(64, 64, 64), n_max=steps - 1, reso=1.0, g=(1e-6, 0.0, 0.0),
+                             in_vis=1.0 / 6.0, ini_v=(0.0, 0.0, 0.0), wr_per=1, PerDir=1, freRe=100, Out_Mode=0,
+                             outputfile="./", fre_backup=0, decbin=1)
+        inputs.write_sp_input(inp, os.path.join(wd, "INPUT.dat"))
+        _run(os.path.join(REF_DIR, "sp_ref"), wd, ranks=ranks)
+        f = _read_ckpt(wd, "f", steps - 1, 19, ranks)
+        u = _read_ckpt(wd, "velocity", steps - 1, 3, ranks)
+        rho = _read_ckpt(wd, "rho", steps - 1, 1, ranks)
+        pick = np.arange(0, f.shape[0], 64)
+        out = {"solid": solid.astype(np.int8), "pick": pick, "f_pick": f[pick], "u_pick": u[pick], "rho_pick": rho[pick],
+               "f_sum": f.sum(0), "u_sum": u.sum(0), "rho_sum": np.array(rho.sum()),
+               "perm_txt": np.array(_text(wd, "Permeability_error_local_Perm.txt")),
+               "params": np.array(json.dumps(dict(shape=[64, 64, 64], nu=1.0 / 6.0, g=[1e-6, 0.0, 0.0], u0=[0, 0, 0],
+                                                  steps=steps, radius=6.0, porosity_target=0.6, seed=12345,
+                                                  count=int(f.shape[0]), ranks=ranks)))}
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, "count =", f.shape[0])
+
+
 def main(only=()):
     """Write every fixture, or only those whose names are given on the command line."""
     _build_ref()
@@ -309,6 +340,7 @@ def main(only=()):
                                                   power_law=(0.05, 1.3))),
         # the field writers: legacy VTK (velocity, density, geometry), the velocity profile and
         # the solute-velocity files at n = 0, 10, 20
+        ("sp_c1_64cube", lambda n: c1_case(n)),
         ("sp_out_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
                                               u0=(1e-3, -2e-3, 5e-4), steps_list=[21], field_outputs=True)),
     ]

@@ -188,3 +188,30 @@ def test_reduced_precision_permeability_vs_fp64(pkg, precision, k_tol):
     _log(f"[gpu] {precision}: permeability rel err {k_err:.3e}, velocity L2 rel err {u_err:.3e}")
     assert k_err < k_tol
     assert u_err < 10 * k_tol
+
+
+def test_fp64_full_size_reference_case(pkg):
+    """The reference's own case at full size (64^3 spheres, 157,141 fluid nodes, tau = 1,
+    g = 1e-6, 2000 steps): every 64th node's state, the per-direction sums of f over all nodes
+    and the permeability the reference printed every 100 steps."""
+    _log("[gpu] fp64 full-size reference case 64^3 x 2000 steps")
+    d, p = load_fixture("sp_c1_64cube")
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    assert s.count == p["count"]
+    s.init(1.0, p["u0"])
+    printed = [float(line.split()[0]) for line in str(d["perm_txt"]).strip().splitlines()]
+    ks = []
+    for n in range(0, p["steps"], 100):   # loop steps n = 0, 100, ..., 1900: after n + 1 steps
+        s.step(n + 1 - s.steps_done)
+        ks.append(s.Comput_Perm(1)[0][0] * 1000)
+    s.step(p["steps"] - s.steps_done)
+    f, u, rho = s.state()
+    pick = d["pick"]
+    # the reference's few-ulp M^-1 literals accumulate to ~1.4e-13 on f over 2000 steps
+    assert np.abs(f[pick] - d["f_pick"]).max() < 5e-13
+    assert np.abs(u[pick] - d["u_pick"]).max() < 2e-14
+    np.testing.assert_allclose(f.sum(0), d["f_sum"], rtol=1e-12)
+    np.testing.assert_allclose(u.sum(0), d["u_sum"], rtol=1e-8)
+    assert len(printed) == len(ks)
+    for a, b in zip(ks, printed):
+        assert abs(a - b) <= 2e-5 * abs(b)    # six printed digits

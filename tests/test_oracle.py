@@ -356,3 +356,19 @@ def test_sp_power_law_matches_reference(oracle_mod, name):
             assert np.abs(newt.f[1:] - ref).max() < 1e-14
         else:
             assert np.abs(newt.f[1:] - ref).max() > 1e-6
+
+
+def test_sp_oracle_full_size_reference_case(oracle_mod):
+    """The reference's own case at full size (64^3, 157,141 nodes, 2000 steps, 8 ranks): the
+    oracle's state at every 64th node and its per-direction sums over all nodes."""
+    d, p = load_fixture("sp_c1_64cube")
+    o = oracle_mod.SPOracle(d["solid"].astype(np.int32), p["nu"], p["g"], p["u0"], nthreads=8)
+    assert o.count == p["count"]
+    o.run(p["steps"])
+    pick = d["pick"]
+    # 2000 steps accumulate the reference's few-ulp M^-1 literals into ~1.4e-13 on f (4e-13
+    # relative) and ~6e-15 on u (|u| up to 1.6e-4); the sum of u cancels to 3e-9 relative
+    assert np.abs(o.f[1:][pick] - d["f_pick"]).max() < 5e-13
+    assert np.abs(o.u[1:][pick] - d["u_pick"]).max() < 2e-14
+    np.testing.assert_allclose(o.f[1:].sum(0), d["f_sum"], rtol=1e-12)
+    np.testing.assert_allclose(o.u[1:].sum(0), d["u_sum"], rtol=1e-8)
