@@ -305,6 +305,13 @@ const char *lbm_last_error(const lbm_handle *h);   /* NULL handle: last create()
 void lbm_destroy(lbm_handle *h);
 /* library version string and the device architecture the kernels were built for */
 const char *lbm_version(void);
+/* Host only, no device use.  *wraps = 1 when some cluster of pore voxels, joined by the 18
+ * D3Q19 links with the box periodic on every axis, wraps round the box along axis (0 = x,
+ * 1 = y, 2 = z), else 0.  Without such a cluster there is no throughflow along the axis at
+ * steady state.  No counterpart in the reference: the server uses it to send samples whose
+ * pores do not wrap along the permeability direction to FP64 at admission (their k is the
+ * lattice's spurious-current residue, below FP32 resolution); callers can screen samples. */
+int lbm_geometry_wraps(int32_t nx, int32_t ny, int32_t nz, const int32_t *solid_ijk, int32_t axis, int32_t *wraps);
 
 #ifdef __cplusplus
 }

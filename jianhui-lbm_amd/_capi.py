@@ -125,6 +125,7 @@ def lib():
         L.lbm_last_error.argtypes = [vp]
         L.lbm_destroy.argtypes = [vp]
         L.lbm_destroy.restype = None
+        L.lbm_geometry_wraps.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ip, ctypes.c_int32, ip]
         L.lbm_sp_create.argtypes = [ctypes.POINTER(SpParams), ip, ctypes.POINTER(vp)]
         L.lbm_sp_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.lbm_sp_init.argtypes = [vp, ctypes.c_double, dp]

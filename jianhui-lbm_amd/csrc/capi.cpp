@@ -38,6 +38,23 @@ const char *lbm_last_error(const lbm_handle *h)
 
 void lbm_destroy(lbm_handle *h) { delete h; }
 
+int lbm_geometry_wraps(int32_t nx, int32_t ny, int32_t nz, const int32_t *solid_ijk, int32_t axis, int32_t *wraps)
+{
+    if (!wraps || axis < 0 || axis > 2) {
+        g_create_error = "lbm_geometry_wraps: axis must be 0, 1 or 2 and wraps non-null";
+        return LBM_ERR_ARG;
+    }
+    lbm::SparseGeometry g;
+    std::string err;
+    const int rc = lbm::build_sparse_geometry(nx, ny, nz, solid_ijk, g, err);
+    if (rc != LBM_OK) {
+        g_create_error = "lbm_geometry_wraps: " + err;
+        return rc;
+    }
+    *wraps = lbm::pores_wrap_along(g, axis) ? 1 : 0;
+    return LBM_OK;
+}
+
 int lbm_sp_create(const lbm_sp_params *p, const int32_t *solid_ijk, lbm_handle **out)
 {
     if (!p || !solid_ijk || !out) {

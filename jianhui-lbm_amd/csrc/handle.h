@@ -41,6 +41,11 @@ struct SparseGeometry {
 int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid_ijk, SparseGeometry &g,
                           std::string &err);
 
+// Whether some cluster of pore nodes joined by the 18 lattice links wraps round the periodic box
+// along axis (0 = x, 1 = y, 2 = z).  Without such a cluster there is no net throughflow along
+// the axis at steady state: the permeability is zero up to the lattice's spurious currents.
+bool pores_wrap_along(const SparseGeometry &g, int axis);
+
 // Round up to a multiple of the block size so every direction row starts 256-byte aligned.
 inline int64_t padded_stride(int64_t n) { return ((n + 255) / 256) * 256; }
 

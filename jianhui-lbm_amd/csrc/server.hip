@@ -724,6 +724,12 @@ void Server::prep_worker()
         bool ok = build_sparse_geometry(job.nx, job.ny, job.nz, job.solid.data(), geo, err) == LBM_OK;
         std::vector<int32_t>().swap(job.solid);
         if (ok) ok = pack_deltas(geo, pr.nbr);   // guaranteed by the window check in submit
+        // A sample whose pores do not wrap round the box along the permeability direction has no
+        // throughflow: its k is the lattice's spurious-current residue, orders of magnitude
+        // below what FP32 deviations resolve.  With escalation on, it goes to the FP64 server
+        // at once instead of after an FP32 run that ends with its k below escalate_k.
+        const bool to_fp64 = ok && prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && fallback_ &&
+                             fallback_->fits(geo.P) && !pores_wrap_along(geo, job.dir - 1);
         pr.ticket = job.ticket;
         pr.t_submit = job.t_submit;
         pr.N = geo.N;
@@ -735,7 +741,10 @@ void Server::prep_worker()
         {
             std::lock_guard<std::mutex> lk(mu_);
             --prep_busy_;
-            if (ok) {
+            if (to_fp64) {
+                fallback_->enqueue(std::move(pr));
+                ++escalated_;
+            } else if (ok) {
                 pending_.push_back(std::move(pr));
             } else {
                 lbm_srv_result r{};

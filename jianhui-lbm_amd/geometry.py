@@ -168,5 +168,23 @@ def fibre_pack(shape, radius: float, porosity: float, seed: int) -> np.ndarray:
     return solid.astype(np.int32)
 
 
+def pores_wrap(solid: np.ndarray, axis: int) -> bool:
+    """Whether a cluster of pore voxels, joined by the 18 D3Q19 links with the box periodic on
+    every axis, wraps round the box along ``axis`` (0 = x, 1 = y, 2 = z).  Without one there is
+    no throughflow along the axis at steady state.  Host code of the library (no device use):
+    ``lbm_geometry_wraps`` of the C ABI."""
+    import ctypes
+    from . import _capi
+    s = np.ascontiguousarray(solid, dtype=np.int32)
+    if s.ndim != 3:
+        raise ValueError("solid must be a 3-D [i][j][k] array")
+    out = ctypes.c_int32(0)
+    rc = _capi.lib().lbm_geometry_wraps(int(s.shape[0]), int(s.shape[1]), int(s.shape[2]),
+                                        s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(axis),
+                                        ctypes.byref(out))
+    _capi.check(rc, None)
+    return bool(out.value)
+
+
 def porosity(solid: np.ndarray) -> float:
     return float(np.count_nonzero(solid == 0)) / solid.size

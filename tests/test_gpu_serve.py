@@ -176,6 +176,33 @@ def test_escalated_sample_continues_in_fp64(pkg):
         assert e["t_done"] >= e["t_admit"] >= e["t_submit"]
 
 
+def test_sample_without_throughflow_goes_to_fp64_at_admission(pkg):
+    """With escalation on, a sample whose pores do not wrap round the box along the force
+    direction (no throughflow: its k is the lattice's spurious-current residue) is sent to the
+    FP64 server when it is prepared.  Its result is that server's, bit for bit and step for
+    step, with no FP32 steps before it; a sample that wraps stays in the FP32 arena."""
+    _log("[gpu] serve: no-throughflow sample routed to fp64")
+    geo = pkg.geometry
+    open_s = geo.grf_sample(24, 3.0, 0.4, seed=101)
+    shut = open_s.copy()
+    shut[0] = 1                                      # a solid x-plane: no path wraps along x
+    assert geo.pores_wrap(open_s, 0) and not geo.pores_wrap(shut, 0) and geo.pores_wrap(shut, 1)
+    reqs = [(shut, 1.0 / 6.0, (1e-5, 0.0, 0.0)), (open_s, 1.0 / 6.0, (1e-5, 0.0, 0.0))]
+    kw = dict(capacity_nodes=400_000, check_interval=50, tol=1e-6, max_steps=20000, block=256, **AA)
+    with pkg.PermeabilityServer(escalate_k=1e-9, **kw) as srv:
+        got = _run(srv, reqs)
+        st = srv.stats()
+    with pkg.PermeabilityServer(precision="fp64", **kw) as srv:
+        ref = _run(srv, reqs[:1])[0]
+    _log(f"[gpu] routed k {got[0]['k'][0]:.3e} after {got[0]['steps']} steps; open sample k {got[1]['k'][0]:.3e}")
+    assert st["done"] == 2 and st["pending"] == 0 and st["active"] == 0
+    assert got[0]["precision"] == "fp64" and got[1]["precision"] == "fp32"
+    assert got[0]["steps"] == ref["steps"] and got[0]["converged"] == ref["converged"]
+    np.testing.assert_array_equal(got[0]["k"], ref["k"])
+    # what remains of the decaying transient when the relative-change rule retires it
+    assert abs(got[0]["k"][0]) < 1e-2 * abs(got[1]["k"][0])
+
+
 def test_rejects_bad_requests(pkg):
     with pkg.PermeabilityServer(capacity_nodes=50_000, block=256) as srv:
         with pytest.raises(pkg.LBMError):
