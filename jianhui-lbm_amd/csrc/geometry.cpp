@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "connectivity.h"
 #include "handle.h"
 #include "lattice.h"
 
@@ -63,62 +64,13 @@ int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid, SparseGe
     return LBM_OK;
 }
 
-// Union-find over the lattice links with each node's displacement (along the axis) from its
-// set's root.  Linking node n to its upstream node m = n - e_q fixes pos(n) - pos(m) = e_q; a
-// link inside one set whose displacements disagree closes a loop that goes round the periodic
-// box along the axis.
 bool pores_wrap_along(const SparseGeometry &g, int axis)
 {
-    const int *E = axis == 0 ? EX : (axis == 1 ? EY : EZ);
-    const int64_t N = g.N, P = g.P;
-    // parent and offset of each node side by side (one cache line serves both); displacements
-    // within a set are bounded by the node count, so 32 bits hold them
-    std::vector<int32_t> pd((size_t)(2 * N));
-    for (int64_t n = 0; n < N; ++n) {
-        pd[(size_t)(2 * n)] = (int32_t)n;
-        pd[(size_t)(2 * n + 1)] = 0;   // pos(n) - pos(parent)
-    }
-    // root of n and pos(n) - pos(root), halving the path on the way (each visited node is
-    // re-pointed at its grandparent)
-    auto find = [&pd](int32_t n, int32_t &d) {
-        d = 0;
-        for (;;) {
-            const int32_t p = pd[(size_t)(2 * n)];
-            if (p == n) return n;
-            const int32_t gp = pd[(size_t)(2 * p)];
-            const int32_t dp = pd[(size_t)(2 * n + 1)], dg = pd[(size_t)(2 * p + 1)];
-            pd[(size_t)(2 * n)] = gp;
-            pd[(size_t)(2 * n + 1)] = dp + dg;
-            d += dp + dg;
-            n = gp;
-        }
-    };
-    int qs[9], nq = 0;
-    for (int q = 1; q < Q; ++q)
-        if (opp(q) > q) qs[nq++] = q;   // each link once: the reverse direction joins the same pair
-    for (int64_t n = 0; n < N; ++n)
-        for (int t = 0; t < nq; ++t) {
-            const int q = qs[t];
-            const uint32_t v = g.nbr[(size_t)((q - 1) * P + n)];
-            if (v & BB_FLAG) continue;
-            int32_t dn, dm;
-            const int32_t rn = find((int32_t)n, dn), rm = find((int32_t)v, dm);
-            if (rn == rm) {
-                if (dn - dm != E[q]) return true;
-                continue;
-            }
-            // pos(rm) - pos(rn) = (pos(m) - dm) - (pos(n) - dn) = dn - dm - e_q; the later root
-            // goes under the earlier one, which keeps the trees shallow in this scan order
-            const int32_t d = dn - dm - E[q];
-            if (rn < rm) {
-                pd[(size_t)(2 * rm)] = rn;
-                pd[(size_t)(2 * rm + 1)] = d;
-            } else {
-                pd[(size_t)(2 * rn)] = rm;
-                pd[(size_t)(2 * rn + 1)] = -d;
-            }
-        }
-    return false;
+    const int64_t P = g.P;
+    return lattice_loop_wraps(g.N, axis, [&g, P](int64_t n, int q) -> int64_t {
+        const uint32_t v = g.nbr[(size_t)((q - 1) * P + n)];
+        return (v & BB_FLAG) ? -1 : (int64_t)v;
+    });
 }
 
 int pack_neighbours(const SparseGeometry &g, PackedNbr &out, std::string &err)

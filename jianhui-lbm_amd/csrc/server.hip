@@ -35,6 +35,7 @@
 
 #include "collide.h"
 #include "common.h"
+#include "connectivity.h"
 #include "handle.h"
 #include "pull.h"
 #include "storage.h"
@@ -288,7 +289,9 @@ struct Prepared {
     int prior_acc = 0;
     std::vector<float> carried;  // 19 x npad FP32 state to continue from (escalation), else start at rest
     int64_t N = 0, npad = 0;
-    std::vector<int16_t> nbr;    // 18 x npad upstream deltas (SRV_BB = bounce back)
+    // 18 x npad upstream deltas (SRV_BB = bounce back); shared with the preparation worker's
+    // connectivity test, which may still read them after the sample is queued
+    std::shared_ptr<std::vector<int16_t>> nbr;
     double nu = 0, g[3] = {0, 0, 0};
     int dir = 1;
     int64_t volume = 0;
@@ -312,6 +315,7 @@ struct Active {
     double k_ev = 0;           // k at the previous event
     bool has_k_ev = false;
     int small_events = 0;      // consecutive events whose k changed by <= tol
+    bool no_wrap = false;      // pores found not to wrap along dir: to FP64 at the next check
 };
 
 // gamma = (G + lambda I)^-1 b for the n x n leading block, Gaussian elimination with partial
@@ -381,28 +385,61 @@ int64_t max_window_pores(const int32_t *solid, int64_t V, int64_t L)
     return best;
 }
 
-// The sample's pull lists as 16-bit deltas: for each link the shortest of up - n, up - n - N,
-// up - n + N (the device adds it to n and reduces modulo N).  False if a link does not fit,
-// which needs an x-plane of more than 32767 fluid nodes.
-bool pack_deltas(const SparseGeometry &g, std::vector<int16_t> &out)
+// The sample's pore numbering (i-major, x slowest: the same as build_sparse_geometry) and its
+// pull links as 16-bit deltas, built in one pass without the 32-bit table: for each link the
+// shortest of up - n, up - n - N, up - n + N (the device adds it to n and reduces modulo N).
+// False if the box has no pores or a link does not fit, which needs an x-plane span of more
+// than 32767 fluid nodes (refused in submit).
+bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int64_t &P, std::vector<int16_t> &out)
 {
-    const int64_t N = g.N, P = g.P;
+    const int64_t V = (int64_t)nx * ny * nz;
+    std::vector<int32_t> idx((size_t)V);
+    int64_t c = 0;
+    for (int64_t v = 0; v < V; ++v) idx[(size_t)v] = solid[v] == 0 ? (int32_t)c++ : -1;
+    N = c;
+    P = padded_stride(N);
+    if (N == 0) return false;
     out.assign((size_t)(18 * P), SRV_BB);
-    bool ok = true;
-    for (int q = 0; q < 18 && ok; ++q)
-        for (int64_t n = 0; n < N; ++n) {
-            const uint32_t v = g.nbr[(size_t)(q * P + n)];
-            if (v & BB_FLAG) continue;
-            int64_t d = (int64_t)v - n, best = d;
-            for (int64_t c : {d - N, d + N})
-                if (std::llabs(c) < std::llabs(best)) best = c;
-            if (best < -32767 || best > 32767) {
-                ok = false;
-                break;
+    const int64_t NN = N;
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int32_t n = idx[(size_t)(((int64_t)i * ny + j) * nz + k)];
+                if (n < 0) continue;
+                for (int q = 1; q < Q; ++q) {
+                    int iu = i - EX[q], ju = j - EY[q], ku = k - EZ[q];
+                    iu = iu < 0 ? iu + nx : (iu >= nx ? iu - nx : iu);
+                    ju = ju < 0 ? ju + ny : (ju >= ny ? ju - ny : ju);
+                    ku = ku < 0 ? ku + nz : (ku >= nz ? ku - nz : ku);
+                    const int32_t up = idx[(size_t)(((int64_t)iu * ny + ju) * nz + ku)];
+                    if (up < 0) continue;   // bounce back: SRV_BB stays
+                    const int64_t d = (int64_t)up - n;
+                    int64_t best = d;
+                    if (std::llabs(d - NN) < std::llabs(best)) best = d - NN;
+                    if (std::llabs(d + NN) < std::llabs(best)) best = d + NN;
+                    if (best < -32767 || best > 32767) {
+                        bad = 1;
+                        continue;
+                    }
+                    out[(size_t)((q - 1) * P + n)] = (int16_t)best;
+                }
             }
-            out[(size_t)(q * P + n)] = (int16_t)best;
-        }
-    return ok;
+    return bad == 0;
+}
+
+// Whether the sample's pores wrap round the box along axis, from its deltas (connectivity.h).
+bool deltas_wrap_along(const std::vector<int16_t> &nbr, int64_t N, int64_t P, int axis)
+{
+    return lattice_loop_wraps(N, axis, [&nbr, N, P](int64_t n, int q) -> int64_t {
+        const int16_t d = nbr[(size_t)((q - 1) * P + n)];
+        if (d == SRV_BB) return -1;
+        int64_t m = n + d;
+        m += m < 0 ? N : 0;
+        m -= m >= N ? N : 0;
+        return m;
+    });
 }
 }  // namespace
 
@@ -426,7 +463,7 @@ public:
 private:
     void shutdown();
     void loop();
-    bool escalate_locked(Active &a);
+    bool escalate_locked(Active &a, bool restart);
     bool make_fallback(int64_t min_nodes);
     int run_chunk(std::unique_lock<std::mutex> &lk);
     bool try_admit_locked();
@@ -518,25 +555,32 @@ bool Server::make_fallback(int64_t min_nodes)
     return true;
 }
 
-bool Server::escalate_locked(Active &a)
+// restart: the FP64 solve starts from rest and the FP32 steps are not counted, so that the
+// result does not depend on how far the FP32 run had got (the hand-over of a sample found not to
+// wrap comes from a preparation thread, at a time that depends on scheduling).
+bool Server::escalate_locked(Active &a, bool restart)
 {
     if (!fallback_ && !make_fallback(a.req.npad)) return false;
     if (!fallback_->fits(a.req.npad)) return false;
-    a.req.nbr.resize((size_t)18 * a.req.npad);
-    hipError_t e = hipMemcpy2DAsync(a.req.nbr.data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
+    a.req.nbr = std::make_shared<std::vector<int16_t>>((size_t)18 * a.req.npad);
+    hipError_t e = hipMemcpy2DAsync(a.req.nbr->data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
                                     (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                     hipMemcpyDeviceToHost, stream_);
-    // the FP64 solve continues from the FP32 state rather than from rest: the transient so
-    // far (the pressure build-up of the slow mode) is kept
-    a.req.carried.resize((size_t)Q * a.req.npad);
-    if (e == hipSuccess)
-        e = hipMemcpy2DAsync(a.req.carried.data(), (size_t)a.req.npad * sizeof(float), (const float *)pops_[cur_] + a.base,
-                             (size_t)cap_ * sizeof(float), (size_t)a.req.npad * sizeof(float), Q, hipMemcpyDeviceToHost,
-                             stream_);
+    // otherwise the FP64 solve continues from the FP32 state rather than from rest: the
+    // transient so far (the pressure build-up of the slow mode) is kept
+    if (!restart) {
+        a.req.carried.resize((size_t)Q * a.req.npad);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(a.req.carried.data(), (size_t)a.req.npad * sizeof(float),
+                                 (const float *)pops_[cur_] + a.base, (size_t)cap_ * sizeof(float),
+                                 (size_t)a.req.npad * sizeof(float), Q, hipMemcpyDeviceToHost, stream_);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) return false;
-    a.req.prior_steps += a.steps;
-    a.req.prior_acc += a.aa_updates;
+    if (!restart) {
+        a.req.prior_steps += a.steps;
+        a.req.prior_acc += a.aa_updates;
+    }
     fallback_->enqueue(std::move(a.req));
     ++escalated_;
     return true;
@@ -718,33 +762,25 @@ void Server::prep_worker()
             prep_q_.pop_front();
             ++prep_busy_;
         }
-        SparseGeometry geo;
-        std::string err;
         Prepared pr;
-        bool ok = build_sparse_geometry(job.nx, job.ny, job.nz, job.solid.data(), geo, err) == LBM_OK;
+        int64_t N = 0, P = 0;
+        auto nbr = std::make_shared<std::vector<int16_t>>();
+        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.solid.data(), N, P, *nbr);   // fits: checked in submit
         std::vector<int32_t>().swap(job.solid);
-        if (ok) ok = pack_deltas(geo, pr.nbr);   // guaranteed by the window check in submit
-        // A sample whose pores do not wrap round the box along the permeability direction has no
-        // throughflow: its k is the lattice's spurious-current residue, orders of magnitude
-        // below what FP32 deviations resolve.  With escalation on, it goes to the FP64 server
-        // at once instead of after an FP32 run that ends with its k below escalate_k.
-        const bool to_fp64 = ok && prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && fallback_ &&
-                             fallback_->fits(geo.P) && !pores_wrap_along(geo, job.dir - 1);
+        pr.nbr = nbr;
         pr.ticket = job.ticket;
         pr.t_submit = job.t_submit;
-        pr.N = geo.N;
-        pr.npad = geo.P;
+        pr.N = N;
+        pr.npad = P;
         pr.nu = job.nu;
         for (int c = 0; c < 3; ++c) pr.g[c] = job.g[c];
         pr.dir = job.dir;
         pr.volume = (int64_t)job.nx * job.ny * job.nz;
+        const bool test_wrap = ok && prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && fallback_ && fallback_->fits(P);
         {
             std::lock_guard<std::mutex> lk(mu_);
             --prep_busy_;
-            if (to_fp64) {
-                fallback_->enqueue(std::move(pr));
-                ++escalated_;
-            } else if (ok) {
+            if (ok) {
                 pending_.push_back(std::move(pr));
             } else {
                 lbm_srv_result r{};
@@ -756,6 +792,25 @@ void Server::prep_worker()
                 done_.push_back(r);
                 ++n_done_;
             }
+        }
+        cv_.notify_all();
+        idle_cv_.notify_all();
+        // A sample whose pores do not wrap round the box along the permeability direction has no
+        // throughflow: its k is the lattice's spurious-current residue, orders of magnitude
+        // below what FP32 deviations resolve.  With escalation on it goes to the FP64 server:
+        // at once if it is still queued, at its next check if it has been admitted meanwhile.
+        // The test runs after queueing so that it does not delay admission.
+        if (test_wrap && !deltas_wrap_along(*nbr, N, P, job.dir - 1)) {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto it = pending_.begin(); it != pending_.end(); ++it)
+                if (it->ticket == job.ticket) {
+                    fallback_->enqueue(std::move(*it));
+                    pending_.erase(it);
+                    ++escalated_;
+                    break;
+                }
+            for (Active &a : active_)
+                if (a.req.ticket == job.ticket) a.no_wrap = true;
         }
         cv_.notify_all();
         idle_cv_.notify_all();
@@ -855,7 +910,7 @@ bool Server::try_admit_locked()
         a.t_admit = clock();
         a.aa_next = aa_warm_;
         // sample-local neighbour rows go straight into the arena rows of the segment
-        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.data(),
+        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr->data(),
                                         (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                         hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess && !a.req.carried.empty()) {
@@ -885,7 +940,7 @@ bool Server::try_admit_locked()
             loop_error_ = true;
             loop_msg_ = std::string("neighbour upload failed: ") + hipGetErrorString(e);
         }
-        for (Active &a : active_) std::vector<int16_t>().swap(a.req.nbr);
+        for (Active &a : active_) a.req.nbr.reset();
     }
     return any;
 }
@@ -1101,7 +1156,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
             status[s] = 0;
         // precision escalation: a flux too small for FP32 deviations, or no convergence
         if (prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 &&
-            (status[s] == 0 ||
+            (status[s] == 0 || (status[s] == 2 && a.no_wrap) ||
              (status[s] == 2 && a.steps >= aa_warm_ && a.steps >= prm_.min_steps && std::fabs(kd) < prm_.escalate_k)))
             status[s] = 3;
         a.k_prev = kd;
@@ -1155,7 +1210,7 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
         }
         if (status[s] == 3) {
             const int64_t base = a.base, npad = a.req.npad;
-            if (escalate_locked(a)) {
+            if (escalate_locked(a, a.no_wrap)) {
                 free_segment(base, npad);
                 changed = true;
                 continue;
@@ -1226,27 +1281,22 @@ int Server::autotune()
                                  0.5 * std::sin(0.13 * (i + 2 * j) + 0.4) * std::cos(0.17 * (k - j));
                 solid[((size_t)i * L + j) * L + k] = v > -0.05 ? 1 : 0;
             }
-    SparseGeometry geo;
-    std::string err;
-    if (build_sparse_geometry(L, L, L, solid.data(), geo, err) != LBM_OK || geo.P > cap_) {
+    int64_t N = 0, P = 0;
+    std::vector<int16_t> d16;
+    if (!build_deltas(L, L, L, solid.data(), N, P, d16) || P > cap_) {
         blk_ = 256;
         return LBM_OK;
     }
     Active a;
-    a.req.N = geo.N;
-    a.req.npad = geo.P;
+    a.req.N = N;
+    a.req.npad = P;
     a.req.nu = 1.0 / 6.0;
     a.req.g[0] = 1e-6;
     a.req.volume = (int64_t)L * L * L;
     a.base = 0;
     a.first_step = 0;
-    std::vector<int16_t> d16;
-    if (!pack_deltas(geo, d16)) {
-        blk_ = 256;
-        return LBM_OK;
-    }
-    LBM_HIP_TRY(hipMemcpy2DAsync(d_nbr_, (size_t)cap_ * sizeof(int16_t), d16.data(), (size_t)geo.P * sizeof(int16_t),
-                                 (size_t)geo.P * sizeof(int16_t), 18, hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemcpy2DAsync(d_nbr_, (size_t)cap_ * sizeof(int16_t), d16.data(), (size_t)P * sizeof(int16_t),
+                                 (size_t)P * sizeof(int16_t), 18, hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
     active_.push_back(std::move(a));
     float best = 1e30f;
