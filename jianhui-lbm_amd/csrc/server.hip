@@ -66,8 +66,72 @@ struct TileDev {
     int64_t base;         // the sample's first arena node (local index origin)
 };
 
+// Addresses: a tile's rows start at the uniform pointer src + base (its sample's first node), and
+// each lane adds a 32-bit BYTE offset, (row * CAP + sample-local node) * sizeof(value), which
+// fits because the arena holds fewer than 2^32 / (19 * sizeof(value)) nodes.  A uniform base plus
+// a 32-bit unsigned offset is the load's own addressing form: one VGPR per address, no 64-bit
+// arithmetic.
+template <typename T>
+__device__ __forceinline__ T *byte_at(T *base, uint32_t off)
+{
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ const T *byte_at(const T *base, uint32_t off)
+{
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
 template <typename ST, int BLK>
 __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
+                                                       const ST *__restrict__ src, ST *__restrict__ dst,
+                                                       const int16_t *__restrict__ nbr, int64_t CAP, int64_t step)
+{
+    using C = typename Store<ST>::C;
+    constexpr uint32_t SZ = sizeof(ST);
+    const TileDev t = tiles[stream_tile(blockIdx.x, gridDim.x)];
+    const int64_t n = t.begin + threadIdx.x;
+    if (n >= t.end) return;
+    const SlotDev sl = slots[t.slot];
+    const C gx = (C)sl.gx, gy = (C)sl.gy, gz = (C)sl.gz, rho0 = (C)sl.rho0;
+    const uint32_t cap = (uint32_t)CAP, ln = (uint32_t)(n - t.base);
+    const ST *__restrict__ S = src + t.base;
+    C h[Q], m[Q];
+    Macro<C> mc;
+    if (step == sl.first_step) {   // f = feq(1, 0): the reference's init (SP:1343-1376)
+        mc.rho = C(1);
+        mc.drho = C(1) - rho0;
+        mc.ux = mc.uy = mc.uz = C(0);
+        sp_eq_moments(mc, m);
+    } else {
+        const int16_t *__restrict__ NB = nbr + t.base;
+        const int32_t cnt = (int32_t)sl.count;
+        h[0] = Store<ST>::load(*byte_at(S, ln * SZ), 1.0f);
+        int16_t d[Q - 1];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) d[q - 1] = *byte_at(NB, ((uint32_t)(q - 1) * cap + ln) * 2u);
+#pragma unroll
+        for (int q = 1; q < Q; ++q) {
+            // upstream node, sample-local: ln + d reduced modulo the node count (|d| <= count/2)
+            int32_t u = (int32_t)ln + d[q - 1];
+            u += u < 0 ? cnt : 0;
+            u -= u >= cnt ? cnt : 0;
+            const uint32_t e = d[q - 1] == SRV_BB ? (uint32_t)opp(q) * cap + ln : (uint32_t)q * cap + (uint32_t)u;
+            h[q] = Store<ST>::load(*byte_at(S, e * SZ), 1.0f);
+        }
+        moments_fwd(h, m);
+        mc = macro_from_moments(m, rho0, gx, gy, gz);
+    }
+    sp_relax(m, mc, gx, gy, gz, (C)sl.sv, (C)sl.sq, h);
+    ST *__restrict__ D = dst + t.base;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) Store<ST>::store_nt(byte_at(D, ((uint32_t)q * cap + ln) * SZ), h[q], 1.0f);
+}
+
+// FP64 arena: 64-bit element indices, which keep this kernel at 70 VGPRs (7 waves per SIMD)
+// where the byte-offset form below needs 84 (5 waves)
+template <typename ST, int BLK>
+__global__ void __launch_bounds__(BLK) srv_step_kernel_wide(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
                                                        const ST *__restrict__ src, ST *__restrict__ dst,
                                                        const int16_t *__restrict__ nbr, int64_t CAP, int64_t step)
 {
@@ -291,7 +355,7 @@ struct Prepared {
     int64_t N = 0, npad = 0;
     // 18 x npad upstream deltas (SRV_BB = bounce back); shared with the preparation worker's
     // connectivity test, which may still read them after the sample is queued
-    std::shared_ptr<std::vector<int16_t>> nbr;
+    std::shared_ptr<int16_t[]> nbr;
     double nu = 0, g[3] = {0, 0, 0};
     int dir = 1;
     int64_t volume = 0;
@@ -390,49 +454,68 @@ int64_t max_window_pores(const int32_t *solid, int64_t V, int64_t L)
 // shortest of up - n, up - n - N, up - n + N (the device adds it to n and reduces modulo N).
 // False if the box has no pores or a link does not fit, which needs an x-plane span of more
 // than 32767 fluid nodes (refused in submit).
-bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int64_t &P, std::vector<int16_t> &out)
+bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int64_t &P, std::shared_ptr<int16_t[]> &out)
 {
-    const int64_t V = (int64_t)nx * ny * nz;
-    std::vector<int32_t> idx((size_t)V);
-    int64_t c = 0;
-    for (int64_t v = 0; v < V; ++v) idx[(size_t)v] = solid[v] == 0 ? (int32_t)c++ : -1;
-    N = c;
+    const int64_t nyz = (int64_t)ny * nz;
+    // numbering: pores per x-plane, their prefix, then each plane's pores in order
+    std::vector<int64_t> first((size_t)nx + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        int64_t c = 0;
+        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) c += solid[v] == 0;
+        first[(size_t)i + 1] = c;
+    }
+    for (int i = 0; i < nx; ++i) first[(size_t)i + 1] += first[(size_t)i];
+    N = first[(size_t)nx];
     P = padded_stride(N);
     if (N == 0) return false;
-    out.assign((size_t)(18 * P), SRV_BB);
+    std::unique_ptr<int32_t[]> idx(new int32_t[(size_t)(nx * nyz)]);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i) {
+        int64_t c = first[(size_t)i];
+        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) idx[(size_t)v] = solid[v] == 0 ? (int32_t)c++ : -1;
+    }
+    // every entry is written below (a delta, or SRV_BB for a solid site and for the padding), so
+    // the rows are allocated uninitialised and first touched by the threads that fill them
+    out = std::shared_ptr<int16_t[]>(new int16_t[(size_t)(18 * P)]);
+    int16_t *o = out.get();
     const int64_t NN = N;
     int bad = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
-                const int32_t n = idx[(size_t)(((int64_t)i * ny + j) * nz + k)];
+                const int32_t n = idx[(size_t)(i * nyz + (int64_t)j * nz + k)];
                 if (n < 0) continue;
                 for (int q = 1; q < Q; ++q) {
                     int iu = i - EX[q], ju = j - EY[q], ku = k - EZ[q];
                     iu = iu < 0 ? iu + nx : (iu >= nx ? iu - nx : iu);
                     ju = ju < 0 ? ju + ny : (ju >= ny ? ju - ny : ju);
                     ku = ku < 0 ? ku + nz : (ku >= nz ? ku - nz : ku);
-                    const int32_t up = idx[(size_t)(((int64_t)iu * ny + ju) * nz + ku)];
-                    if (up < 0) continue;   // bounce back: SRV_BB stays
-                    const int64_t d = (int64_t)up - n;
-                    int64_t best = d;
-                    if (std::llabs(d - NN) < std::llabs(best)) best = d - NN;
-                    if (std::llabs(d + NN) < std::llabs(best)) best = d + NN;
-                    if (best < -32767 || best > 32767) {
-                        bad = 1;
-                        continue;
+                    const int32_t up = idx[(size_t)(iu * nyz + (int64_t)ju * nz + ku)];
+                    int16_t w = SRV_BB;   // bounce back
+                    if (up >= 0) {
+                        const int64_t d = (int64_t)up - n;
+                        int64_t best = d;
+                        if (std::llabs(d - NN) < std::llabs(best)) best = d - NN;
+                        if (std::llabs(d + NN) < std::llabs(best)) best = d + NN;
+                        if (best < -32767 || best > 32767)
+                            bad = 1;
+                        else
+                            w = (int16_t)best;
                     }
-                    out[(size_t)((q - 1) * P + n)] = (int16_t)best;
+                    o[(size_t)((q - 1) * P + n)] = w;
                 }
             }
+    for (int q = 0; q < 18; ++q)
+        for (int64_t n = N; n < P; ++n) o[(size_t)(q * P + n)] = SRV_BB;
     return bad == 0;
 }
 
 // Whether the sample's pores wrap round the box along axis, from its deltas (connectivity.h).
-bool deltas_wrap_along(const std::vector<int16_t> &nbr, int64_t N, int64_t P, int axis)
+bool deltas_wrap_along(const int16_t *nbr, int64_t N, int64_t P, int axis)
 {
-    return lattice_loop_wraps(N, axis, [&nbr, N, P](int64_t n, int q) -> int64_t {
+    return lattice_loop_wraps(N, axis, [nbr, N, P](int64_t n, int q) -> int64_t {
         const int16_t d = nbr[(size_t)((q - 1) * P + n)];
         if (d == SRV_BB) return -1;
         int64_t m = n + d;
@@ -514,6 +597,8 @@ private:
     std::deque<Prepared> pending_;
     std::deque<PrepJob> prep_q_;   // submitted, geometry not built yet
     int prep_busy_ = 0;
+    int nprep_ = 1;          // preparation workers
+    int prep_threads_ = 2;   // host threads the preparation jobs share
     std::vector<std::thread> prep_thr_;
     std::condition_variable prep_cv_;
     std::vector<Active> active_;   // in tile order
@@ -544,7 +629,10 @@ bool Server::make_fallback(int64_t min_nodes)
     lbm_srv_params fp = prm_;
     fp.precision = LBM_FP64;
     fp.escalate_k = 0.0;
-    fp.block = blk_;
+    // small tiles: the FP64 server runs a few samples beside the FP32 arena's launches, and
+    // 128-node workgroups find room between those launches' workgroups sooner (measured on the
+    // 32-request stream: the escalated sample's 11,700 FP64 steps then end before the FP32 work)
+    fp.block = 128;
     fp.capacity_nodes = std::max<int64_t>({int64_t(4) << 20, cap_ / 8, min_nodes});
     fp.mem_fraction = 0.0;
     auto f = std::unique_ptr<Server>(new (std::nothrow) Server());
@@ -562,8 +650,8 @@ bool Server::escalate_locked(Active &a, bool restart)
 {
     if (!fallback_ && !make_fallback(a.req.npad)) return false;
     if (!fallback_->fits(a.req.npad)) return false;
-    a.req.nbr = std::make_shared<std::vector<int16_t>>((size_t)18 * a.req.npad);
-    hipError_t e = hipMemcpy2DAsync(a.req.nbr->data(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
+    a.req.nbr = std::shared_ptr<int16_t[]>(new int16_t[(size_t)18 * a.req.npad]);
+    hipError_t e = hipMemcpy2DAsync(a.req.nbr.get(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
                                     (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                     hipMemcpyDeviceToHost, stream_);
     // otherwise the FP64 solve continues from the FP32 state rather than from rest: the
@@ -647,7 +735,8 @@ int Server::create(const lbm_srv_params &p)
         const double bytes = std::min(frac * (double)free_b, 64e9);
         cap = (int64_t)(bytes / (double)(2 * Q * st_ + 18 * sizeof(int16_t) + (aa_m_ ? (3 + 2 * aa_m_) * Q * st_ : 0)));
     }
-    cap = std::min<int64_t>(cap, (int64_t)0x7fffffff);
+    // the step kernel's 32-bit byte offsets reach (19 * CAP - 1) * sizeof(value)
+    cap = std::min<int64_t>(cap, (int64_t)(0xffffffffu / (Q * st_)) - 511);
     cap_ = ((cap + 511) / 512) * 512;
     if (cap_ < 512) return fail(LBM_ERR_ARG, "arena too small");
     for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * cap_ * st_));
@@ -681,8 +770,9 @@ int Server::create(const lbm_srv_params &p)
     // geometry preparation off the caller's thread: each worker numbers a sample and builds
     // its neighbour deltas while the device steps the admitted ones
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nprep = (int)std::min(8u, std::max(1u, hw / 2));
-    for (int i = 0; i < nprep; ++i) prep_thr_.emplace_back([this] { prep_worker(); });
+    nprep_ = (int)std::min(8u, std::max(1u, hw / 2));
+    prep_threads_ = (int)std::min(16u, hw);
+    for (int i = 0; i < nprep_; ++i) prep_thr_.emplace_back([this] { prep_worker(); });
     return LBM_OK;
 }
 
@@ -751,9 +841,9 @@ int Server::submit(const lbm_srv_request &r, int64_t *ticket)
 
 void Server::prep_worker()
 {
-    omp_set_num_threads(2);   // several workers at once: keep each one's parallel loops small
     for (;;) {
         PrepJob job;
+        int sharing = 1;
         {
             std::unique_lock<std::mutex> lk(mu_);
             prep_cv_.wait(lk, [this] { return stop_ || !prep_q_.empty(); });
@@ -761,11 +851,15 @@ void Server::prep_worker()
             job = std::move(prep_q_.front());
             prep_q_.pop_front();
             ++prep_busy_;
+            sharing = std::min(nprep_, prep_busy_ + (int)prep_q_.size());
         }
+        // the host threads are shared by the jobs in flight: a lone job (a request arriving on
+        // an idle server) gets up to 8 of them for its parallel loops
+        omp_set_num_threads(std::max(1, std::min(8, prep_threads_ / std::max(1, sharing))));
         Prepared pr;
         int64_t N = 0, P = 0;
-        auto nbr = std::make_shared<std::vector<int16_t>>();
-        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.solid.data(), N, P, *nbr);   // fits: checked in submit
+        std::shared_ptr<int16_t[]> nbr;
+        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.solid.data(), N, P, nbr);   // fits: checked in submit
         std::vector<int32_t>().swap(job.solid);
         pr.nbr = nbr;
         pr.ticket = job.ticket;
@@ -800,7 +894,7 @@ void Server::prep_worker()
         // below what FP32 deviations resolve.  With escalation on it goes to the FP64 server:
         // at once if it is still queued, at its next check if it has been admitted meanwhile.
         // The test runs after queueing so that it does not delay admission.
-        if (test_wrap && !deltas_wrap_along(*nbr, N, P, job.dir - 1)) {
+        if (test_wrap && !deltas_wrap_along(nbr.get(), N, P, job.dir - 1)) {
             std::lock_guard<std::mutex> lk(mu_);
             for (auto it = pending_.begin(); it != pending_.end(); ++it)
                 if (it->ticket == job.ticket) {
@@ -910,7 +1004,7 @@ bool Server::try_admit_locked()
         a.t_admit = clock();
         a.aa_next = aa_warm_;
         // sample-local neighbour rows go straight into the arena rows of the segment
-        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr->data(),
+        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.get(),
                                         (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                         hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess && !a.req.carried.empty()) {
@@ -995,7 +1089,7 @@ hipError_t Server::launch_steps(int nsteps)
 {
     for (int s = 0; s < nsteps; ++s) {
         if (prm_.precision == LBM_FP64)
-            srv_step_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
+            srv_step_kernel_wide<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
                 d_tiles_, d_slots_, (const double *)pops_[cur_], (double *)pops_[cur_ ^ 1], d_nbr_, cap_, global_step_);
         else
             srv_step_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
@@ -1282,7 +1376,7 @@ int Server::autotune()
                 solid[((size_t)i * L + j) * L + k] = v > -0.05 ? 1 : 0;
             }
     int64_t N = 0, P = 0;
-    std::vector<int16_t> d16;
+    std::shared_ptr<int16_t[]> d16;
     if (!build_deltas(L, L, L, solid.data(), N, P, d16) || P > cap_) {
         blk_ = 256;
         return LBM_OK;
@@ -1295,7 +1389,7 @@ int Server::autotune()
     a.req.volume = (int64_t)L * L * L;
     a.base = 0;
     a.first_step = 0;
-    LBM_HIP_TRY(hipMemcpy2DAsync(d_nbr_, (size_t)cap_ * sizeof(int16_t), d16.data(), (size_t)P * sizeof(int16_t),
+    LBM_HIP_TRY(hipMemcpy2DAsync(d_nbr_, (size_t)cap_ * sizeof(int16_t), d16.get(), (size_t)P * sizeof(int16_t),
                                  (size_t)P * sizeof(int16_t), 18, hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
     active_.push_back(std::move(a));

@@ -179,8 +179,9 @@ def test_escalated_sample_continues_in_fp64(pkg):
 def test_sample_without_throughflow_goes_to_fp64_at_admission(pkg):
     """With escalation on, a sample whose pores do not wrap round the box along the force
     direction (no throughflow: its k is the lattice's spurious-current residue) is sent to the
-    FP64 server when it is prepared.  Its result is that server's, bit for bit and step for
-    step, with no FP32 steps before it; a sample that wraps stays in the FP32 arena."""
+    FP64 server (128-node tiles) when it is prepared.  Its result is that of an FP64 server with
+    the same tiles, bit for bit and step for step, with no FP32 steps before it; a sample that
+    wraps stays in the FP32 arena."""
     _log("[gpu] serve: no-throughflow sample routed to fp64")
     geo = pkg.geometry
     open_s = geo.grf_sample(24, 3.0, 0.4, seed=101)
@@ -192,7 +193,8 @@ def test_sample_without_throughflow_goes_to_fp64_at_admission(pkg):
     with pkg.PermeabilityServer(escalate_k=1e-9, **kw) as srv:
         got = _run(srv, reqs)
         st = srv.stats()
-    with pkg.PermeabilityServer(precision="fp64", **kw) as srv:
+    # the internal FP64 server cuts samples into 128-node tiles (tile order fixes the flux sums)
+    with pkg.PermeabilityServer(precision="fp64", **{**kw, "block": 128}) as srv:
         ref = _run(srv, reqs[:1])[0]
     _log(f"[gpu] routed k {got[0]['k'][0]:.3e} after {got[0]['steps']} steps; open sample k {got[1]['k'][0]:.3e}")
     assert st["done"] == 2 and st["pending"] == 0 and st["active"] == 0
