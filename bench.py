@@ -138,8 +138,9 @@ def load_serving_truth(seed, nreq):
     return np.array([x["k"] for x in d["samples"][:nreq]])
 
 
-def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000):
-    """Server vs sequential plain solves on the same seeded request stream."""
+def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000, quick=False):
+    """Server vs sequential plain solves on the same seeded request stream (quick: the
+    accelerated server alone, e.g. under a profiler)."""
     geo = pkg.geometry
     rng = np.random.default_rng(seed)
     edges = rng.integers(48, 161, nreq)
@@ -250,6 +251,8 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000):
                 "fp64_samples": int(sum(1 for t in truth if abs(t) < 1e-3))}, span
 
     served, span_srv = serve(True)
+    if quick:
+        return dict(requests=nreq, arrival_rate_per_s=rate, seed=seed, **served)
     batched, span_bat = serve(False)
     seq_ref, span_ref = sequential(1e-6)
     seq_acc, span_acc = sequential(1e-7)
@@ -520,6 +523,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-quick", action="store_true", help="with --serve-only: the accelerated server leg alone")
     ap.add_argument("--workload", default="sp", choices=["sp", "cg", "solute"])
     ap.add_argument("--faces", action="store_true",
                     help="single-phase workload driven by pressure faces on x (Sub_BC 3) instead of a body force")
@@ -536,7 +540,7 @@ def main():
 
     pkg = load_package()
     if args.serve_only:
-        print(json.dumps({"serving": serving_bench(pkg, args.serve_requests, args.serve_rate)}))
+        print(json.dumps({"serving": serving_bench(pkg, args.serve_requests, args.serve_rate, quick=args.serve_quick)}))
         return
     if args.accuracy and args.workload == "sp":
         print(json.dumps({"accuracy": accuracy_study(pkg, args.edge)}))
