@@ -233,7 +233,8 @@ int lbm_st_diagnostics(lbm_handle *h, lbm_st_diag *out);
  * when |k_n - k_(n-interval)| <= tol |k_n|.  A sample's result does not depend on which other
  * samples share its launches.  Calls are thread-safe; the loop runs on its own host thread. */
 typedef struct lbm_srv_params {
-    int64_t capacity_nodes;   /* arena size in fluid nodes; 0 = mem_fraction of free device memory */
+    int64_t capacity_nodes;   /* arena size in fluid nodes; 0 = mem_fraction of free device memory (at most
+                                 64 GB); capped at 2^32 / (19 * value size) nodes, 56.5 M FP32, 28.2 M FP64 */
     double mem_fraction;      /* used when capacity_nodes == 0 (default 0.5) */
     int32_t check_interval;   /* steps between permeability checks (freRe), default 100 */
     double tol;               /* relative permeability change that retires a sample, default 1e-6 */
