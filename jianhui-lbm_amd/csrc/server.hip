@@ -424,7 +424,7 @@ struct PrepJob {
     int64_t ticket = 0;
     double t_submit = 0;
     int nx = 0, ny = 0, nz = 0;
-    std::vector<int32_t> solid;
+    std::vector<uint8_t> pore;   // 1 = pore voxel, [i][j][k]
     double nu = 0, g[3] = {0, 0, 0};
     int dir = 1;
 };
@@ -432,18 +432,23 @@ struct PrepJob {
 // Largest number of pore voxels in any cyclic run of L consecutive voxels (i-major order).  A
 // pull link from voxel v reaches back at most ny*nz + nz voxels (the xy diagonals), and its
 // node-index distance, taken modulo the node count, is the number of pores in between.
-int64_t max_window_pores(const int32_t *solid, int64_t V, int64_t L)
+int64_t max_window_pores(const uint8_t *pore, int64_t V, int64_t L)
 {
+    int64_t c = 0;
     if (L >= V) {
-        int64_t c = 0;
-        for (int64_t v = 0; v < V; ++v) c += solid[v] == 0;
+        for (int64_t v = 0; v < V; ++v) c += pore[v];
         return c;
     }
-    int64_t c = 0;
-    for (int64_t v = V - L; v < V; ++v) c += solid[v] == 0;
+    // the run ending at v starts at v - L + 1 (cyclically): the voxel leaving it is v - L + V
+    // while v < L, and v - L after that
+    for (int64_t v = V - L; v < V; ++v) c += pore[v];
     int64_t best = c;
-    for (int64_t v = 0; v < V; ++v) {
-        c += (solid[v] == 0) - (solid[(v - L + V) % V] == 0);
+    for (int64_t v = 0; v < L; ++v) {
+        c += (int64_t)pore[v] - pore[v - L + V];
+        best = std::max(best, c);
+    }
+    for (int64_t v = L; v < V; ++v) {
+        c += (int64_t)pore[v] - pore[v - L];
         best = std::max(best, c);
     }
     return best;
@@ -454,7 +459,7 @@ int64_t max_window_pores(const int32_t *solid, int64_t V, int64_t L)
 // shortest of up - n, up - n - N, up - n + N (the device adds it to n and reduces modulo N).
 // False if the box has no pores or a link does not fit, which needs an x-plane span of more
 // than 32767 fluid nodes (refused in submit).
-bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int64_t &P, std::shared_ptr<int16_t[]> &out)
+bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64_t &P, std::shared_ptr<int16_t[]> &out)
 {
     const int64_t nyz = (int64_t)ny * nz;
     // numbering: pores per x-plane, their prefix, then each plane's pores in order
@@ -462,7 +467,7 @@ bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int6
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i) {
         int64_t c = 0;
-        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) c += solid[v] == 0;
+        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) c += pore[v];
         first[(size_t)i + 1] = c;
     }
     for (int i = 0; i < nx; ++i) first[(size_t)i + 1] += first[(size_t)i];
@@ -473,7 +478,7 @@ bool build_deltas(int nx, int ny, int nz, const int32_t *solid, int64_t &N, int6
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nx; ++i) {
         int64_t c = first[(size_t)i];
-        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) idx[(size_t)v] = solid[v] == 0 ? (int32_t)c++ : -1;
+        for (int64_t v = i * nyz; v < (i + 1) * nyz; ++v) idx[(size_t)v] = pore[v] ? (int32_t)c++ : -1;
     }
     // every entry is written below (a delta, or SRV_BB for a solid site and for the padding), so
     // the rows are allocated uninitialised and first touched by the threads that fill them
@@ -812,19 +817,24 @@ int Server::submit(const lbm_srv_request &r, int64_t *ticket)
     if (r.g[r.perm_dir - 1] == 0.0) return fail(LBM_ERR_ARG, "body force along perm_dir is zero");
     const int64_t V = (int64_t)r.nx * r.ny * r.nz;
     if (V >= ((int64_t)1 << 31)) return fail(LBM_ERR_ARG, "box has 2^31 or more voxels");
-    // the checks that decide admissibility are O(voxels) and done here; the numbering and the
-    // neighbour deltas are built by a preparation worker
+    // the checks that decide admissibility are O(voxels) and done here, on a byte copy of the
+    // pore flags made in the same pass as the pore count; the numbering and the neighbour deltas
+    // are built by a preparation worker
+    PrepJob job;
+    job.pore.resize((size_t)V);
     int64_t N = 0;
-    for (int64_t v = 0; v < V; ++v) N += r.solid_ijk[v] == 0;
+    for (int64_t v = 0; v < V; ++v) {
+        const uint8_t p = r.solid_ijk[v] == 0;
+        job.pore[(size_t)v] = p;
+        N += p;
+    }
     if (N == 0) return fail(LBM_ERR_ARG, "geometry has no pore voxels");
     if (padded_stride(N) > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
-    if (max_window_pores(r.solid_ijk, V, (int64_t)r.ny * r.nz + r.nz) > 32767)
+    if (max_window_pores(job.pore.data(), V, (int64_t)r.ny * r.nz + r.nz) > 32767)
         return fail(LBM_ERR_ARG, "sample has more than 32767 fluid nodes within one x-plane span (solve it with lbm_sp_*)");
-    PrepJob job;
     job.nx = r.nx;
     job.ny = r.ny;
     job.nz = r.nz;
-    job.solid.assign(r.solid_ijk, r.solid_ijk + V);
     job.nu = r.nu;
     for (int c = 0; c < 3; ++c) job.g[c] = r.g[c];
     job.dir = r.perm_dir;
@@ -859,8 +869,8 @@ void Server::prep_worker()
         Prepared pr;
         int64_t N = 0, P = 0;
         std::shared_ptr<int16_t[]> nbr;
-        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.solid.data(), N, P, nbr);   // fits: checked in submit
-        std::vector<int32_t>().swap(job.solid);
+        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.pore.data(), N, P, nbr);   // fits: checked in submit
+        std::vector<uint8_t>().swap(job.pore);
         pr.nbr = nbr;
         pr.ticket = job.ticket;
         pr.t_submit = job.t_submit;
@@ -1367,17 +1377,17 @@ int Server::autotune()
 {
     // a 96^3 random porous sample (hashed blobs, porosity ~0.35)
     const int L = 96;
-    std::vector<int32_t> solid((size_t)L * L * L);
+    std::vector<uint8_t> pore((size_t)L * L * L);
     for (int i = 0; i < L; ++i)
         for (int j = 0; j < L; ++j)
             for (int k = 0; k < L; ++k) {
                 const double v = std::sin(0.37 * i + 1.3) * std::sin(0.29 * j + 0.7) * std::sin(0.41 * k + 2.1) +
                                  0.5 * std::sin(0.13 * (i + 2 * j) + 0.4) * std::cos(0.17 * (k - j));
-                solid[((size_t)i * L + j) * L + k] = v > -0.05 ? 1 : 0;
+                pore[((size_t)i * L + j) * L + k] = v > -0.05 ? 0 : 1;
             }
     int64_t N = 0, P = 0;
     std::shared_ptr<int16_t[]> d16;
-    if (!build_deltas(L, L, L, solid.data(), N, P, d16) || P > cap_) {
+    if (!build_deltas(L, L, L, pore.data(), N, P, d16) || P > cap_) {
         blk_ = 256;
         return LBM_OK;
     }
