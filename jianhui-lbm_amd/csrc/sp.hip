@@ -127,7 +127,7 @@ template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false, bool FU
 __global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = stream_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = (FUSED ? rr_tile(blockIdx.x) : stream_tile(blockIdx.x, gridDim.x)) * BLOCK + threadIdx.x;
     if (n >= a.N) return;
     C h[Q], m[Q];
     Macro<C> mc;

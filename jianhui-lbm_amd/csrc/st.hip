@@ -72,7 +72,7 @@ __device__ __forceinline__ void st_gather(const StArgs<C> &a, int64_t n, C (&g)[
 template <typename C, int MODE>
 __global__ void __launch_bounds__(BLOCK) st_step_kernel(StArgs<C> a)
 {
-    const int64_t n = stream_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = rr_tile(blockIdx.x) * BLOCK + threadIdx.x;
     if (n >= a.N) return;
     const C ux = a.u[n], uy = a.u[a.P + n], uz = a.u[2 * a.P + n];
     C g[Q7], c;
