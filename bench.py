@@ -275,7 +275,56 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000, quick=False):
     return out
 
 
-def cg_bench(pkg, steps, warmup, precision, edge=192):
+def cg_reference_cpu_baseline(solid, psi, nu_l, nu_g, cos_theta, cap_a, g, budget_s):
+    """The reference colour-gradient program (oracle/_ref/cg_ref, compiled from its own source
+    by `make -C oracle ref`) under mpirun on the host's cores, on the same geometry and phase
+    field (read from phase.dat, ini_Sat < 0), diagnostics off: the time per step is the
+    difference of two runs of different length, so start-up and file reading drop out.  None if
+    the binary or mpirun is not there."""
+    import subprocess
+    import tempfile
+    binary = os.path.join(ROOT, "oracle", "_ref", "cg_ref")
+    mpirun = "/opt/conda/bin/mpirun"
+    if not (os.path.exists(binary) and os.path.exists(mpirun)):
+        return None
+    pkg = load_package()
+    ranks = max(2, min(16, os.cpu_count() or 2))
+    env = dict(os.environ, PATH=os.path.dirname(mpirun) + ":" + os.environ.get("PATH", ""))
+
+    def run(n_max, wd):
+        inp = pkg.inputs.CGInput(filename="geometry.bin", filenamepsi="phase.dat", n_xyz=tuple(solid.shape),
+                                 n_max=n_max, reso=1.0, g=tuple(g), niu_l=nu_l, niu_g=nu_g, cos_theta=cos_theta,
+                                 CapA=cap_a, Permeability=1.0, PerDir=1, freRe=1 << 30, outputfile="./",
+                                 ini_Sat=-1.0, fre_backup=-1, decbin=1)
+        pkg.inputs.write_cg_input(inp, os.path.join(wd, "INPUT.dat"))
+        t0 = time.perf_counter()
+        proc = subprocess.run([mpirun, "-np", str(ranks), binary, "INPUT.dat"], cwd=wd, env=env, capture_output=True,
+                              text=True, timeout=900)
+        if proc.returncode != 0:
+            raise RuntimeError("reference run failed: " + proc.stdout[-500:] + proc.stderr[-500:])
+        return time.perf_counter() - t0
+
+    try:
+        with tempfile.TemporaryDirectory() as wd:
+            pkg.geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+            pkg.geometry.write_phase_ascii(os.path.join(wd, "phase.dat"), psi)
+            t_short = run(1, wd)                       # 2 steps (n = 0, 1)
+            t_probe = run(5, wd)                       # 6 steps
+            per = max((t_probe - t_short) / 4.0, 1e-3)
+            extra = max(2, min(2000, int(budget_s / per)))
+            t_long = run(1 + extra, wd)
+    except (RuntimeError, subprocess.TimeoutExpired, OSError) as e:
+        print(f"[bench] reference CPU run unavailable: {e}", file=sys.stderr, flush=True)
+        return None
+    per = (t_long - t_short) / extra
+    count = int(np.count_nonzero(solid == 0))
+    return {"value": round(count / per / 1e6, 3), "unit": "MLUPS", "cores": ranks, "kind": "reference",
+            "sample": f"the reference program (Multi_Component_CG_MPI_SPARSE_Software, FP64) under mpirun -np {ranks} "
+                      f"on the same geometry and phase field: ({t_long:.1f} s for {2 + extra} steps - {t_short:.1f} s "
+                      f"for 2 steps) / {extra}"}
+
+
+def cg_bench(pkg, steps, warmup, precision, edge=192, cpu_s=15.0, no_cpu=False):
     """Colour-gradient workload: 192^3 sphere pack (r = 6, porosity 0.45), S_w = 0.5 by voxel
     (seeded), nu_l = nu_g = 1/6, CapA = 0.01, cos theta = 0, g = 1e-5."""
     geo = pkg.geometry
@@ -302,8 +351,10 @@ def cg_bench(pkg, steps, warmup, precision, edge=192):
                         "traffic": read_traffic(f"cg_step_{precision}_{edge}cube"), "bytes_per_node": b_alg,
                         "kernels": "cg_collide_kernel + cg_colour_kernel per step",
                         "avg_step_us": round(ms * 1e3 / steps, 2)},
-           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"]}
+           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"], "cpu_baseline": None}
     s.close()
+    if not no_cpu:
+        out["cpu_baseline"] = cg_reference_cpu_baseline(solid, psi, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), cpu_s)
     return out
 
 
@@ -556,7 +607,8 @@ def main():
         print(json.dumps({"cg_accuracy": cg_accuracy(pkg)}))
         return
     if args.workload == "cg":
-        print(json.dumps(cg_bench(pkg, args.steps, args.warmup, args.precision if args.precision != "fp16" else "fp32")))
+        print(json.dumps(cg_bench(pkg, args.steps, args.warmup, args.precision if args.precision != "fp16" else "fp32",
+                                  cpu_s=args.cpu_seconds, no_cpu=args.no_cpu)))
         return
     edge = args.edge
     solid = pkg.geometry.sphere_pack((edge, edge, edge), 8.0, 0.35, seed=2024)
