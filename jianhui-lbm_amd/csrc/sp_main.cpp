@@ -8,9 +8,10 @@
 // Velocity_ave_max.txt, General_Results.txt, and the checkpoint files of Backup (SP:4507).
 // Pressure and velocity faces (INPUT lines 7-12, Sub_BC) run on the device
 // (lbm_sp_set_boundaries); the library refuses the velocity faces of Sub_BC 2 and 3.  The
-// power-law collision (nnf_module = 1) runs on the device too (lbm_sp_set_power_law).  Features
-// outside the hot path (VTK output, lattice velocity scaling, group/local permeability) are
-// reported and refused or skipped, never silently approximated.
+// power-law collision (nnf_module = 1) runs on the device too (lbm_sp_set_power_law), and so do
+// the field writers and the local permeability (loc_perm = 1).  Features outside the hot path
+// (lattice velocity scaling, group permeability) are reported and refused or skipped, never
+// silently approximated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -76,8 +77,8 @@ int main(int argc, char **argv)
     if (in.nnf_module != 0 && in.nnf_module != 1) return die("nnf_module must be 0 or 1");
     if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
         return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
-    if (in.gperm > 0 || in.loc_perm == 1)
-        std::cerr << "lbm_sp: note: group and local permeability outputs are skipped\n";
+    if (in.gperm > 0) std::cerr << "lbm_sp: note: group permeability output is skipped\n";
+    if (in.loc_perm == 1 && in.nx < 2) return die("local permeability (loc_perm = 1) needs at least two x-planes");
     const int nx = in.nx, ny = in.ny, nz = in.nz;
     std::vector<int32_t> solid;
     if (!lbmhost::read_geometry(in.filename, nx, ny, nz, in.decbin, solid, err)) return die(err);
@@ -200,6 +201,7 @@ int main(int argc, char **argv)
 
     const double rr = in.reso * in.reso * 1000;   // printed permeability unit (SP:742)
     double Permia[3] = {0, 0, 0}, error_perm = 0, u_max = 0, u_ave = 0, u_ave2 = -10000000, error = 0;
+    double Permia_LOCAL[3] = {0, 0, 0};   // stays 0 unless loc_perm = 1, as in the reference
     double Re = 0;
     int U_max_ref = 0;
     const int dir = (in.PerDir >= 1 && in.PerDir <= 3) ? in.PerDir - 1 : 0;
@@ -224,7 +226,8 @@ int main(int argc, char **argv)
             fin << "The" << n - in.freRe << "th computation result:" << std::endl;
             fin << "The permiability is: " << Permia[0] * rr << ", " << Permia[1] * rr << ", " << Permia[2] * rr << std::endl;
             fin << "The relative error of permiability computing is: " << error_perm << std::endl;
-            fin << "The LOCAL permiability is: " << 0 << ", " << 0 << ", " << 0 << std::endl;
+            fin << "The LOCAL permiability is: " << Permia_LOCAL[0] * rr << ", " << Permia_LOCAL[1] * rr << ", "
+                << Permia_LOCAL[2] * rr << std::endl;
             fin << "The Maximum velocity is: " << std::setprecision(6) << u_max << "   Re=" << Re
                 << "     Courant Number=" << u_max << std::endl;
             fin << "The max relative error of velocity is: " << std::setiosflags(std::ios::scientific) << error << std::endl;
@@ -248,6 +251,17 @@ int main(int argc, char **argv)
         for (int a = 0; a < 3; ++a) Perm[a] = dbox.Q[a] / vbox * in.in_vis / (in.g[a] + dp);
         error_perm = (Perm[dir] - Permia[dir]) / Permia[dir];
         for (int a = 0; a < 3; ++a) Permia[a] = Perm[a];
+        if (in.loc_perm == 1) {
+            // Comput_Perm_LOCAL (SP:4831-4883): the flux through the x-plane the last rank holds
+            // as Solid[nx_l - 2], i.e. x = NX - 1, the plane before the last, over the per_*
+            // cross-section of each direction
+            const int32_t plane[6] = {nx - 2, nx - 2, 0, ny - 1, 0, nz - 1};
+            lbm_sp_diag dl{};
+            if (check(lbm_sp_diagnostics(h, plane, &dl), h, "lbm_sp_diagnostics")) return 1;
+            const double ex = box[1] - box[0] + 1, ey = box[3] - box[2] + 1, ez = box[5] - box[4] + 1;
+            const double area[3] = {ey * ez, ex * ez, ex * ey};
+            for (int a = 0; a < 3; ++a) Permia_LOCAL[a] = dl.Q[a] / area[a] * in.in_vis / (in.g[a] + dp);
+        }
         {
             const double ax = dbox.Q[0] / (vbox * porosity), ay = dbox.Q[1] / (vbox * porosity), az = dbox.Q[2] / (vbox * porosity);
             u_ave = std::sqrt(ax * ax + ay * ay + az * az);
@@ -261,7 +275,8 @@ int main(int argc, char **argv)
             std::ofstream fin(fResults, std::ios::app);
             fin << "The" << n << "th computation result:" << std::endl;
             fin << "The permiability is: " << Permia[0] * rr << ", " << Permia[1] * rr << ", " << Permia[2] * rr << std::endl;
-            fin << "The LOCAL permiability is: " << 0 << ", " << 0 << ", " << 0 << std::endl;
+            fin << "The LOCAL permiability is: " << Permia_LOCAL[0] * rr << ", " << Permia_LOCAL[1] * rr << ", "
+                << Permia_LOCAL[2] * rr << std::endl;
             fin << "The relative error of permiability computing is: " << error_perm << std::endl;
             fin << "The Maximum velocity is: " << std::setprecision(6) << u_max << "   Re=" << Re
                 << "     Courant Number=" << u_max << std::endl;
@@ -271,7 +286,7 @@ int main(int argc, char **argv)
         }
         if (in.wr_per == 1) {
             std::ofstream f(fPerm, std::ios::app);
-            f << Permia[dir] * rr << " " << error_perm << " " << 0 << std::endl;
+            f << Permia[dir] * rr << " " << error_perm << " " << Permia_LOCAL[dir] * rr << std::endl;
         }
         std::ofstream(fBody, std::ios::app) << in.g[0] << std::endl;
         std::ofstream(fVel, std::ios::app) << u_ave << "  " << u_max << " " << error << std::endl;

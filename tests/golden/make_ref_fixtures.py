@@ -92,12 +92,12 @@ def _field_outputs(wd, freRe, n_max, out):
 
 
 def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None,
-            field_outputs=False):
+            field_outputs=False, loc_perm=False):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
                   steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc,
-                  power_law=power_law)
+                  power_law=power_law, loc_perm=loc_perm)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="sp_fix_")
         try:
@@ -112,6 +112,8 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
                 inp.nnf_mn = tuple(power_law)
             if field_outputs:                         # freVe, freDe, vel_sol = freRe
                 inp.freVe = inp.freDe = inp.vel_sol = freRe
+            if loc_perm:                              # Comput_Perm_LOCAL (SP:714-715, 4831-4883)
+                inp.loc_perm = 1
             if box is not None:
                 inp.par_per = (1, 1, 1)
                 inp.per_x = (box[1], box[0])
@@ -126,6 +128,8 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             out[f"perm_txt_{steps}"] = np.array(_text(wd, "Permeability_error_local_Perm.txt"))
             out[f"vel_txt_{steps}"] = np.array(_text(wd, "Velocity_ave_max.txt"))
             out[f"general_txt_{steps}"] = np.array(_text(wd, "General_Results.txt"))
+            if loc_perm:
+                out[f"results_txt_{steps}"] = np.array(_text(wd, "Results.txt"))
             if field_outputs:
                 _field_outputs(wd, freRe, steps - 1, out)
         finally:
@@ -341,6 +345,15 @@ def main(only=()):
         # the field writers: legacy VTK (velocity, density, geometry), the velocity profile and
         # the solute-velocity files at n = 0, 10, 20
         ("sp_c1_64cube", lambda n: c1_case(n)),
+        # the local permeability (loc_perm = 1): the flux through the plane x = NX - 1 over the
+        # per_* cross-section, with a partial box and body force, and with pressure faces (dp != 0)
+        ("sp_locperm_box_20x10x12", lambda n: sp_case(n, (20, 10, 12), 2.5, 0.5, 11, nu=1.0 / 6.0,
+                                                       g=(1e-5, 2e-6, -1e-6), u0=(0.0, 0.0, 0.0), steps_list=[21],
+                                                       box=[4, 15, 1, 8, 2, 10], loc_perm=True)),
+        ("sp_locperm_bc_16x12x10", lambda n: sp_case(n, (16, 12, 10), 3.0, 0.6, 37, nu=0.1, g=(2e-5, 0.0, 0.0),
+                                                      u0=(2e-3, 1e-3, -1e-3), steps_list=[21],
+                                                      bc={"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.01}},
+                                                      loc_perm=True)),
         ("sp_out_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
                                               u0=(1e-3, -2e-3, 5e-4), steps_list=[21], field_outputs=True)),
     ]

@@ -193,6 +193,35 @@ def test_lbm_sp_power_law_matches_reference(pkg, tmp_path, name):
     assert rel_err(f, d[f"f_{last}"]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["sp_locperm_box_20x10x12", "sp_locperm_bc_16x12x10"])
+def test_lbm_sp_local_permeability_matches_reference(pkg, tmp_path, name):
+    """loc_perm = 1 (Comput_Perm_LOCAL, SP:4831-4883): the flux through the plane x = NX - 1
+    over the per_* cross-section, printed in Results.txt (all three directions) and in the third
+    column of Permeability_error_local_Perm.txt, as the reference printed it (6 digits)."""
+    _log(f"[gpu] lbm_sp local permeability on {name}")
+    d0, p0 = load_fixture(name)
+    over = {"loc_perm": 1}
+    if p0.get("bc"):
+        over.update(_bc_over(p0["bc"]))
+    d, p = _sp_case(pkg, tmp_path, name, n_max=p0["steps"][-1] - 1, **over)
+    _run("lbm_sp", str(tmp_path))
+    last = p["steps"][-1]
+
+    def local_lines(text):
+        return [[float(x) for x in ln.split(":")[1].split(",")] for ln in text.splitlines() if "LOCAL" in ln]
+
+    got, ref = local_lines(open(tmp_path / "Results.txt").read()), local_lines(str(d[f"results_txt_{last}"]))
+    assert len(got) == len(ref) >= 2 and any(abs(v) > 0 for row in ref for v in row)
+    for g, r in zip(got, ref):
+        for a in range(3):
+            assert abs(g[a] - r[a]) <= 2e-5 * abs(r[a]) + 1e-9, (g, r)
+    got_p = _rows(open(tmp_path / "Permeability_error_local_Perm.txt").read())
+    ref_p = _rows(str(d[f"perm_txt_{last}"]))
+    assert len(got_p) == len(ref_p)
+    for g, r in zip(got_p, ref_p):
+        assert abs(g[2] - r[2]) <= 2e-5 * abs(r[2])
+
+
 def test_driver_refuses_out_of_path_features(pkg, tmp_path):
     _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, lattice=(1, 1.0, 2.0))
     proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
