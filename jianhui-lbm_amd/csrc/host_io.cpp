@@ -97,6 +97,12 @@ bool read_sp_input(const std::string &path, SpInput &in, std::string &err)
          get_int(L, 35, 0, in.vel_sol, err) && get_int(L, 37, 0, in.gperm, err) && get_int(L, 45, 0, in.decbin, err) &&
          get_int(L, 47, 0, in.nnf_module, err);
     if (ok && in.nnf_module == 1) ok = get_dbl(L, 48, 0, in.nnf_m, err) && get_dbl(L, 48, 1, in.nnf_n, err);
+    if (ok && in.gperm > 0) {   // group permeability grid (SP:298-303)
+        for (int a = 0; ok && a < 4; ++a)
+            ok = get_int(L, 38, a, in.n_gperm[a], err) && get_int(L, 39, a, in.size_gperm[a], err);
+        for (int gi = 0; ok && gi < 4; ++gi)
+            for (int a = 0; ok && a < 3; ++a) ok = get_int(L, 40 + gi, a, in.c_gperm[gi][a], err);
+    }
     return ok;
 }
 

@@ -28,6 +28,8 @@ struct SpInput {
     int per_x[2] = {0, 0}, per_y[2] = {0, 0}, per_z[2] = {0, 0};   // (p, n) as written
     int fre_backup = -1, mode_backup_ini = 0, vel_sol = -1;
     int gperm = 0;
+    int n_gperm[4] = {1, 1, 1, 1}, size_gperm[4] = {8, 8, 8, 8};   // lines 38-39
+    int c_gperm[4][3] = {{0}};                                      // lines 40-43: cube-grid origins
     int decbin = 0;
     int nnf_module = 0;
     double nnf_m = 0.0, nnf_n = 1.0;   // line 48, read when nnf_module = 1 (SP:307-308)

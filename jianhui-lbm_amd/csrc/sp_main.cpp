@@ -9,9 +9,8 @@
 // Pressure and velocity faces (INPUT lines 7-12, Sub_BC) run on the device
 // (lbm_sp_set_boundaries); the library refuses the velocity faces of Sub_BC 2 and 3.  The
 // power-law collision (nnf_module = 1) runs on the device too (lbm_sp_set_power_law), and so do
-// the field writers and the local permeability (loc_perm = 1).  Features outside the hot path
-// (lattice velocity scaling, group permeability) are reported and refused or skipped, never
-// silently approximated.
+// the field writers and the local and group permeabilities (loc_perm = 1, gperm > 0).  Lattice
+// velocity scaling, outside the hot path, is refused, never silently approximated.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -77,7 +76,6 @@ int main(int argc, char **argv)
     if (in.nnf_module != 0 && in.nnf_module != 1) return die("nnf_module must be 0 or 1");
     if ((int)in.lattice[0] == 1 && (in.lattice[1] != in.lattice[2]))
         return die("self-defined lattice velocity (lattice_v = 1) is not part of this build");
-    if (in.gperm > 0) std::cerr << "lbm_sp: note: group permeability output is skipped\n";
     if (in.loc_perm == 1 && in.nx < 2) return die("local permeability (loc_perm = 1) needs at least two x-planes");
     const int nx = in.nx, ny = in.ny, nz = in.nz;
     std::vector<int32_t> solid;
@@ -261,6 +259,27 @@ int main(int argc, char **argv)
             const double ex = box[1] - box[0] + 1, ey = box[3] - box[2] + 1, ez = box[5] - box[4] + 1;
             const double area[3] = {ey * ez, ex * ez, ex * ey};
             for (int a = 0; a < 3; ++a) Permia_LOCAL[a] = dl.Q[a] / area[a] * in.in_vis / (in.g[a] + dp);
+        }
+        if (in.gperm > 0 && n % in.gperm == 0) {
+            // Comput_Grop_Perm (SP:4886-5090): for each of the four cube grids, the permeability
+            // along PerDir and the porosity of every cube (origin c + i*size, edge size, x then y
+            // then z), one line per cube in Perm_<size>.output, rewritten at each call
+            for (int gi = 0; gi < 4; ++gi) {
+                const int L = in.size_gperm[gi], m = in.n_gperm[gi];
+                std::ofstream fo(out + "Perm_" + std::to_string(L) + ".output", std::ios::out);
+                const double vol = (double)L * L * L;
+                for (int ix = 0; ix < m; ++ix)
+                    for (int iy = 0; iy < m; ++iy)
+                        for (int iz = 0; iz < m; ++iz) {
+                            const int32_t b[6] = {in.c_gperm[gi][0] + ix * L, in.c_gperm[gi][0] + ix * L + L - 1,
+                                                  in.c_gperm[gi][1] + iy * L, in.c_gperm[gi][1] + iy * L + L - 1,
+                                                  in.c_gperm[gi][2] + iz * L, in.c_gperm[gi][2] + iz * L + L - 1};
+                            lbm_sp_diag dg{};
+                            if (check(lbm_sp_diagnostics(h, b, &dg), h, "lbm_sp_diagnostics")) return 1;
+                            const double k = dg.Q[dir] / vol * in.in_vis / (in.g[dir] + dp);
+                            fo << k * rr << " " << dg.count_in_box / vol << std::endl;
+                        }
+            }
         }
         {
             const double ax = dbox.Q[0] / (vbox * porosity), ay = dbox.Q[1] / (vbox * porosity), az = dbox.Q[2] / (vbox * porosity);

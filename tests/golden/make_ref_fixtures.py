@@ -92,12 +92,12 @@ def _field_outputs(wd, freRe, n_max, out):
 
 
 def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None,
-            field_outputs=False, loc_perm=False):
+            field_outputs=False, loc_perm=False, gperm=None):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
                   steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc,
-                  power_law=power_law, loc_perm=loc_perm)
+                  power_law=power_law, loc_perm=loc_perm, gperm=gperm)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="sp_fix_")
         try:
@@ -114,6 +114,11 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
                 inp.freVe = inp.freDe = inp.vel_sol = freRe
             if loc_perm:                              # Comput_Perm_LOCAL (SP:714-715, 4831-4883)
                 inp.loc_perm = 1
+            if gperm is not None:                     # Comput_Grop_Perm (SP:718-719, 4886-5090)
+                inp.gperm = gperm["every"]
+                inp.n_gperm = tuple(gperm["n"])
+                inp.size_gperm = tuple(gperm["size"])
+                inp.c_gperm1, inp.c_gperm2, inp.c_gperm3, inp.c_gperm4 = (tuple(c) for c in gperm["c"])
             if box is not None:
                 inp.par_per = (1, 1, 1)
                 inp.per_x = (box[1], box[0])
@@ -130,6 +135,9 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             out[f"general_txt_{steps}"] = np.array(_text(wd, "General_Results.txt"))
             if loc_perm:
                 out[f"results_txt_{steps}"] = np.array(_text(wd, "Results.txt"))
+            if gperm is not None:
+                for size in sorted(set(gperm["size"])):
+                    out[f"gperm_{size}_{steps}"] = np.array(_text(wd, f"Perm_{size}.output"))
             if field_outputs:
                 _field_outputs(wd, freRe, steps - 1, out)
         finally:
@@ -354,6 +362,11 @@ def main(only=()):
                                                       u0=(2e-3, 1e-3, -1e-3), steps_list=[21],
                                                       bc={"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.01}},
                                                       loc_perm=True)),
+        # the group permeability (gperm > 0): four cube grids, the last partly outside the box
+        ("sp_gperm_20x10x12", lambda n: sp_case(n, (20, 10, 12), 2.5, 0.5, 12, nu=1.0 / 6.0, g=(1e-5, 0.0, 0.0),
+                                                 u0=(0.0, 0.0, 0.0), steps_list=[21],
+                                                 gperm={"every": 10, "n": [2, 1, 3, 2], "size": [4, 6, 3, 5],
+                                                        "c": [[0, 0, 0], [5, 2, 3], [1, 1, 1], [12, 4, 6]]})),
         ("sp_out_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
                                               u0=(1e-3, -2e-3, 5e-4), steps_list=[21], field_outputs=True)),
     ]

@@ -222,6 +222,28 @@ def test_lbm_sp_local_permeability_matches_reference(pkg, tmp_path, name):
         assert abs(g[2] - r[2]) <= 2e-5 * abs(r[2])
 
 
+def test_lbm_sp_group_permeability_matches_reference(pkg, tmp_path):
+    """gperm > 0 (Comput_Grop_Perm, SP:4886-5090): for four cube grids (the last partly outside
+    the box), the permeability along PerDir and the porosity of each cube in Perm_<size>.output,
+    as the reference printed them (6 digits)."""
+    _log("[gpu] lbm_sp group permeability")
+    d0, p0 = load_fixture("sp_gperm_20x10x12")
+    gp = p0["gperm"]
+    over = {"gperm": gp["every"], "n_gperm": tuple(gp["n"]), "size_gperm": tuple(gp["size"])}
+    for i, c in enumerate(gp["c"]):
+        over[f"c_gperm{i + 1}"] = tuple(c)
+    d, p = _sp_case(pkg, tmp_path, "sp_gperm_20x10x12", n_max=p0["steps"][-1] - 1, **over)
+    _run("lbm_sp", str(tmp_path))
+    last = p["steps"][-1]
+    for size in sorted(set(gp["size"])):
+        got = _rows(open(tmp_path / f"Perm_{size}.output").read())
+        ref = _rows(str(d[f"gperm_{size}_{last}"]))
+        assert len(got) == len(ref) > 0
+        for g, r in zip(got, ref):
+            assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0]) + 1e-9, (size, g, r)
+            assert abs(g[1] - r[1]) <= 1e-6, (size, g, r)
+
+
 def test_driver_refuses_out_of_path_features(pkg, tmp_path):
     _sp_case(pkg, tmp_path, "sp_mrt_16x12x8", n_max=3, lattice=(1, 1.0, 2.0))
     proc = subprocess.run([os.path.join(PKG_DIR, "lbm_sp"), "INPUT.dat"], cwd=str(tmp_path), capture_output=True,
