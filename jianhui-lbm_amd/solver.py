@@ -197,6 +197,39 @@ class SinglePhase:
         avg = d["Q"] / (V * por)
         return k, rel, float(np.sqrt(avg @ avg))
 
+    def Comput_Perm_LOCAL(self, box: Optional[Sequence[int]] = None, dp: float = 0.0) -> np.ndarray:
+        """k[3] as Comput_Perm_LOCAL (SP:4831-4883): the flux through the x-plane before the last
+        (the last rank's Solid[nx_l - 2]) over the cross-section of the per_* box (the whole box
+        if None) in each direction, times nu / (g_d + dp)."""
+        nx, ny, nz = self.shape
+        if nx < 2:
+            raise ValueError("the local permeability needs at least two x-planes")
+        d = self.diagnostics((nx - 2, nx - 2, 0, ny - 1, 0, nz - 1))
+        b = (0, nx - 1, 0, ny - 1, 0, nz - 1) if box is None else box
+        ex, ey, ez = b[1] - b[0] + 1, b[3] - b[2] + 1, b[5] - b[4] + 1
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return d["Q"] / np.array([ey * ez, ex * ez, ex * ey], float) * self.nu / (np.asarray(self.g) + dp)
+
+    def Comput_Grop_Perm(self, n: Sequence[int], size: Sequence[int], origins: Sequence[Sequence[int]],
+                         PerDir: int = 1, dp: float = 0.0) -> list:
+        """Comput_Grop_Perm (SP:4886-5090): for each cube grid g (n[g]^3 cubes of edge size[g]
+        from origins[g], x then y then z), a list of (k along PerDir, porosity) per cube, with
+        k = Q / size^3 * nu / (g + dp) -- the lines of the reference's Perm_<size>.output."""
+        i = {1: 0, 2: 1, 3: 2}.get(PerDir, 0)
+        out = []
+        for m, L, c in zip(n, size, origins):
+            vol = float(L) ** 3
+            rows = []
+            for ix in range(m):
+                for iy in range(m):
+                    for iz in range(m):
+                        b = (c[0] + ix * L, c[0] + ix * L + L - 1, c[1] + iy * L, c[1] + iy * L + L - 1,
+                             c[2] + iz * L, c[2] + iz * L + L - 1)
+                        d = self.diagnostics(b)
+                        rows.append((d["Q"][i] / vol * self.nu / (self.g[i] + dp), d["count_in_box"] / vol))
+            out.append(rows)
+        return out
+
 
 class ColorGradient:
     """Sparse D3Q19 MRT colour-gradient two-component solver on one GPU (the reference's CG

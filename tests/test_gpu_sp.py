@@ -215,3 +215,30 @@ def test_fp64_full_size_reference_case(pkg):
     assert len(printed) == len(ks)
     for a, b in zip(ks, printed):
         assert abs(a - b) <= 2e-5 * abs(b)    # six printed digits
+
+
+def test_python_local_and_group_permeability_match_reference(pkg):
+    """SinglePhase.Comput_Perm_LOCAL and Comput_Grop_Perm (SP:4831-5090) after the fixtures'
+    21 steps against what the reference printed: Results.txt's LOCAL line and the
+    Perm_<size>.output files (6 digits)."""
+    d, p = load_fixture("sp_locperm_box_20x10x12")
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    s.init(1.0, p["u0"])
+    s.step(p["steps"][-1])
+    k = s.Comput_Perm_LOCAL(p["box"]) * 1000.0
+    line = [ln for ln in str(d["results_txt_21"]).splitlines() if "LOCAL" in ln][-1]
+    ref = [float(x) for x in line.split(":")[1].split(",")]
+    np.testing.assert_allclose(k, ref, rtol=2e-5)
+
+    d, p = load_fixture("sp_gperm_20x10x12")
+    gp = p["gperm"]
+    s = pkg.SinglePhase(d["solid"].astype(np.int32), p["nu"], p["g"], precision="fp64")
+    s.init(1.0, p["u0"])
+    s.step(p["steps"][-1])
+    groups = s.Comput_Grop_Perm(gp["n"], gp["size"], gp["c"])
+    last_of_size = {size: rows for size, rows in zip(gp["size"], groups)}   # a later grid rewrites the file
+    for size, rows in last_of_size.items():
+        ref = [[float(x) for x in ln.split()] for ln in str(d[f"gperm_{size}_21"]).strip().splitlines()]
+        assert len(rows) == len(ref)
+        for (kk, por), (rk, rp) in zip(rows, ref):
+            assert abs(kk * 1000.0 - rk) <= 2e-5 * abs(rk) + 1e-9 and abs(por - rp) <= 1e-6
