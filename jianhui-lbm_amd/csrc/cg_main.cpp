@@ -195,6 +195,21 @@ int main(int argc, char **argv)
             Per_l[a] = Pl[a];
             Per_g[a] = Pg[a];
         }
+        // Comput_Perm_LOCAL (CG:6214-6290, called at every check, CG:1009): the two phases'
+        // fluxes through the x-plane the last rank holds as Solid[nx_l - 2] (x = NX - 1), over
+        // the per_* cross-section of each direction
+        double Pl_loc[3] = {0, 0, 0}, Pg_loc[3] = {0, 0, 0};
+        if (nx >= 2) {
+            const int32_t plane[6] = {nx - 2, nx - 2, 0, ny - 1, 0, nz - 1};
+            lbm_cg_diag dl{};
+            if (check(lbm_cg_diagnostics(h, plane, &dl), h, "lbm_cg_diagnostics")) return 1;
+            const double ex = box[1] - box[0] + 1, ey = box[3] - box[2] + 1, ez = box[5] - box[4] + 1;
+            const double area[3] = {ey * ez, ex * ez, ex * ey};
+            for (int a = 0; a < 3; ++a) {
+                Pl_loc[a] = dl.Ql[a] / area[a] * in.niu_l / (gs[a] + 0.0);
+                Pg_loc[a] = dl.Qg[a] / area[a] * in.niu_g / (gs[a] + 0.0);
+            }
+        }
         // Comput_Saturation (CG:6604-6763)
         const double S_l_r = S_l;
         S_l = dbox.wet_count / (vbox * porosity);
@@ -214,10 +229,15 @@ int main(int argc, char **argv)
         }
         if (in.wr_per == 1) {
             std::ofstream f1(fRel1, std::ios::app);
-            f1 << Per_l[dir] * rr_unit << " " << S_l << " " << 0 << " " << 1 - S_l << " " << in.p_xn << " " << in.p_xp << " " << gs[dir]
-               << " " << 0 << std::endl;
+            // the pressures of the PerDir faces (CG:1092-1101); the last column and the
+            // component-2 slope are the least-squares module's, 0 with it off
+            const double pn = dir == 1 ? in.p_yn : (dir == 2 ? in.p_zn : in.p_xn);
+            const double pp = dir == 1 ? in.p_yp : (dir == 2 ? in.p_zp : in.p_xp);
+            f1 << Per_l[dir] * rr_unit << " " << S_l << " " << Pl_loc[dir] * rr_unit << " " << 1 - S_l << " " << pn << " "
+               << pp << " " << gs[dir] << " " << 0 << std::endl;
             std::ofstream f2(fRel2, std::ios::app);
-            f2 << Per_g[dir] * rr_unit << " " << 1 - S_l << " " << 0 << " " << S_l << " " << 0 << " " << 0 << " " << std::endl;
+            f2 << Per_g[dir] * rr_unit << " " << 1 - S_l << " " << Pg_loc[dir] * rr_unit << " " << S_l << " " << 0 << " " << 0
+               << " " << std::endl;
         }
         std::ofstream(fBody, std::ios::app) << S_l << " " << 1 - S_l << " " << std::fabs((S_l_r - S_l) / S_l) << " "
                                             << dbox.sd_l << " " << dbox.sd_g << std::endl;

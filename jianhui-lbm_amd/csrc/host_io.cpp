@@ -117,7 +117,8 @@ bool read_cg_input(const std::string &path, CgInput &in, std::string &err)
               get_int(L, 4, 1, in.ny, err) && get_int(L, 4, 2, in.nz, err) && get_int(L, 5, 0, in.n_max, err) &&
               get_dbl(L, 6, 0, in.reso, err) && get_int(L, 7, 0, in.in_BC, err) && get_int(L, 8, 0, in.in_psi_BC, err);
     for (int a = 0; ok && a < 3; ++a) ok = get_dbl(L, 9, a, in.g[a], err);
-    ok = ok && get_dbl(L, 10, 1, in.p_xp, err) && get_dbl(L, 10, 3, in.p_xn, err);
+    ok = ok && get_dbl(L, 10, 1, in.p_xp, err) && get_dbl(L, 10, 3, in.p_xn, err) && get_dbl(L, 11, 1, in.p_yp, err) &&
+         get_dbl(L, 11, 3, in.p_yn, err) && get_dbl(L, 12, 1, in.p_zp, err) && get_dbl(L, 12, 3, in.p_zn, err);
     ok = ok && get_dbl(L, 19, 0, in.niu_l, err) && get_dbl(L, 20, 0, in.niu_g, err) && get_dbl(L, 21, 0, in.cos_theta, err) &&
          get_dbl(L, 22, 0, in.CapA, err);
     for (int a = 0; ok && a < 3; ++a) ok = get_dbl(L, 23, a, in.ini_v[a], err);

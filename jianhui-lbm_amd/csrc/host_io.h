@@ -43,7 +43,8 @@ struct CgInput {
     double reso = 1.0;
     int in_BC = 0, in_psi_BC = 0;
     double g[3] = {0, 0, 0};
-    double p_xp = 1.0, p_xn = 1.0;   // pressure-boundary values, echoed in the rel-perm file
+    double p_xp = 1.0, p_xn = 1.0;   // pressure-boundary values (lines 10-12), echoed in the rel-perm file
+    double p_yp = 1.0, p_yn = 1.0, p_zp = 1.0, p_zn = 1.0;
     double niu_l = 0.1, niu_g = 0.1, cos_theta = 0, CapA = 0.01;
     double ini_v[3] = {0, 0, 0};
     double Permeability = 1.0;

@@ -131,10 +131,17 @@ def test_lbm_cg_outputs_match_reference(pkg, tmp_path, name):
         band = (ties if i == 0 else 2) / n + 2e-6
         assert abs(g[0] - r[0]) <= band
         assert abs(g[3] - r[3]) <= 2e-3 * abs(r[3]) + 0.5 and abs(g[4] - r[4]) <= 2e-3 * abs(r[4]) + 0.5
-    kg = _rows(open(tmp_path / "Relative_Permeability_Component1.txt").read())
-    kr = _rows(str(d[f"relperm1_txt_{last}"]))
-    for g, r in zip(kg, kr):
-        assert abs(g[0] - r[0]) <= 1e-4 * abs(r[0]) + 1e-6 * max(1.0, abs(r[0]))
+    # columns: k, saturation, the local k through the plane x = NX - 1 (Comput_Perm_LOCAL,
+    # CG:6214-6290), ..., the PerDir face pressures
+    for comp in (1, 2):
+        kg = _rows(open(tmp_path / f"Relative_Permeability_Component{comp}.txt").read())
+        kr = _rows(str(d[f"relperm{comp}_txt_{last}"]))
+        assert len(kg) == len(kr)
+        for g, r in zip(kg, kr):
+            for c in (0, 2):
+                assert abs(g[c] - r[c]) <= 1e-4 * abs(r[c]) + 1e-6 * max(1.0, abs(r[c])), (comp, c, g, r)
+            if comp == 1:
+                assert g[4:7] == r[4:7]
     f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
     psi = _read_ckpt(tmp_path / f"LBM_checkpoint_psi_{last - 1}.0.bin_input", 1)
     assert rel_err(f, d[f"f_{last}"]) < 1e-12
