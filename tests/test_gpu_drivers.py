@@ -66,13 +66,21 @@ def test_lbm_sp_outputs_match_reference(pkg, tmp_path, name):
         assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0])             # k, 6 significant digits
         if np.isfinite(r[1]) and abs(r[1]) > 1e-3:
             assert abs(g[1] - r[1]) <= 1e-4 * abs(r[1]) + 1e-6   # relative change of k
+        assert abs(g[2] - r[2]) <= 2e-5 * abs(r[2]) + 1e-12       # local k (0 with loc_perm off)
     vg = _rows(open(tmp_path / "Velocity_ave_max.txt").read())
     vr = _rows(str(d[f"vel_txt_{last}"]))
+    assert len(vg) == len(vr)
+    # u_ave, u_max.  The third column, Error's relative velocity change, is the maximum over MPI
+    # ranks of each rank's own sum (SP:3651-3685), so the reference's value depends on its rank
+    # count (these files come from 2 ranks; 1 rank aborts); the driver prints the one-rank value.
     for g, r in zip(vg, vr):
-        assert abs(g[0] - r[0]) <= 2e-5 * abs(r[0]) and abs(g[1] - r[1]) <= 2e-5 * abs(r[1])
+        for c in range(2):
+            assert abs(g[c] - r[c]) <= 2e-5 * abs(r[c]), (c, g, r)
     gen = open(tmp_path / "General_Results.txt").read().split()
-    assert gen[0] == "geometry.bin"
-    assert abs(float(gen[1]) - float(str(d[f"general_txt_{last}"]).split()[1])) <= 2e-5 * abs(float(gen[1]))
+    ref_gen = str(d[f"general_txt_{last}"]).split()
+    assert gen[0] == ref_gen[0] == "geometry.bin" and len(gen) == len(ref_gen)
+    for a, b in zip(gen[1:], ref_gen[1:]):                  # k along x, y, z (inf where g_d = 0)
+        assert a == b or abs(float(a) - float(b)) <= 2e-5 * abs(float(b)), (gen, ref_gen)
     # the final checkpoint (label n_max) is the reference's state after n_max + 1 steps
     f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{last - 1}.0.bin_input", 19)
     u = _read_ckpt(tmp_path / f"LBM_checkpoint_velocity_{last - 1}.0.bin_input", 3)
