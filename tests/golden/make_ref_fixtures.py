@@ -149,7 +149,7 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
 
 
 def cg_case(name, shape, radius, por, seed, psi_mode, psi_seed, nu_l, nu_g, cos_theta, cap_a, g,
-            steps_list, freRe=10, bodyforce_apply=0, rel_perm=(0.99, 0.92)):
+            steps_list, freRe=10, bodyforce_apply=0, rel_perm=(0.99, 0.92), per_dir=1):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     rng = np.random.default_rng(psi_seed)
     if psi_mode == "binary":
@@ -160,7 +160,7 @@ def cg_case(name, shape, radius, por, seed, psi_mode, psi_seed, nu_l, nu_g, cos_
     params = dict(shape=list(shape), nu_l=nu_l, nu_g=nu_g, cos_theta=cos_theta, cap_a=cap_a, g=list(g),
                   freRe=freRe, steps=list(steps_list), bodyforce_apply=bodyforce_apply,
                   rel_perm_psi=list(rel_perm), radius=radius, porosity_target=por, seed=seed,
-                  psi_mode=psi_mode, psi_seed=psi_seed)
+                  psi_mode=psi_mode, psi_seed=psi_seed, per_dir=per_dir)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="cg_fix_")
         try:
@@ -168,7 +168,7 @@ def cg_case(name, shape, radius, por, seed, psi_mode, psi_seed, nu_l, nu_g, cos_
             geometry.write_phase_ascii(os.path.join(wd, "phase.dat"), psi)
             inp = inputs.CGInput(filename="geometry.bin", filenamepsi="phase.dat", n_xyz=tuple(shape),
                                  n_max=steps - 1, reso=1.0, g=tuple(g), niu_l=nu_l, niu_g=nu_g,
-                                 cos_theta=cos_theta, CapA=cap_a, Permeability=1.0, wr_per=1, PerDir=1,
+                                 cos_theta=cos_theta, CapA=cap_a, Permeability=1.0, wr_per=1, PerDir=per_dir,
                                  freRe=freRe, outputfile="./", stab=(0, 0), ini_Sat=-1.0,
                                  rel_perm_psi=tuple(rel_perm), fre_backup=0, decbin=1,
                                  bodyforce_apply=bodyforce_apply)
@@ -321,6 +321,10 @@ def main(only=()):
         ("cg_mixed_gated_14x10x8", lambda n: cg_case(n, (14, 10, 8), 2.5, 0.55, 9, "uniform", 10, nu_l=1.0 / 6.0,
                                                       nu_g=1.0 / 6.0, cos_theta=0.0, cap_a=0.005,
                                                       g=(2e-5, 1e-5, 0.0), steps_list=[1, 21], bodyforce_apply=1)),
+        # PerDir = 2: the relative-permeability files report the y components and y-face pressures
+        ("cg_perdir2_14x12x8", lambda n: cg_case(n, (14, 12, 8), 2.5, 0.55, 13, "binary", 14, nu_l=0.1, nu_g=0.1,
+                                                  cos_theta=0.2, cap_a=0.01, g=(0.0, 2e-5, 0.0), steps_list=[1, 31],
+                                                  per_dir=2)),
         ("st6_16x12x8", lambda n: st_case(n, "sp_mrt_16x12x8", niu_s=0.02, u_scale=100.0, c_seed=21, n_max=20,
                                           labels=[5, 20])),
         # face boundaries (SP:2751-3620), one fixture per rule, u0 != 0 so that the x- face

@@ -111,7 +111,7 @@ def test_lbm_sp_fp32_precision_flag(pkg, tmp_path):
     assert rel_err(f, d["f_101"]) < 1e-6
 
 
-@pytest.mark.parametrize("name", ["cg_binary_16x12x8", "cg_mixed_gated_14x10x8"])
+@pytest.mark.parametrize("name", ["cg_binary_16x12x8", "cg_mixed_gated_14x10x8", "cg_perdir2_14x12x8"])
 def test_lbm_cg_outputs_match_reference(pkg, tmp_path, name):
     _log(f"[gpu] lbm_cg on {name}")
     d, p = load_fixture(name)
@@ -120,7 +120,8 @@ def test_lbm_cg_outputs_match_reference(pkg, tmp_path, name):
     pkg.geometry.write_phase_ascii(str(tmp_path / "phase.dat"), d["psi_field"])
     inp = pkg.inputs.CGInput(filename="geometry.bin", filenamepsi="phase.dat", n_xyz=tuple(p["shape"]),
                              n_max=last - 1, reso=1.0, g=tuple(p["g"]), niu_l=p["nu_l"], niu_g=p["nu_g"],
-                             cos_theta=p["cos_theta"], CapA=p["cap_a"], Permeability=1.0, wr_per=1, PerDir=1,
+                             cos_theta=p["cos_theta"], CapA=p["cap_a"], Permeability=1.0, wr_per=1,
+                             PerDir=p.get("per_dir", 1),
                              freRe=p["freRe"], outputfile="./", stab=(0, 0), ini_Sat=-1.0,
                              rel_perm_psi=tuple(p["rel_perm_psi"]), fre_backup=0, decbin=1,
                              bodyforce_apply=p["bodyforce_apply"])
