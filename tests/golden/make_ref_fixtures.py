@@ -92,18 +92,18 @@ def _field_outputs(wd, freRe, n_max, out):
 
 
 def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box=None, bc=None, power_law=None,
-            field_outputs=False, loc_perm=False, gperm=None):
+            field_outputs=False, loc_perm=False, gperm=None, per_dir=1):
     solid = geometry.sphere_pack(shape, radius, por, seed)
     out = {"solid": solid.astype(np.int8)}
     params = dict(shape=list(shape), nu=nu, g=list(g), u0=list(u0), freRe=freRe, box=box,
                   steps=list(steps_list), radius=radius, porosity_target=por, seed=seed, bc=bc,
-                  power_law=power_law, loc_perm=loc_perm, gperm=gperm)
+                  power_law=power_law, loc_perm=loc_perm, gperm=gperm, per_dir=per_dir)
     for steps in steps_list:
         wd = tempfile.mkdtemp(prefix="sp_fix_")
         try:
             geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
             inp = inputs.SPInput(filename="geometry.bin", n_xyz=tuple(shape), n_max=steps - 1, reso=1.0,
-                                 g=tuple(g), in_vis=nu, ini_v=tuple(u0), wr_per=1, PerDir=1, freRe=freRe,
+                                 g=tuple(g), in_vis=nu, ini_v=tuple(u0), wr_per=1, PerDir=per_dir, freRe=freRe,
                                  Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
             if bc is not None:
                 _bc_fields(inp, bc)
@@ -313,6 +313,9 @@ def main(only=()):
     cases = [
         ("sp_mrt_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
                                               u0=(1e-3, -2e-3, 5e-4), steps_list=[1, 101])),
+        # PerDir = 3: the permeability files report the z component
+        ("sp_perdir3_12x10x16", lambda n: sp_case(n, (12, 10, 16), 2.5, 0.55, 15, nu=0.12, g=(1e-6, 0.0, 2e-5),
+                                                   u0=(0.0, 1e-3, 0.0), steps_list=[31], loc_perm=True, per_dir=3)),
         ("sp_tau1_box_20x10x12", lambda n: sp_case(n, (20, 10, 12), 2.5, 0.5, 11, nu=1.0 / 6.0, g=(1e-6, 0.0, 0.0),
                                                     u0=(0.0, 0.0, 0.0), steps_list=[1, 51],
                                                     box=[4, 15, 1, 8, 2, 10])),

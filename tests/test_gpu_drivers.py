@@ -41,19 +41,21 @@ def _sp_case(pkg, tmp_path, name, n_max, **over):
     solid = d["solid"].astype(np.int32)
     pkg.geometry.write_binary(str(tmp_path / "geometry.bin"), solid)
     inp = pkg.inputs.SPInput(filename="geometry.bin", n_xyz=tuple(p["shape"]), n_max=n_max, reso=1.0,
-                             g=tuple(p["g"]), in_vis=p["nu"], ini_v=tuple(p["u0"]), wr_per=1, PerDir=1,
+                             g=tuple(p["g"]), in_vis=p["nu"], ini_v=tuple(p["u0"]), wr_per=1, PerDir=p.get("per_dir", 1),
                              freRe=p["freRe"], Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
     box = p["box"]
     if box is not None:
         inp.par_per = (1, 1, 1)
         inp.per_x, inp.per_y, inp.per_z = (box[1], box[0]), (box[3], box[2]), (box[5], box[4])
+    if p.get("loc_perm"):
+        inp.loc_perm = 1
     for k, v in over.items():
         setattr(inp, k, v)
     pkg.inputs.write_sp_input(inp, str(tmp_path / "INPUT.dat"))
     return d, p
 
 
-@pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12"])
+@pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12", "sp_perdir3_12x10x16"])
 def test_lbm_sp_outputs_match_reference(pkg, tmp_path, name):
     _log(f"[gpu] lbm_sp on {name}")
     d, p = _sp_case(pkg, tmp_path, name, n_max=p_last(name) - 1)
