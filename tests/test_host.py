@@ -3,8 +3,10 @@
 No GPU work happens here: the library is loaded and its symbol table checked, nothing that
 touches HIP is called."""
 import ctypes
+import json
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -131,3 +133,13 @@ def test_golden_fixtures_are_data_only():
         if name.endswith(".npz"):
             with np.load(os.path.join(d, name), allow_pickle=False) as z:
                 assert "solid" in z.files and "params" in z.files
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: asked for more it prints one JSON line saying so and exits
+    with status 2, before it loads the library or touches a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]
