@@ -141,13 +141,7 @@ def load_serving_truth(seed, nreq):
 def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000, quick=False):
     """Server vs sequential plain solves on the same seeded request stream (quick: the
     accelerated server alone, e.g. under a profiler)."""
-    geo = pkg.geometry
-    rng = np.random.default_rng(seed)
-    edges = rng.integers(48, 161, nreq)
-    pors = rng.uniform(0.15, 0.45, nreq)
-    gaps = rng.exponential(1.0 / rate, nreq)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    samples = [geo.grf_sample(int(e), 6.0, float(p), seed=seed * 1000 + i) for i, (e, p) in enumerate(zip(edges, pors))]
+    samples, arrivals, _, _ = pkg.request_stream(nreq, rate, seed)
     nu, g, interval = 1.0 / 6.0, (1e-6, 0.0, 0.0), 100
     nodes = [int(np.count_nonzero(s == 0)) for s in samples]
     truth = load_serving_truth(seed, nreq)

@@ -107,8 +107,8 @@ int lbm_sp_set_power_law(lbm_handle *h, double m, double n);
 int lbm_sp_get_viscosity(lbm_handle *h, double *nu);
 /* Post-streaming u and rho on the dense box, [i][j][k] (u: [i][j][k][3]), solid voxels u = 0,
  * rho = 1: the arrays the reference gathers on rank 0 for its VTK and profile writers
- * (output_velocity_b SP:4265-4390, output_density_b SP:4405-4505), scattered on the device and
- * copied back in one transfer each.  Either pointer may be NULL. */
+ * (output_velocity_b SP:4265-4390, output_density_b SP:4405-4505), scattered on the device into
+ * 16 MiB chunks of the box and copied back chunk by chunk.  Either pointer may be NULL. */
 int lbm_sp_get_dense(lbm_handle *h, double *u_ijk, double *rho_ijk);
 /* f = feq(rho, u0), u = u0, rho everywhere -- init (SP:1282-1383) */
 int lbm_sp_init(lbm_handle *h, double rho, const double u0[3]);
@@ -127,6 +127,12 @@ int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho);
 /* FP64 device reductions for Error (SP:3625-3704) and Comput_Perm (SP:4565-4712).
  * box = {xn, xp, yn, yp, zn, zp} inclusive voxel bounds (par_per_* / per_*) or NULL. */
 int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out);
+/* Error (SP:3625-3704) as a run of the reference on nranks MPI ranks computes it: the box split
+ * into x-slabs over the ranks as Parallelize_Geometry splits it (SP:1085-1108), each rank's sums
+ * over its own nodes but its last (SP:3651), err_r = sqrt(sum|u-u0|^2)/(sqrt(sum|u|^2)+1e-30);
+ * out[0] = the largest err_r, out[1] = the largest per-rank u_max, out[2] = the sum of |u| over
+ * the ranks (SP:3677-3685).  nranks = 1 is lbm_sp_diagnostics' err, umax and sum_speed. */
+int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3]);
 /* number of time steps taken since create/init/load */
 int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps);
 

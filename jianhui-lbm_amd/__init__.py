@@ -12,5 +12,5 @@ bench.py / tests/conftest.py); inside, modules import each other relatively.
 """
 from . import geometry, inputs  # noqa: F401
 from ._capi import LBMError, build, declared_symbols, lib  # noqa: F401
-from .serving import PermeabilityServer  # noqa: F401
+from .serving import PermeabilityServer, request_stream  # noqa: F401
 from .solver import ColorGradient, SinglePhase, Solute  # noqa: F401

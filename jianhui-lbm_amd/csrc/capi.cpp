@@ -140,6 +140,12 @@ int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho)
     return s ? s->get_state(f, u, rho) : LBM_ERR_ARG;
 }
 
+int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3])
+{
+    auto *s = as_sp(h);
+    return s ? s->error_ranks(nranks, out) : LBM_ERR_ARG;
+}
+
 int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out)
 {
     auto *s = as_sp(h);

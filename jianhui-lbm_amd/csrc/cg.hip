@@ -20,6 +20,7 @@
 //   col      2 x P  rho_r, rho_b after colour streaming (the reference's rho_r/rho_b)
 //   psi      1 x P  (rho_r - rho_b)/(rho_r + rho_b)
 //   cs       8 x P  per node, contiguous: rho_r, rho_b, ux, uy, uz, nx, ny, nz
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -325,7 +326,7 @@ __global__ void __launch_bounds__(BLOCK) cg_diag_kernel(CgArgs<ST> a, const int3
             v[6] = dx * dx + dy * dy + dz * dz;
             v[7] = s2;
             v[8] = sqrt(s2);
-            v[9] = s2;
+            v[9] = s2 > 0.0 ? s2 : 0.0;   // a NaN never passes the reference's "> u_max" (CG:5005-5006)
         }
         v[10] = (isfinite(ux) && isfinite(uy) && isfinite(uz) && isfinite(ps)) ? 0.0 : 1.0;
     }
@@ -336,15 +337,17 @@ __global__ void __launch_bounds__(BLOCK) cg_diag_kernel(CgArgs<ST> a, const int3
 }
 
 // ---------------------------------------------------------------- state I/O
+// state of the nodes n0 <= n < n1 at local index n - n0 of the outputs (any may be null)
 template <typename ST, int MODE>
-__global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, double *__restrict__ f_out,
-                                                             double *__restrict__ u_out, double *__restrict__ rho_out,
-                                                             double *__restrict__ psi_out, double *__restrict__ rr_out,
-                                                             double *__restrict__ rb_out)
+__global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, int64_t n0, int64_t n1,
+                                                             double *__restrict__ f_out, double *__restrict__ u_out,
+                                                             double *__restrict__ rho_out, double *__restrict__ psi_out,
+                                                             double *__restrict__ rr_out, double *__restrict__ rb_out)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t l = n - n0;
     C h[Q], m[Q];
     uint32_t nb[Q - 1];
     Macro<C> mc;
@@ -364,47 +367,51 @@ __global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, doubl
             for (int q = 0; q < Q; ++q) f[q] = (double)h[q] + weight(q) * (double)a.rho0;
         }
 #pragma unroll
-        for (int qc = 0; qc < Q; ++qc) f_out[n * Q + qc] = f[cg_to_canon(qc)];
+        for (int qc = 0; qc < Q; ++qc) f_out[l * Q + qc] = f[cg_to_canon(qc)];
     }
     if (u_out) {
-        u_out[n * 3 + 0] = (double)mc.ux;
-        u_out[n * 3 + 1] = (double)mc.uy;
-        u_out[n * 3 + 2] = (double)mc.uz;
+        u_out[l * 3 + 0] = (double)mc.ux;
+        u_out[l * 3 + 1] = (double)mc.uy;
+        u_out[l * 3 + 2] = (double)mc.uz;
     }
-    if (rho_out) rho_out[n] = (double)mc.rho;
-    if (psi_out) psi_out[n] = (double)a.psi[n];
-    if (rr_out) rr_out[n] = (double)a.col[n];
-    if (rb_out) rb_out[n] = (double)a.col[a.P + n];
+    if (rho_out) rho_out[l] = (double)mc.rho;
+    if (psi_out) psi_out[l] = (double)a.psi[n];
+    if (rr_out) rr_out[l] = (double)a.col[n];
+    if (rb_out) rb_out[l] = (double)a.col[a.P + n];
 }
 
+// a chunk of a post-streaming state: nodes n0 <= n < n1 at local index n - n0 of the inputs
 template <typename ST>
-__global__ void __launch_bounds__(BLOCK) cg_load_state_kernel(CgArgs<ST> a, const double *__restrict__ f,
-                                                              const double *__restrict__ u, const double *__restrict__ rho,
+__global__ void __launch_bounds__(BLOCK) cg_load_state_kernel(CgArgs<ST> a, int64_t n0, int64_t n1,
+                                                              const double *__restrict__ f, const double *__restrict__ u,
+                                                              const double *__restrict__ rho,
                                                               const double *__restrict__ psi, const double *__restrict__ rr,
                                                               const double *__restrict__ rb, typename Store<ST>::C *rho_aux)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t l = n - n0;
 #pragma unroll
     for (int qc = 0; qc < Q; ++qc) {
         const int q = cg_to_canon(qc);
-        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((C)(f[n * Q + qc] - weight(q) * (double)a.rho0), a.scale);
+        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((C)(f[l * Q + qc] - weight(q) * (double)a.rho0), a.scale);
     }
-    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (C)u[n * 3 + c];
-    rho_aux[n] = (C)rho[n];
-    a.psi[n] = (C)psi[n];
-    a.col[n] = (C)rr[n];
-    a.col[a.P + n] = (C)rb[n];
+    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (C)u[l * 3 + c];
+    rho_aux[n] = (C)rho[l];
+    a.psi[n] = (C)psi[l];
+    a.col[n] = (C)rr[l];
+    a.col[a.P + n] = (C)rb[l];
 }
 
+// phase of the nodes n0 <= n < n1, psi_node at local index n - n0
 template <typename C>
 __global__ void __launch_bounds__(BLOCK) cg_init_colour_kernel(const double *__restrict__ psi_node, C *col, C *psi,
-                                                               int64_t N, int64_t P)
+                                                               int64_t n0, int64_t n1, int64_t P)
 {
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= N) return;
-    double p = psi_node[n];
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    double p = psi_node[n - n0];
     p = p > 1.0 ? 1.0 : (p < -1.0 ? -1.0 : p);
     const double rr = (p * 1.0 + 1.0) / 2;   // rho_r = (psi rho + rho)/2 with rho = 1 (CG:2531-2532)
     psi[n] = (C)p;
@@ -466,6 +473,7 @@ CgSolver::~CgSolver() { release(); }
 void CgSolver::release()
 {
     (void)hipSetDevice(device_);
+    stage_.release();
     for (void *&p : pops_)
         if (p) (void)hipFree(p), p = nullptr;
     nbr_.release();
@@ -533,23 +541,28 @@ int CgSolver::init(const double *psi_ijk, const double u0[3])
 {
     if (!psi_ijk || !u0) return fail(LBM_ERR_ARG, "null phase field or velocity");
     LBM_HIP_TRY(hipSetDevice(device_));
-    const int64_t N = geo_.N;
-    std::vector<double> pn((size_t)N);
-    for (int64_t n = 0; n < N; ++n) pn[(size_t)n] = psi_ijk[geo_.supinv[(size_t)n]];
-    double *d = nullptr;
-    LBM_HIP_TRY(hipMalloc(&d, (size_t)N * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(d, pn.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) {
-        dim3 g((unsigned)grid_), b(BLOCK);
+    LBM_HIP_TRY(stage_.ensure());
+    const int64_t N = geo_.N, chunk = stage_chunk(STAGE_BYTES, 1);
+    std::vector<double> pn((size_t)std::min(chunk, N));
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N - n0);
+        for (int64_t n = 0; n < c; ++n) pn[(size_t)n] = psi_ijk[geo_.supinv[(size_t)(n0 + n)]];
+        double *d = (double *)stage_.buf;
+        e = hipMemcpyAsync(d, pn.data(), (size_t)c * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) break;
+        dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (prm_.precision == LBM_FP64)
-            cg_init_colour_kernel<double><<<g, b, 0, stream_>>>(d, (double *)d_col_, (double *)d_psi_, N, geo_.P);
+            cg_init_colour_kernel<double><<<g, b, 0, stream_>>>(d, (double *)d_col_, (double *)d_psi_, n0, n0 + c, geo_.P);
         else
-            cg_init_colour_kernel<float><<<g, b, 0, stream_>>>(d, (float *)d_col_, (float *)d_psi_, N, geo_.P);
+            cg_init_colour_kernel<float><<<g, b, 0, stream_>>>(d, (float *)d_col_, (float *)d_psi_, n0, n0 + c, geo_.P);
         e = hipGetLastError();
+        // the host chunk is refilled next round
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     for (int c = 0; c < 3; ++c) u_i_[c] = u0[c];
     mode_ = STATE_INIT;
     steps_ = 0;
@@ -561,34 +574,33 @@ int CgSolver::load_state(const double *f, const double *u, const double *rho, co
 {
     if (!f || !u || !rho || !psi || !rr || !rb) return fail(LBM_ERR_ARG, "null state array");
     LBM_HIP_TRY(hipSetDevice(device_));
-    const int64_t N = geo_.N;
-    double *df = nullptr, *dx = nullptr;
-    LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
-    LBM_HIP_TRY(hipMalloc(&dx, (size_t)N * 7 * sizeof(double)));   // u(3), rho, psi, rr, rb
-    double *du = dx, *dr = dx + 3 * N, *dp = dx + 4 * N, *drr = dx + 5 * N, *drb = dx + 6 * N;
-    hipError_t e = hipMemcpyAsync(df, f, (size_t)N * Q * sizeof(double), hipMemcpyHostToDevice, stream_);
-    const double *srcs[5] = {u, rho, psi, rr, rb};
-    double *dsts[5] = {du, dr, dp, drr, drb};
-    const int64_t lens[5] = {3 * N, N, N, N, N};
-    for (int k = 0; k < 5 && e == hipSuccess; ++k)
-        e = hipMemcpyAsync(dsts[k], srcs[k], (size_t)lens[k] * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) {
-        dim3 g((unsigned)grid_), b(BLOCK);
+    LBM_HIP_TRY(stage_.ensure());
+    const int64_t N = geo_.N, chunk = stage_chunk(STAGE_BYTES, Q + 7);
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N - n0);
+        double *df = (double *)stage_.buf, *du = df + c * Q, *dr = du + 3 * c, *dp = dr + c, *drr = dp + c, *drb = drr + c;
+        const double *srcs[6] = {f + n0 * Q, u + n0 * 3, rho + n0, psi + n0, rr + n0, rb + n0};
+        double *dsts[6] = {df, du, dr, dp, drr, drb};
+        const int64_t lens[6] = {c * Q, 3 * c, c, c, c, c};
+        for (int k = 0; k < 6 && e == hipSuccess; ++k)
+            e = hipMemcpyAsync(dsts[k], srcs[k], (size_t)lens[k] * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) break;
+        dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (prm_.precision == LBM_FP64) {
             auto a = make_cg_args<double>(prm_, visc_, u_i_, nullptr, pops_[cur_], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                           d_rho_, N, geo_.P);
-            cg_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (double *)d_rho_);
+            cg_load_state_kernel<double><<<g, b, 0, stream_>>>(a, n0, n0 + c, df, du, dr, dp, drr, drb, (double *)d_rho_);
         } else {
             auto a = make_cg_args<float>(prm_, visc_, u_i_, nullptr, pops_[cur_], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                          d_rho_, N, geo_.P);
-            cg_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, dp, drr, drb, (float *)d_rho_);
+            cg_load_state_kernel<float><<<g, b, 0, stream_>>>(a, n0, n0 + c, df, du, dr, dp, drr, drb, (float *)d_rho_);
         }
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(df);
-    (void)hipFree(dx);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     mode_ = STATE_DIRECT;
     steps_ = 0;
     return LBM_OK;
@@ -670,42 +682,51 @@ int CgSolver::step(int nsteps, double *elapsed_ms)
 int CgSolver::get_state(double *f, double *u, double *rho, double *psi, double *rr, double *rb)
 {
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
-    LBM_HIP_TRY(hipSetDevice(device_));
-    const int64_t N = geo_.N;
-    double *bufs[6] = {nullptr};
     double *outs[6] = {f, u, rho, psi, rr, rb};
-    const int64_t lens[6] = {N * Q, 3 * N, N, N, N, N};
-    for (int k = 0; k < 6; ++k)
-        if (outs[k]) LBM_HIP_TRY(hipMalloc(&bufs[k], (size_t)lens[k] * sizeof(double)));
+    const int width[6] = {Q, 3, 1, 1, 1, 1};
+    int per_node = 0;
+    for (int k = 0; k < 6; ++k) per_node += outs[k] ? width[k] : 0;
+    if (per_node == 0) return LBM_OK;
+    LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(stage_.ensure());
+    const int64_t N = geo_.N, chunk = stage_chunk(STAGE_BYTES, per_node);
     const int mode = cg_kmode(mode_);
-    dim3 g((unsigned)grid_), b(BLOCK);
-    hipError_t e;
-    if (prm_.precision == LBM_FP64) {
-        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
-                                      d_rho_, N, geo_.P);
-        if (mode == K_INIT)
-            cg_get_state_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
-        else if (mode == K_DIRECT)
-            cg_get_state_kernel<double, K_DIRECT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
-        else
-            cg_get_state_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
-    } else {
-        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
-                                     d_rho_, N, geo_.P);
-        if (mode == K_INIT)
-            cg_get_state_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
-        else if (mode == K_DIRECT)
-            cg_get_state_kernel<float, K_DIRECT><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
-        else
-            cg_get_state_kernel<float, K_PULL><<<g, b, 0, stream_>>>(a, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N - n0), n1 = n0 + c;
+        double *bufs[6] = {nullptr};
+        double *p = (double *)stage_.buf;
+        for (int k = 0; k < 6; ++k)
+            if (outs[k]) bufs[k] = p, p += c * width[k];
+        dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
+        if (prm_.precision == LBM_FP64) {
+            auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
+                                          d_rho_, N, geo_.P);
+            if (mode == K_INIT)
+                cg_get_state_kernel<double, K_INIT><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+            else if (mode == K_DIRECT)
+                cg_get_state_kernel<double, K_DIRECT><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+            else
+                cg_get_state_kernel<double, K_PULL><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        } else {
+            auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
+                                         d_rho_, N, geo_.P);
+            if (mode == K_INIT)
+                cg_get_state_kernel<float, K_INIT><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+            else if (mode == K_DIRECT)
+                cg_get_state_kernel<float, K_DIRECT><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+            else
+                cg_get_state_kernel<float, K_PULL><<<g, b, 0, stream_>>>(a, n0, n1, bufs[0], bufs[1], bufs[2], bufs[3], bufs[4], bufs[5]);
+        }
+        e = hipGetLastError();
+        for (int k = 0; k < 6 && e == hipSuccess; ++k)
+            if (outs[k])
+                e = hipMemcpyAsync(outs[k] + n0 * width[k], bufs[k], (size_t)c * width[k] * sizeof(double),
+                                   hipMemcpyDeviceToHost, stream_);
     }
-    e = hipGetLastError();
-    for (int k = 0; k < 6 && e == hipSuccess; ++k)
-        if (outs[k]) e = hipMemcpyAsync(outs[k], bufs[k], (size_t)lens[k] * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    for (double *p : bufs)
-        if (p) (void)hipFree(p);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     return LBM_OK;
 }
 

@@ -64,6 +64,30 @@ int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid, SparseGe
     return LBM_OK;
 }
 
+void reference_rank_ends(const SparseGeometry &g, int nranks, std::vector<int64_t> &ends)
+{
+    // pore count of each x-plane, and the node id where each plane starts
+    const int64_t nyz = (int64_t)g.ny * g.nz;
+    std::vector<int64_t> loc_por((size_t)g.nx, 0), first((size_t)g.nx + 1, 0);
+    for (int64_t n = 0; n < g.N; ++n) ++loc_por[(size_t)(g.supinv[(size_t)n] / nyz)];
+    for (int i = 0; i < g.nx; ++i) first[(size_t)i + 1] = first[(size_t)i] + loc_por[(size_t)i];
+    // cut r + 1 where the running plane sum first reaches ave * (r + 1): before or after the plane
+    // that crosses it, whichever sum is closer (SP:1089-1102); the last rank keeps the rest
+    const double ave = (double)g.N / nranks;
+    ends.assign((size_t)nranks, g.N);
+    for (int r = 0; r < nranks - 1; ++r) {
+        int64_t pre = 0, aft = 0;
+        int n_i = 0;
+        while (aft < ave * (r + 1) && n_i < g.nx) {
+            pre = aft;
+            aft += loc_por[(size_t)n_i];
+            ++n_i;
+        }
+        const int disp = ((double)(aft - ave * (r + 1)) > (double)(ave * (r + 1) - pre)) ? n_i - 1 : n_i;
+        ends[(size_t)r] = first[(size_t)disp];
+    }
+}
+
 bool pores_wrap_along(const SparseGeometry &g, int axis)
 {
     const int64_t P = g.P;

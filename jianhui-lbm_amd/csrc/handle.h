@@ -41,6 +41,11 @@ struct SparseGeometry {
 int build_sparse_geometry(int nx, int ny, int nz, const int32_t *solid_ijk, SparseGeometry &g,
                           std::string &err);
 
+// The reference's x-slab split of the box over nranks MPI ranks (Parallelize_Geometry,
+// SP:1085-1108): ends[r] is the exclusive end of rank r's range of node ids (the numbering is
+// i-major, so each rank's slab is a range).
+void reference_rank_ends(const SparseGeometry &g, int nranks, std::vector<int64_t> &ends);
+
 // Whether some cluster of pore nodes joined by the 18 lattice links wraps round the periodic box
 // along axis (0 = x, 1 = y, 2 = z).  Without such a cluster there is no net throughflow along
 // the axis at steady state: the permeability is zero up to the lattice's spurious currents.
@@ -128,6 +133,30 @@ struct BcDev {
 
 enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
 
+// FP64 state crosses between host and device in chunks through one bounded device buffer,
+// allocated on a handle's first state transfer and kept until the handle is destroyed, so a
+// checkpoint of a large state (f alone is 0.9 GB in FP64 for 5.9 M nodes) needs no full-size
+// scratch.
+constexpr size_t STAGE_BYTES = size_t(16) << 20;
+struct Staging {
+    void *buf = nullptr;
+    hipError_t ensure()
+    {
+        return buf ? hipSuccess : hipMalloc(&buf, STAGE_BYTES);
+    }
+    void release()
+    {
+        if (buf) (void)hipFree(buf), buf = nullptr;
+    }
+};
+// nodes per chunk when each node carries per_node doubles through a buffer of the given size
+// (a multiple of the block size, at least one block)
+inline int64_t stage_chunk(size_t bytes, int per_node)
+{
+    const int64_t c = (int64_t)(bytes / (sizeof(double) * (size_t)per_node)) / 256 * 256;
+    return c > 256 ? c : 256;
+}
+
 // a coupled solute's buffers and rates for the fused flow + solute launch (compute type of the
 // flow handle)
 struct FusedSolute {
@@ -135,6 +164,8 @@ struct FusedSolute {
     void *dst;
     double omega, w0, wq, inv_cs2;
 };
+
+struct DiagBox;
 
 class SpSolver : public lbm_handle {
 public:
@@ -146,7 +177,11 @@ public:
     int load_state(const double *f, const double *u, const double *rho);
     int step(int nsteps, double *elapsed_ms);
     int get_state(double *f, double *u, double *rho);
+    // get_state through a caller-chosen device buffer of the given size (no allocation)
+    int get_state_via(double *f, double *u, double *rho, void *buf, size_t bytes);
     int diagnostics(const int32_t *box, lbm_sp_diag *out);
+    // Error over the reference's nranks x-slab ranks: err, u_max, sum of |u|
+    int error_ranks(int nranks, double out[3]);
     int set_boundaries(const lbm_sp_bc *bc);
     int set_power_law(double m, double n);
     int get_viscosity(double *nu);
@@ -176,7 +211,9 @@ private:
     void release_boundaries();
     BcDev bc_view(bool for_step) const;
     int reset_viscosity();
-    hipError_t state_to_device(double *df, double *du, double *dr);
+    int reduce_range(const DiagBox &db, int64_t n_lo, int64_t n_hi);
+    hipError_t state_to_device(int64_t n0, int64_t n1, const int32_t *supinv, int64_t v0, double *df, double *du,
+                               double *dr);
 
     lbm_sp_params prm_{};
     SparseGeometry geo_;
@@ -208,6 +245,7 @@ private:
     // power-law collision (none unless set_power_law was called)
     void *d_nu_vn_ = nullptr;   // P compute type
     double pl_m_ = 0.0, pl_n_ = 0.0;
+    Staging stage_;   // host transfers of the FP64 state
 };
 
 class CgSolver : public lbm_handle {
@@ -254,6 +292,7 @@ private:
     StateMode mode_ = STATE_EMPTY;
     double u_i_[3] = {0, 0, 0};
     int64_t steps_ = 0;
+    Staging stage_;
 };
 
 class StSolver : public lbm_handle {
@@ -300,6 +339,7 @@ private:
     double *d_partial_ = nullptr, *d_result_ = nullptr, *h_result_ = nullptr;
     StateMode mode_ = STATE_EMPTY;
     int64_t steps_ = 0;
+    Staging stage_;
 };
 
 }  // namespace lbm

@@ -187,22 +187,6 @@ __global__ void __launch_bounds__(BLOCK) fill_kernel(C *__restrict__ p, C v, int
     if (i < n) p[i] = v;
 }
 
-// node-ordered u [N][3] and rho [N] into the dense [i][j][k] grid (voxel supinv[n])
-__global__ void __launch_bounds__(BLOCK) scatter_dense_kernel(const int32_t *__restrict__ supinv, int64_t N,
-                                                              const double *__restrict__ u, const double *__restrict__ rho,
-                                                              double *__restrict__ gu, double *__restrict__ gr)
-{
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= N) return;
-    const int64_t s = supinv[n];
-    if (gu) {
-        gu[s * 3 + 0] = u[n * 3 + 0];
-        gu[s * 3 + 1] = u[n * 3 + 1];
-        gu[s * 3 + 2] = u[n * 3 + 2];
-    }
-    if (gr) gr[s] = rho[n];
-}
-
 // ---------------------------------------------------------------- diagnostics
 constexpr int NDIAG = 8;   // Q0 Q1 Q2 err_num err_den speed_sum umax2 nan  (+ box count below)
 
@@ -211,16 +195,19 @@ struct DiagBox {
     int32_t ny, nz;
 };
 
+// Sums over the nodes n_lo <= n < n_hi (all nodes, or one reference rank's share of them for
+// Error over ranks); the Error terms skip the range's last node.
 template <typename ST, int MODE>
 __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int32_t *__restrict__ supinv,
-                                                        DiagBox box, double *__restrict__ partial)
+                                                        DiagBox box, int64_t n_lo, int64_t n_hi,
+                                                        double *__restrict__ partial)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t n = n_lo + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double v[NDIAG + 1];
 #pragma unroll
     for (int k = 0; k < NDIAG + 1; ++k) v[k] = 0.0;
-    if (n < a.N) {
+    if (n < n_hi) {
         C h[Q], m[Q];
         Macro<C> mc;
         if (MODE == K_INIT) {
@@ -245,14 +232,14 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
             v[2] = uz;
             v[8] = 1.0;
         }
-        if (n < a.N - 1) {   // the reference's Error loop stops before the last node (SP:3651)
+        if (n < n_hi - 1) {   // the reference's Error loop stops before a rank's last node (SP:3651)
             const double dx = ux - (double)a.u_aux[n], dy = uy - (double)a.u_aux[a.P + n],
                          dz = uz - (double)a.u_aux[2 * a.P + n];
             const double s2 = ux * ux + uy * uy + uz * uz;
             v[3] = dx * dx + dy * dy + dz * dz;
             v[4] = s2;
             v[5] = sqrt(s2);
-            v[6] = s2;
+            v[6] = s2 > 0.0 ? s2 : 0.0;   // a NaN never passes the reference's "> u_max" (SP:3660)
         }
         v[7] = (isfinite(ux) && isfinite(uy) && isfinite(uz)) ? 0.0 : 1.0;
     }
@@ -261,13 +248,18 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
 }
 
 // ---------------------------------------------------------------- state I/O
+// Post-streaming f, u and rho of the nodes n0 <= n < n1.  Without a voxel map the node's values
+// land at local index n - n0 of the outputs; with one (dense output) at supinv[n] - v0.
 template <typename ST, int MODE>
-__global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, double *__restrict__ f_out,
-                                                             double *__restrict__ u_out, double *__restrict__ rho_out)
+__global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, int64_t n0, int64_t n1,
+                                                             const int32_t *__restrict__ supinv, int64_t v0,
+                                                             double *__restrict__ f_out, double *__restrict__ u_out,
+                                                             double *__restrict__ rho_out)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t l = supinv ? (int64_t)supinv[n] - v0 : n - n0;
     C h[Q], m[Q];
     Macro<C> mc;
     if (MODE == K_INIT) {
@@ -278,7 +270,7 @@ __global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, doubl
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const double eu = EX[q] * ux + EY[q] * uy + EZ[q] * uz;
-            if (f_out) f_out[n * Q + q] = weight(q) * (double)mc.rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
+            if (f_out) f_out[l * Q + q] = weight(q) * (double)mc.rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uv);
         }
     } else {
         gather<ST, MODE>(a, n, h);
@@ -287,29 +279,32 @@ __global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, doubl
         mc = node_macro<ST, MODE>(a, n, m);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (f_out) f_out[n * Q + q] = (double)h[q] + weight(q) * (double)a.rho0;
+            if (f_out) f_out[l * Q + q] = (double)h[q] + weight(q) * (double)a.rho0;
     }
     if (u_out) {
-        u_out[n * 3 + 0] = (double)mc.ux;
-        u_out[n * 3 + 1] = (double)mc.uy;
-        u_out[n * 3 + 2] = (double)mc.uz;
+        u_out[l * 3 + 0] = (double)mc.ux;
+        u_out[l * 3 + 1] = (double)mc.uy;
+        u_out[l * 3 + 2] = (double)mc.uz;
     }
-    if (rho_out) rho_out[n] = (double)mc.rho;
+    if (rho_out) rho_out[l] = (double)mc.rho;
 }
 
+// a chunk of a post-streaming state, nodes n0 <= n < n1 at local index n - n0 of the inputs
 template <typename ST>
-__global__ void __launch_bounds__(BLOCK) sp_load_state_kernel(SpArgs<ST> a, const double *__restrict__ f,
-                                                              const double *__restrict__ u, const double *__restrict__ rho,
+__global__ void __launch_bounds__(BLOCK) sp_load_state_kernel(SpArgs<ST> a, int64_t n0, int64_t n1,
+                                                              const double *__restrict__ f, const double *__restrict__ u,
+                                                              const double *__restrict__ rho,
                                                               typename Store<ST>::C *__restrict__ rho_aux)
 {
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t l = n - n0;
 #pragma unroll
     for (int q = 0; q < Q; ++q)
-        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((typename Store<ST>::C)(f[n * Q + q] - weight(q) * (double)a.rho0),
+        a.dst[(int64_t)q * a.P + n] = Store<ST>::put((typename Store<ST>::C)(f[l * Q + q] - weight(q) * (double)a.rho0),
                                                     a.scale);
-    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (typename Store<ST>::C)u[n * 3 + c];
-    rho_aux[n] = (typename Store<ST>::C)rho[n];
+    for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (typename Store<ST>::C)u[l * 3 + c];
+    rho_aux[n] = (typename Store<ST>::C)rho[l];
 }
 
 // ---------------------------------------------------------------- host side
@@ -397,32 +392,33 @@ hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m
 }
 
 template <typename ST>
-hipError_t launch_diag_t(SpArgs<ST> a, const BcDev &bc, int mode, const int32_t *supinv, DiagBox box, double *partial,
-                         int64_t grid, hipStream_t s)
+hipError_t launch_diag_t(SpArgs<ST> a, const BcDev &bc, int mode, const int32_t *supinv, DiagBox box, int64_t n_lo,
+                         int64_t n_hi, double *partial, int64_t grid, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
     a.bc = bc;
     if (mode == K_INIT)
-        sp_diag_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, supinv, box, partial);
+        sp_diag_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, supinv, box, n_lo, n_hi, partial);
     else if (mode == K_DIRECT)
-        sp_diag_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, supinv, box, partial);
+        sp_diag_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, supinv, box, n_lo, n_hi, partial);
     else
-        sp_diag_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, supinv, box, partial);
+        sp_diag_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, supinv, box, n_lo, n_hi, partial);
     return hipGetLastError();
 }
 
 template <typename ST>
-hipError_t launch_get_t(SpArgs<ST> a, const BcDev &bc, int mode, double *f, double *u, double *rho, int64_t grid,
-                        hipStream_t s)
+hipError_t launch_get_t(SpArgs<ST> a, const BcDev &bc, int mode, int64_t n0, int64_t n1, const int32_t *supinv,
+                        int64_t v0, double *f, double *u, double *rho, hipStream_t s)
 {
-    dim3 g((unsigned)grid), b(BLOCK);
+    if (n1 <= n0) return hipSuccess;
+    dim3 g((unsigned)((n1 - n0 + BLOCK - 1) / BLOCK)), b(BLOCK);
     a.bc = bc;
     if (mode == K_INIT)
-        sp_get_state_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, f, u, rho);
+        sp_get_state_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, n0, n1, supinv, v0, f, u, rho);
     else if (mode == K_DIRECT)
-        sp_get_state_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, f, u, rho);
+        sp_get_state_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, n0, n1, supinv, v0, f, u, rho);
     else
-        sp_get_state_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, f, u, rho);
+        sp_get_state_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, n0, n1, supinv, v0, f, u, rho);
     return hipGetLastError();
 }
 
@@ -448,6 +444,7 @@ int SpSolver::release()
     if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
     release_boundaries();
     if (d_nu_vn_) (void)hipFree(d_nu_vn_), d_nu_vn_ = nullptr;
+    stage_.release();
     if (stream_) (void)hipStreamDestroy(stream_), stream_ = nullptr;
     return LBM_OK;
 }
@@ -520,38 +517,38 @@ int SpSolver::load_state(const double *f, const double *u, const double *rho)
 {
     if (!f || !u || !rho) return fail(LBM_ERR_ARG, "null state array");
     LBM_HIP_TRY(hipSetDevice(device_));
-    const int64_t N = geo_.N;
-    double *df = nullptr, *du = nullptr, *dr = nullptr;
-    LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
-    LBM_HIP_TRY(hipMalloc(&du, (size_t)N * 3 * sizeof(double)));
-    LBM_HIP_TRY(hipMalloc(&dr, (size_t)N * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(df, f, (size_t)N * Q * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(du, u, (size_t)N * 3 * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) e = hipMemcpyAsync(dr, rho, (size_t)N * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) {
-        const int nxt = cur_;   // DIRECT state lives in the current buffer
-        const double zero[3] = {0, 0, 0};
-        dim3 g((unsigned)grid_), b(BLOCK);
+    LBM_HIP_TRY(stage_.ensure());
+    const int64_t N = geo_.N, chunk = stage_chunk(STAGE_BYTES, Q + 4);
+    const int nxt = cur_;   // DIRECT state lives in the current buffer
+    const double zero[3] = {0, 0, 0};
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N - n0);
+        double *df = (double *)stage_.buf, *du = df + c * Q, *dr = du + c * 3;
+        e = hipMemcpyAsync(df, f + n0 * Q, (size_t)c * Q * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(du, u + n0 * 3, (size_t)c * 3 * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess) e = hipMemcpyAsync(dr, rho + n0, (size_t)c * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) break;
+        dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (prm_.precision == LBM_FP64) {
             auto a = make_args<double>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                        scale_, inv_scale_);
-            sp_load_state_kernel<double><<<g, b, 0, stream_>>>(a, df, du, dr, (double *)d_rho_);
+            sp_load_state_kernel<double><<<g, b, 0, stream_>>>(a, n0, n0 + c, df, du, dr, (double *)d_rho_);
         } else if (prm_.precision == LBM_FP32) {
             auto a = make_args<float>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                       scale_, inv_scale_);
-            sp_load_state_kernel<float><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
+            sp_load_state_kernel<float><<<g, b, 0, stream_>>>(a, n0, n0 + c, df, du, dr, (float *)d_rho_);
         } else {
             auto a = make_args<__half>(nullptr, pops_[nxt], nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_, 1.0, zero,
                                        scale_, inv_scale_);
-            sp_load_state_kernel<__half><<<g, b, 0, stream_>>>(a, df, du, dr, (float *)d_rho_);
+            sp_load_state_kernel<__half><<<g, b, 0, stream_>>>(a, n0, n0 + c, df, du, dr, (float *)d_rho_);
         }
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(df);
-    (void)hipFree(du);
-    (void)hipFree(dr);
+    // the host arrays are borrowed for this call only: wait for the copies either way
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     mode_ = STATE_DIRECT;
     steps_ = 0;
     // the reference's restart leaves niu_vn unset (it is set in init, SP:1355); here it starts
@@ -721,42 +718,57 @@ int SpSolver::step(int nsteps, double *elapsed_ms)
     return LBM_OK;
 }
 
-// post-streaming f (AoS [N][19]), u ([N][3]) and rho into device buffers (any may be null)
-hipError_t SpSolver::state_to_device(double *df, double *du, double *dr)
+// post-streaming f (AoS [.][19]), u ([.][3]) and rho of nodes n0 <= n < n1 into device buffers
+// (any may be null), at local index n - n0, or at supinv[n] - v0 of a dense voxel range
+hipError_t SpSolver::state_to_device(int64_t n0, int64_t n1, const int32_t *supinv, int64_t v0, double *df, double *du,
+                                     double *dr)
 {
     const int64_t N = geo_.N;
     const int mode = kmode_of(mode_);
     if (prm_.precision == LBM_FP64)
         return launch_get_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
                                               rho_i_, u_i_, scale_, inv_scale_),
-                            bc_view(false), mode, df, du, dr, grid_, stream_);
+                            bc_view(false), mode, n0, n1, supinv, v0, df, du, dr, stream_);
     if (prm_.precision == LBM_FP32)
         return launch_get_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
                                              rho_i_, u_i_, scale_, inv_scale_),
-                            bc_view(false), mode, df, du, dr, grid_, stream_);
+                            bc_view(false), mode, n0, n1, supinv, v0, df, du, dr, stream_);
     return launch_get_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, N, geo_.P, prm_, sv_, sq_,
                                           rho_i_, u_i_, scale_, inv_scale_),
-                        bc_view(false), mode, df, du, dr, grid_, stream_);
+                        bc_view(false), mode, n0, n1, supinv, v0, df, du, dr, stream_);
 }
 
 int SpSolver::get_state(double *f, double *u, double *rho)
 {
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
     LBM_HIP_TRY(hipSetDevice(device_));
-    const int64_t N = geo_.N;
-    double *df = nullptr, *du = nullptr, *dr = nullptr;
-    if (f) LBM_HIP_TRY(hipMalloc(&df, (size_t)N * Q * sizeof(double)));
-    if (u) LBM_HIP_TRY(hipMalloc(&du, (size_t)N * 3 * sizeof(double)));
-    if (rho) LBM_HIP_TRY(hipMalloc(&dr, (size_t)N * sizeof(double)));
-    hipError_t e = state_to_device(df, du, dr);
-    if (e == hipSuccess && f) e = hipMemcpyAsync(f, df, (size_t)N * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess && u) e = hipMemcpyAsync(u, du, (size_t)N * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess && rho) e = hipMemcpyAsync(rho, dr, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    if (df) (void)hipFree(df);
-    if (du) (void)hipFree(du);
-    if (dr) (void)hipFree(dr);
+    LBM_HIP_TRY(stage_.ensure());
+    return get_state_via(f, u, rho, stage_.buf, STAGE_BYTES);
+}
+
+// the state crosses through buf in chunks of whole blocks of nodes; each chunk's copies are
+// ordered on the stream before the next chunk's kernel overwrites buf
+int SpSolver::get_state_via(double *f, double *u, double *rho, void *buf, size_t bytes)
+{
+    const int per_node = (f ? Q : 0) + (u ? 3 : 0) + (rho ? 1 : 0);
+    if (per_node == 0) return LBM_OK;
+    if (!buf || bytes < (size_t)BLOCK * per_node * sizeof(double))
+        return fail(LBM_ERR_ARG, "state transfer buffer smaller than one block of nodes");
+    const int64_t N = geo_.N, chunk = stage_chunk(bytes, per_node);
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N - n0);
+        double *df = f ? (double *)buf : nullptr;
+        double *du = u ? (double *)buf + (f ? c * Q : 0) : nullptr;
+        double *dr = rho ? (double *)buf + (f ? c * Q : 0) + (u ? c * 3 : 0) : nullptr;
+        e = state_to_device(n0, n0 + c, nullptr, 0, df, du, dr);
+        if (e == hipSuccess && f) e = hipMemcpyAsync(f + n0 * Q, df, (size_t)c * Q * sizeof(double), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess && u) e = hipMemcpyAsync(u + n0 * 3, du, (size_t)c * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess && rho) e = hipMemcpyAsync(rho + n0, dr, (size_t)c * sizeof(double), hipMemcpyDeviceToHost, stream_);
+    }
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     return LBM_OK;
 }
 
@@ -765,31 +777,64 @@ int SpSolver::get_dense(double *u_ijk, double *rho_ijk)
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
     if (!u_ijk && !rho_ijk) return LBM_OK;
     LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(stage_.ensure());
+    // the box in chunks of consecutive voxels; the fluid nodes of a voxel range are a range of
+    // node ids, since the numbering is i-major like the voxel index
     const int64_t N = geo_.N, V = (int64_t)geo_.nx * geo_.ny * geo_.nz;
-    double *du = nullptr, *dr = nullptr, *gu = nullptr, *gr = nullptr;
+    const int64_t chunk = stage_chunk(STAGE_BYTES, (u_ijk ? 3 : 0) + (rho_ijk ? 1 : 0));
+    const std::vector<int32_t> &sup = geo_.supinv;
     hipError_t e = hipSuccess;
-    auto alloc = [&](double **p, size_t n) {
-        if (e == hipSuccess) e = hipMalloc((void **)p, n * sizeof(double));
-    };
-    if (u_ijk) alloc(&du, (size_t)N * 3), alloc(&gu, (size_t)V * 3);
-    if (rho_ijk) alloc(&dr, (size_t)N), alloc(&gr, (size_t)V);
-    // solid voxels keep the values the reference's writers put there: u = 0, rho = 1
-    if (e == hipSuccess && gu) e = hipMemsetAsync(gu, 0, (size_t)V * 3 * sizeof(double), stream_);
-    if (e == hipSuccess && gr) {
-        fill_kernel<double><<<dim3((unsigned)((V + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream_>>>(gr, 1.0, V);
-        e = hipGetLastError();
+    for (int64_t v0 = 0; v0 < V && e == hipSuccess; v0 += chunk) {
+        const int64_t c = std::min(chunk, V - v0);
+        double *gu = u_ijk ? (double *)stage_.buf : nullptr;
+        double *gr = rho_ijk ? (double *)stage_.buf + (u_ijk ? c * 3 : 0) : nullptr;
+        // solid voxels keep the values the reference's writers put there: u = 0, rho = 1
+        const dim3 b(BLOCK);
+        if (gu) {
+            fill_kernel<double><<<dim3((unsigned)((c * 3 + BLOCK - 1) / BLOCK)), b, 0, stream_>>>(gu, 0.0, c * 3);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && gr) {
+            fill_kernel<double><<<dim3((unsigned)((c + BLOCK - 1) / BLOCK)), b, 0, stream_>>>(gr, 1.0, c);
+            e = hipGetLastError();
+        }
+        const int64_t n0 = std::lower_bound(sup.begin(), sup.end(), v0) - sup.begin();
+        const int64_t n1 = std::lower_bound(sup.begin(), sup.end(), (int64_t)(v0 + c)) - sup.begin();
+        if (e == hipSuccess) e = state_to_device(n0, std::min(n1, N), d_supinv_, v0, nullptr, gu, gr);
+        if (e == hipSuccess && gu)
+            e = hipMemcpyAsync(u_ijk + v0 * 3, gu, (size_t)c * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess && gr)
+            e = hipMemcpyAsync(rho_ijk + v0, gr, (size_t)c * sizeof(double), hipMemcpyDeviceToHost, stream_);
     }
-    if (e == hipSuccess) e = state_to_device(nullptr, du, dr);
-    if (e == hipSuccess) {
-        scatter_dense_kernel<<<dim3((unsigned)grid_), dim3(BLOCK), 0, stream_>>>(d_supinv_, N, du, dr, gu, gr);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess && gu) e = hipMemcpyAsync(u_ijk, gu, (size_t)V * 3 * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess && gr) e = hipMemcpyAsync(rho_ijk, gr, (size_t)V * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    for (double *p : {du, dr, gu, gr})
-        if (p) (void)hipFree(p);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
+    return LBM_OK;
+}
+
+// the diagnostic sums over nodes [n_lo, n_hi) into h_result_ (NDIAG + 1 values)
+int SpSolver::reduce_range(const DiagBox &db, int64_t n_lo, int64_t n_hi)
+{
+    const int mode = kmode_of(mode_);
+    const int64_t grid = std::max<int64_t>(1, (n_hi - n_lo + BLOCK - 1) / BLOCK);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64)
+        e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, d_supinv_, db, n_lo, n_hi, d_partial_, grid, stream_);
+    else if (prm_.precision == LBM_FP32)
+        e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                           rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, d_supinv_, db, n_lo, n_hi, d_partial_, grid, stream_);
+    else
+        e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, d_supinv_, db, n_lo, n_hi, d_partial_, grid, stream_);
+    LBM_HIP_TRY(e);
+    reduce_partials_kernel<NDIAG + 1><<<1, BLOCK, 0, stream_>>>(d_partial_, grid, 1u << 6, d_result_);
+    LBM_HIP_TRY(hipGetLastError());
+    LBM_HIP_TRY(hipMemcpyAsync(h_result_, d_result_, (NDIAG + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;
 }
 
@@ -810,25 +855,8 @@ int SpSolver::diagnostics(const int32_t *box, lbm_sp_diag *out)
         db.zn = box[4];
         db.zp = box[5];
     }
-    const int mode = kmode_of(mode_);
-    hipError_t e;
-    if (prm_.precision == LBM_FP64)
-        e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
-                                            rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
-    else if (prm_.precision == LBM_FP32)
-        e = launch_diag_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
-                                           rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
-    else
-        e = launch_diag_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
-                                            rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(false), mode, d_supinv_, db, d_partial_, grid_, stream_);
-    LBM_HIP_TRY(e);
-    reduce_partials_kernel<NDIAG + 1><<<1, BLOCK, 0, stream_>>>(d_partial_, grid_, 1u << 6, d_result_);
-    LBM_HIP_TRY(hipGetLastError());
-    LBM_HIP_TRY(hipMemcpyAsync(h_result_, d_result_, (NDIAG + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    const int rc = reduce_range(db, 0, geo_.N);
+    if (rc != LBM_OK) return rc;
     const double *r = h_result_;
     out->Q[0] = r[0];
     out->Q[1] = r[1];
@@ -838,6 +866,39 @@ int SpSolver::diagnostics(const int32_t *box, lbm_sp_diag *out)
     out->sum_speed = r[5];
     out->nan_count = r[7];
     out->count_in_box = r[8];
+    return LBM_OK;
+}
+
+int SpSolver::error_ranks(int nranks, double out[3])
+{
+    if (!out) return fail(LBM_ERR_ARG, "null output");
+    if (nranks < 1) return fail(LBM_ERR_ARG, "rank count must be at least 1");
+    if (nranks > geo_.nx) return fail(LBM_ERR_ARG, "more ranks than x-planes");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    std::vector<int64_t> ends;
+    reference_rank_ends(geo_, nranks, ends);
+    DiagBox db{};
+    db.ny = geo_.ny;
+    db.nz = geo_.nz;
+    // per rank: its own err and u_max, then the largest of each and the sum of the speed sums
+    // (SP:3666-3685)
+    double err = 0, umax = 0, speed = 0;
+    int64_t lo = 0;
+    for (int r = 0; r < nranks; ++r) {
+        const int64_t hi = ends[(size_t)r];
+        const int rc = reduce_range(db, lo, hi);
+        if (rc != LBM_OK) return rc;
+        const double e = std::sqrt(h_result_[3]) / (std::sqrt(h_result_[4]) + 1e-30);
+        const double um = std::sqrt(h_result_[6]);
+        if (r == 0 || e > err) err = e;
+        if (um > umax) umax = um;
+        speed += h_result_[5];
+        lo = hi;
+    }
+    out[0] = err;
+    out[1] = umax;
+    out[2] = speed;
     return LBM_OK;
 }
 

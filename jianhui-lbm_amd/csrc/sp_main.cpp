@@ -1,6 +1,6 @@
 // sp_main.cpp -- lbm_sp: drop-in single-phase driver for SINGLE_PHASE_MPI_Software_Spars.
 //
-//   lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N]
+//   lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N] [--ranks K]
 //
 // Reads the reference's INPUT.dat and geometry file, runs the time loop of SP:662-841 on one
 // GPU through the C ABI, and writes the reference's result files under the output prefix:
@@ -11,6 +11,11 @@
 // power-law collision (nnf_module = 1) runs on the device too (lbm_sp_set_power_law), and so do
 // the field writers and the local and group permeabilities (loc_perm = 1, gperm > 0).  Lattice
 // velocity scaling, outside the hot path, is refused, never silently approximated.
+//
+// --ranks K prints Error's velocity change and maximum speed (Results.txt, Velocity_ave_max.txt)
+// as the reference computes them on K MPI ranks, each rank over its own x-slab without its last
+// node (SP:3651-3685; lbm_sp_error_ranks), so that the files equal those of an `mpirun -np K` run
+// of the reference.  The default, 1, is the single-domain value.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -55,17 +60,20 @@ void hms(double s, int &h, int &m, int &sec)
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::cerr << "usage: lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N]\n";
+        std::cerr << "usage: lbm_sp INPUT.dat [checkpoint_prefix] [--precision fp64|fp32|fp16] [--device N] [--ranks K]\n";
         return 2;
     }
     std::string input = argv[1], pfix = "./";
-    int precision = LBM_FP64, device = 0;
+    int precision = LBM_FP64, device = 0, ranks = 1;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--precision") && a + 1 < argc) {
             const std::string p = argv[++a];
             precision = p == "fp32" ? LBM_FP32 : (p == "fp16" ? LBM_FP16 : LBM_FP64);
         } else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) {
             device = std::atoi(argv[++a]);
+        } else if (!std::strcmp(argv[a], "--ranks") && a + 1 < argc) {
+            ranks = std::atoi(argv[++a]);
+            if (ranks < 1) return die("--ranks needs a positive rank count");
         } else {
             pfix = argv[a];
         }
@@ -243,6 +251,12 @@ int main(int argc, char **argv)
         }
         error = dfull.err;
         u_max = dfull.umax;
+        if (ranks > 1) {
+            double er[3];
+            if (check(lbm_sp_error_ranks(h, ranks, er), h, "lbm_sp_error_ranks")) return 1;
+            error = er[0];
+            u_max = er[1];
+        }
         u_ave = dfull.sum_speed / (vfull * porosity);
         if (u_max >= 10.0) U_max_ref += 1;
         double Perm[3];

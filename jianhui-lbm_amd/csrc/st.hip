@@ -14,6 +14,7 @@
 //
 // Data layout (row stride P): pops[2] 7 x P, u 3 x P (the flow handle's when coupled),
 // c_init P (the concentration of an equilibrium start).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -99,12 +100,14 @@ __global__ void __launch_bounds__(BLOCK) st_step_kernel(StArgs<C> a)
 }
 
 // post-streaming populations and C in the reference's layout (g[count][7], C[count])
+// of the nodes n0 <= n < n1, at local index n - n0
 template <typename C, int MODE>
-__global__ void __launch_bounds__(BLOCK) st_get_state_kernel(StArgs<C> a, double *__restrict__ g_out,
-                                                             double *__restrict__ c_out)
+__global__ void __launch_bounds__(BLOCK) st_get_state_kernel(StArgs<C> a, int64_t n0, int64_t n1,
+                                                             double *__restrict__ g_out, double *__restrict__ c_out)
 {
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
+    const int64_t l = n - n0;
     C g[Q7];
     if (MODE == K_INIT) {
         const C c = a.cinit[n], wc = a.wq * c;
@@ -118,19 +121,20 @@ __global__ void __launch_bounds__(BLOCK) st_get_state_kernel(StArgs<C> a, double
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < Q7; ++q) {
-        if (g_out) g_out[n * Q7 + q] = (double)g[q];
+        if (g_out) g_out[l * Q7 + q] = (double)g[q];
         s += (double)g[q];
     }
-    if (c_out) c_out[n] = s;
+    if (c_out) c_out[l] = s;
 }
 
 template <typename C>
-__global__ void __launch_bounds__(BLOCK) st_load_state_kernel(StArgs<C> a, const double *__restrict__ g_in)
+__global__ void __launch_bounds__(BLOCK) st_load_state_kernel(StArgs<C> a, int64_t n0, int64_t n1,
+                                                              const double *__restrict__ g_in)
 {
-    const int64_t n = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= a.N) return;
+    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= n1) return;
 #pragma unroll
-    for (int q = 0; q < Q7; ++q) a.dst[(int64_t)q * a.P + n] = (C)g_in[n * Q7 + q];
+    for (int q = 0; q < Q7; ++q) a.dst[(int64_t)q * a.P + n] = (C)g_in[(n - n0) * Q7 + q];
 }
 
 // mass, min, max, first and second moments, non-finite count (FP64, fixed reduction order)
@@ -230,6 +234,7 @@ void StSolver::release()
     for (void **b : bufs)
         if (*b) (void)hipFree(*b), *b = nullptr;
     if (h_result_) (void)hipHostFree(h_result_), h_result_ = nullptr;
+    stage_.release();
     if (ev0_) (void)hipEventDestroy(ev0_), ev0_ = nullptr;
     if (ev1_) (void)hipEventDestroy(ev1_), ev1_ = nullptr;
     if (own_stream_) (void)hipStreamDestroy(own_stream_), own_stream_ = nullptr;
@@ -372,22 +377,26 @@ int StSolver::load_state(const double *g)
 {
     if (!g) return fail(LBM_ERR_ARG, "null state");
     LBM_HIP_TRY(hipSetDevice(device_));
-    double *dg = nullptr;
-    LBM_HIP_TRY(hipMalloc(&dg, (size_t)N_ * Q7 * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(dg, g, (size_t)N_ * Q7 * sizeof(double), hipMemcpyHostToDevice, stream_);
-    if (e == hipSuccess) {
-        dim3 gr((unsigned)grid_), b(BLOCK);
+    LBM_HIP_TRY(stage_.ensure());
+    const int64_t chunk = stage_chunk(STAGE_BYTES, Q7);
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N_ && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N_ - n0);
+        double *dg = (double *)stage_.buf;
+        e = hipMemcpyAsync(dg, g + n0 * Q7, (size_t)c * Q7 * sizeof(double), hipMemcpyHostToDevice, stream_);
+        if (e != hipSuccess) break;
+        dim3 gr((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (st_bytes_ == 8)
             st_load_state_kernel<double><<<gr, b, 0, stream_>>>(
-                make_st_args<double>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), dg);
+                make_st_args<double>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), n0, n0 + c, dg);
         else
             st_load_state_kernel<float><<<gr, b, 0, stream_>>>(
-                make_st_args<float>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), dg);
+                make_st_args<float>(nullptr, pops_[cur_], nbr_, nullptr, nullptr, N_, P_, prm_), n0, n0 + c, dg);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(dg);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     mode_ = STATE_DIRECT;
     steps_ = 0;
     return LBM_OK;
@@ -450,31 +459,39 @@ int StSolver::step(int nsteps, double *elapsed_ms)
 int StSolver::get_state(double *g, double *conc)
 {
     if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    if (!g && !conc) return LBM_OK;
     LBM_HIP_TRY(hipSetDevice(device_));
+    LBM_HIP_TRY(stage_.ensure());
     const void *u = velocity();
-    double *dg = nullptr, *dc = nullptr;
-    if (g) LBM_HIP_TRY(hipMalloc(&dg, (size_t)N_ * Q7 * sizeof(double)));
-    if (conc) LBM_HIP_TRY(hipMalloc(&dc, (size_t)N_ * sizeof(double)));
-    dim3 gr((unsigned)grid_), b(BLOCK);
     const int mode = st_kmode(mode_);
-    if (st_bytes_ == 8) {
-        auto a = make_st_args<double>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
-        if (mode == K_INIT) st_get_state_kernel<double, K_INIT><<<gr, b, 0, stream_>>>(a, dg, dc);
-        else if (mode == K_DIRECT) st_get_state_kernel<double, K_DIRECT><<<gr, b, 0, stream_>>>(a, dg, dc);
-        else st_get_state_kernel<double, K_PULL><<<gr, b, 0, stream_>>>(a, dg, dc);
-    } else {
-        auto a = make_st_args<float>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
-        if (mode == K_INIT) st_get_state_kernel<float, K_INIT><<<gr, b, 0, stream_>>>(a, dg, dc);
-        else if (mode == K_DIRECT) st_get_state_kernel<float, K_DIRECT><<<gr, b, 0, stream_>>>(a, dg, dc);
-        else st_get_state_kernel<float, K_PULL><<<gr, b, 0, stream_>>>(a, dg, dc);
+    const int64_t chunk = stage_chunk(STAGE_BYTES, (g ? Q7 : 0) + (conc ? 1 : 0));
+    hipError_t e = hipSuccess;
+    for (int64_t n0 = 0; n0 < N_ && e == hipSuccess; n0 += chunk) {
+        const int64_t c = std::min(chunk, N_ - n0);
+        double *dg = g ? (double *)stage_.buf : nullptr;
+        double *dc = conc ? (double *)stage_.buf + (g ? c * Q7 : 0) : nullptr;
+        dim3 gr((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
+        const int64_t n1 = n0 + c;
+        if (st_bytes_ == 8) {
+            auto a = make_st_args<double>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+            if (mode == K_INIT) st_get_state_kernel<double, K_INIT><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+            else if (mode == K_DIRECT) st_get_state_kernel<double, K_DIRECT><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+            else st_get_state_kernel<double, K_PULL><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+        } else {
+            auto a = make_st_args<float>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
+            if (mode == K_INIT) st_get_state_kernel<float, K_INIT><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+            else if (mode == K_DIRECT) st_get_state_kernel<float, K_DIRECT><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+            else st_get_state_kernel<float, K_PULL><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess && g)
+            e = hipMemcpyAsync(g + n0 * Q7, dg, (size_t)c * Q7 * sizeof(double), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess && conc)
+            e = hipMemcpyAsync(conc + n0, dc, (size_t)c * sizeof(double), hipMemcpyDeviceToHost, stream_);
     }
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess && g) e = hipMemcpyAsync(g, dg, (size_t)N_ * Q7 * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess && conc) e = hipMemcpyAsync(conc, dc, (size_t)N_ * sizeof(double), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    if (dg) (void)hipFree(dg);
-    if (dc) (void)hipFree(dc);
+    const hipError_t es = hipStreamSynchronize(stream_);
     LBM_HIP_TRY(e);
+    LBM_HIP_TRY(es);
     return LBM_OK;
 }
 

@@ -13,7 +13,21 @@ from typing import List, Sequence
 
 import numpy as np
 
-from . import _capi
+from . import _capi, geometry
+
+
+def request_stream(nreq: int = 32, rate: float = 200.0, seed: int = 7):
+    """The serving workload's seeded request stream: edge lengths uniform in 48..160 lu,
+    porosities uniform in 0.15..0.45, thresholded Gaussian random fields of correlation length
+    6 lu (request i from seed 1000 seed + i), Poisson arrivals at ``rate`` per second.
+    Returns (samples, arrival times in seconds from the first, edges, porosities)."""
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(48, 161, nreq)
+    pors = rng.uniform(0.15, 0.45, nreq)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    samples = [geometry.grf_sample(int(e), 6.0, float(p), seed=seed * 1000 + i) for i, (e, p) in enumerate(zip(edges, pors))]
+    return samples, arrivals, edges, pors
 
 
 class PermeabilityServer:

@@ -153,6 +153,14 @@ class SinglePhase:
         _capi.check(_capi.lib().lbm_sp_get_state(self._h, _dp(f), _dp(u), _dp(rho)), self._h)
         return f, u, rho
 
+    def dense(self):
+        """(u[nx,ny,nz,3], rho[nx,ny,nz]) on the box, solid voxels u = 0 and rho = 1: the arrays
+        the reference's field writers gather (output_velocity_b SP:4265-4390)."""
+        u = np.empty(self.shape + (3,))
+        rho = np.empty(self.shape)
+        _capi.check(_capi.lib().lbm_sp_get_dense(self._h, _dp(u), _dp(rho)), self._h)
+        return u, rho
+
     def velocity(self):
         u = np.empty((self.count, 3))
         _capi.check(_capi.lib().lbm_sp_get_state(self._h, None, _dp(u), None), self._h)
@@ -173,11 +181,14 @@ class SinglePhase:
             return int(np.prod(self.shape))
         return (box[1] - box[0] + 1) * (box[3] - box[2] + 1) * (box[5] - box[4] + 1)
 
-    def Error(self, porosity: Optional[float] = None):
-        """(error, u_max, u_ave) as Error() returns them (SP:3625-3704)."""
-        d = self.diagnostics()
+    def Error(self, porosity: Optional[float] = None, ranks: int = 1):
+        """(error, u_max, u_ave) as Error() returns them (SP:3625-3704) in a run of the reference
+        on ``ranks`` MPI ranks: each rank skips its own last node and the largest per-rank
+        error and u_max are taken (the reference's one-rank run aborts; it runs on 2 or more)."""
+        out = np.zeros(3)
+        _capi.check(_capi.lib().lbm_sp_error_ranks(self._h, int(ranks), _dp(out)), self._h)
         por = self.porosity if porosity is None else porosity
-        return d["err"], d["umax"], d["sum_speed"] / (self.box_volume() * por)
+        return float(out[0]), float(out[1]), float(out[2]) / (self.box_volume() * por)
 
     def Comput_Perm(self, PerDir: int = 1, box: Optional[Sequence[int]] = None, dp: float = 0.0):
         """(k[3], relative change of k[PerDir-1], u_ave) as Comput_Perm (SP:4565-4712).
@@ -314,6 +325,12 @@ class ColorGradient:
         _capi.check(_capi.lib().lbm_cg_steps_done(self._h, ctypes.byref(s)), self._h)
         return int(s.value)
 
+    def colour(self):
+        """(rho_r[count], rho_b[count]) after the last colour streaming."""
+        rr, rb = np.empty(self.count), np.empty(self.count)
+        _capi.check(_capi.lib().lbm_cg_get_state(self._h, None, None, None, None, _dp(rr), _dp(rb)), self._h)
+        return rr, rb
+
     def state(self):
         """(f[count,19] CG ordering, u[count,3], rho, psi, rho_r, rho_b)."""
         f = np.empty((self.count, 19))
@@ -435,6 +452,12 @@ class Solute:
         ms = ctypes.c_double()
         _capi.check(_capi.lib().lbm_st_step_timed(self._h, int(nsteps), ctypes.byref(ms)), self._h)
         return ms.value
+
+    def concentration(self):
+        """C[count] post-streaming (the populations stay on the device)."""
+        c = np.empty(self.count)
+        _capi.check(_capi.lib().lbm_st_get_state(self._h, None, _dp(c)), self._h)
+        return c
 
     def state(self):
         """(g[count,7], C[count]) post-streaming."""

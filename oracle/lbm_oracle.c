@@ -469,21 +469,68 @@ int orc_sp_run_ex(int nx, int ny, int nz, int count, const int32_t *idx, const i
     return 0;
 }
 
+/* Error over the reference's ranks (SP:3625-3704): each rank loops its own nodes i = 1..Count-1
+ * and so skips its own last node (SP:3651); it forms its own err_r = sqrt(t1)/(sqrt(t2)+1e-30)
+ * (SP:3666-3669); rank 0 takes the largest err_r and the largest per-rank u_max and sums the
+ * per-rank speed sums (SP:3677-3685).  rank_end[r] is the exclusive end of rank r's node range
+ * in the global 0-based node order (ranks own consecutive x-slabs, so consecutive node ids);
+ * u and u0 carry the padding row 0, node n at row n + 1. */
+double orc_sp_error_ranks(int nranks, const int64_t *rank_end, const double *u, const double *u0, double *umax,
+                          double *sum_speed)
+{
+    double err = 0, um_all = 0, t3_all = 0;
+    int64_t lo = 0;
+    for (int r = 0; r < nranks; r++) {
+        const int64_t hi = rank_end[r];
+        double t1 = 0, t2 = 0, t3 = 0, um = 0;
+        for (int64_t n = lo + 1; n < hi; n++) {   /* rows lo+1 .. hi-1: the rank's nodes but its last */
+            const double *a = u + n * 3, *b = u0 + n * 3;
+            t1 += (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+            const double s2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+            t2 += s2;
+            t3 += sqrt(s2);
+            if (s2 > um) um = s2;
+        }
+        const double e = sqrt(t1) / (sqrt(t2) + 1e-30);
+        um = sqrt(um);
+        if (r == 0 || e > err) err = e;
+        if (um > um_all) um_all = um;
+        t3_all += t3;
+        lo = hi;
+    }
+    if (umax) *umax = um_all;
+    if (sum_speed) *sum_speed = t3_all;
+    return err;
+}
+
 double orc_sp_error(int count, const double *u, const double *u0, double *umax, double *sum_speed)
 {
-    /* the loop stops at count-1: the last node is skipped (SP:3651) */
-    double t1 = 0, t2 = 0, t3 = 0, um = 0;
-    for (int n = 1; n < count; n++) {
-        const double *a = u + n * 3, *b = u0 + n * 3;
-        t1 += (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
-        double s2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-        t2 += s2;
-        t3 += sqrt(s2);
-        if (s2 > um) um = s2;
+    /* one rank: the loop stops at count-1, the last node is skipped (SP:3651) */
+    const int64_t end = count;
+    return orc_sp_error_ranks(1, &end, u, u0, umax, sum_speed);
+}
+
+/* The reference's x-slab split over ranks (Parallelize_Geometry, SP:1085-1108): with
+ * loc_por[i] the pore count of x-plane i and ave = total/nranks, the cut i+1 is placed where the
+ * running plane sum first reaches ave*(i+1), before or after the plane that crosses it,
+ * whichever sum is closer; the last rank takes the remaining planes.  disp[r] = first plane of
+ * rank r (nranks entries). */
+void orc_slab_split(int nx, const int32_t *loc_por, int nranks, int32_t *disp)
+{
+    int64_t total = 0;
+    for (int i = 0; i < nx; i++) total += loc_por[i];
+    const double ave = (double)total / nranks;
+    disp[0] = 0;
+    for (int i = 0; i < nranks - 1; i++) {
+        int64_t pre = 0, aft = 0;
+        int n_i = 0;
+        while (aft < ave * (i + 1)) {
+            pre = aft;
+            aft += loc_por[n_i];
+            n_i += 1;
+        }
+        disp[i + 1] = ((double)(aft - ave * (i + 1)) > (double)(ave * (i + 1) - pre)) ? n_i - 1 : n_i;
     }
-    if (umax) *umax = sqrt(um);
-    if (sum_speed) *sum_speed = t3;
-    return sqrt(t1) / (sqrt(t2) + 1e-30);
 }
 
 static int in_box(int s, int ny, int nz, const int *box)

@@ -70,6 +70,11 @@ void orc_sp_run(int nx, int ny, int nz, int count, const int32_t *idx, const int
                 double *f, double *F, int nsteps, int nthreads);
 /* velocity-change error (SP:3625-3704): returns err; *umax and *sum_speed as there */
 double orc_sp_error(int count, const double *u, const double *u0, double *umax, double *sum_speed);
+/* the same over the reference's ranks: rank r owns nodes [rank_end[r-1], rank_end[r]) */
+double orc_sp_error_ranks(int nranks, const int64_t *rank_end, const double *u, const double *u0, double *umax,
+                          double *sum_speed);
+/* the reference's x-slab split (SP:1085-1108): first x-plane of each rank from the plane pore counts */
+void orc_slab_split(int nx, const int32_t *loc_por, int nranks, int32_t *disp);
 /* flux sums for the permeability (SP:4617-4645). box = {xn,xp,yn,yp,zn,zp} or NULL */
 void orc_flux(int ny, int nz, int count, const int32_t *supinv, const double *u, const int *box,
               double Q[3]);

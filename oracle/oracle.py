@@ -37,6 +37,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.orc_number_fluid.restype = ctypes.c_int
         L.orc_sp_error.restype = ctypes.c_double
+        L.orc_sp_error_ranks.restype = ctypes.c_double
         L.orc_cg_wet_count.restype = ctypes.c_double
         L.orc_invert19.restype = ctypes.c_int
         L.orc_st_mass.restype = ctypes.c_double
@@ -109,6 +110,36 @@ def number_fluid(solid: np.ndarray):
     return count, idx, supinv
 
 
+def slab_split(solid: np.ndarray, ranks: int) -> np.ndarray:
+    """First x-plane of each of ``ranks`` ranks under the reference's split (SP:1085-1108)."""
+    loc_por = np.ascontiguousarray((np.asarray(solid) == 0).reshape(solid.shape[0], -1).sum(axis=1), np.int32)
+    disp = np.zeros(int(ranks), np.int32)
+    lib().orc_slab_split(int(solid.shape[0]), _i(loc_por), int(ranks), _i(disp))
+    return disp
+
+
+def rank_ends(solid: np.ndarray, ranks: int) -> np.ndarray:
+    """Exclusive end of each rank's range of the global (0-based, i-major) node numbering."""
+    disp = slab_split(solid, ranks)
+    per_plane = (np.asarray(solid) == 0).reshape(solid.shape[0], -1).sum(axis=1)
+    first = np.concatenate([[0], np.cumsum(per_plane)])        # node id of each plane's first node
+    bounds = list(disp[1:]) + [solid.shape[0]]
+    return np.array([first[b] for b in bounds], np.int64)
+
+
+def error_ranks(u: np.ndarray, u0: np.ndarray, ends) -> tuple:
+    """Error (SP:3625-3704) over ranks owning the node ranges ending at ``ends``; u and u0 carry
+    the padding row 0 (the oracle's layout)."""
+    u = np.ascontiguousarray(u, np.float64)
+    u0 = np.ascontiguousarray(u0, np.float64)
+    ends = np.ascontiguousarray(ends, np.int64)
+    umax = ctypes.c_double()
+    ssum = ctypes.c_double()
+    err = lib().orc_sp_error_ranks(int(ends.size), ends.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _d(u), _d(u0),
+                                   ctypes.byref(umax), ctypes.byref(ssum))
+    return err, umax.value, ssum.value
+
+
 def sp_relaxation(nu: float):
     S = np.zeros(19)
     sv = ctypes.c_double()
@@ -170,11 +201,11 @@ class SPOracle:
                      int(nsteps), self.nthreads)
         self.steps += nsteps
 
-    def error(self):
-        umax = ctypes.c_double()
-        ssum = ctypes.c_double()
-        err = lib().orc_sp_error(self.count, _d(self.u), _d(self.u0), ctypes.byref(umax), ctypes.byref(ssum))
-        return err, umax.value, ssum.value
+    def error(self, ranks: int = 1):
+        """(err, u_max, sum of |u|) of Error (SP:3625-3704) as the reference computes it on
+        ``ranks`` MPI ranks: each rank skips its own last node, err and u_max are the largest
+        per-rank values, the speed sums add up."""
+        return error_ranks(self.u, self.u0, rank_ends(self.solid, ranks))
 
     def flux(self, box=None):
         Q = np.zeros(3)
@@ -235,11 +266,9 @@ class CGOracle:
                            _d(self.psi))
             self.steps += 1
 
-    def error(self):
-        umax = ctypes.c_double()
-        ssum = ctypes.c_double()
-        err = lib().orc_sp_error(self.count, _d(self.u), _d(self.u0), ctypes.byref(umax), ctypes.byref(ssum))
-        return err, umax.value, ssum.value
+    def error(self, ranks: int = 1):
+        """Error (CG:4979-5040, the single-phase routine's loop and rank reduction) on ``ranks`` ranks."""
+        return error_ranks(self.u, self.u0, rank_ends(self.solid, ranks))
 
     def flux(self, box=None):
         Ql = np.zeros(3)

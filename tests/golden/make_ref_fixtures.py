@@ -20,9 +20,11 @@ from __future__ import annotations
 import json
 import os
 import shutil
+import signal
 import subprocess
 import sys
 import tempfile
+import time
 
 import numpy as np
 
@@ -50,6 +52,37 @@ def _run(binary, workdir, input_name="INPUT.dat", ranks=2):
     return proc.stdout
 
 
+def _run_until_quiet(binary, workdir, need, ranks=2, limit=120.0, quiet=3.0):
+    """Run the reference until the files ``need`` exist and no file in workdir has changed for
+    ``quiet`` seconds, then end the process group this call started.  For the high-velocity
+    stop (SP:839): its counter U_max_ref is raised on rank 0 only, from the maximum gathered
+    there (SP:712), so rank 0 alone leaves the time loop, writes its checkpoint (SP:843) and
+    finishes, while the other ranks wait in the next step's exchange and never return."""
+    env = dict(os.environ)
+    env["PATH"] = MPI_BIN + ":" + env.get("PATH", "")
+    proc = subprocess.Popen(["mpirun", "-np", str(ranks), binary, "INPUT.dat"], cwd=workdir, env=env,
+                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, start_new_session=True)
+    t0, last, last_change = time.time(), None, time.time()
+    try:
+        while time.time() - t0 < limit:
+            if proc.poll() is not None:
+                break
+            state = sorted((f, os.path.getsize(os.path.join(workdir, f))) for f in os.listdir(workdir))
+            if state != last:
+                last, last_change = state, time.time()
+            if all(os.path.exists(os.path.join(workdir, f)) for f in need) and time.time() - last_change > quiet:
+                break
+            time.sleep(0.2)
+    finally:
+        if proc.poll() is None:
+            os.killpg(proc.pid, signal.SIGKILL)
+        out = proc.communicate()[0]
+    missing = [f for f in need if not os.path.exists(os.path.join(workdir, f))]
+    if missing:
+        raise RuntimeError(f"reference run in {workdir} did not write {missing}:\n{out[-3000:]}")
+    return out
+
+
 def _read_ckpt(workdir, name, label, width, ranks=2):
     parts = []
     for rank in range(ranks):
@@ -58,6 +91,12 @@ def _read_ckpt(workdir, name, label, width, ranks=2):
         a = a.reshape(-1, width) if width > 1 else a
         parts.append(a[1:])       # record 0 is padding
     return np.concatenate(parts, axis=0)
+
+
+def _rank_counts(workdir, label, ranks=2):
+    """Fluid nodes of each rank: the record count of its density checkpoint, less the padding."""
+    return [int(np.fromfile(os.path.join(workdir, f"LBM_checkpoint_rho_{label}.{r}.bin_input"), "<f8").size - 1)
+            for r in range(ranks)]
 
 
 def _text(workdir, fname):
@@ -130,6 +169,7 @@ def sp_case(name, shape, radius, por, seed, nu, g, u0, steps_list, freRe=10, box
             out[f"f_{steps}"] = _read_ckpt(wd, "f", label, 19)
             out[f"u_{steps}"] = _read_ckpt(wd, "velocity", label, 3)
             out[f"rho_{steps}"] = _read_ckpt(wd, "rho", label, 1)
+            params["rank_counts"] = _rank_counts(wd, label)
             out[f"perm_txt_{steps}"] = np.array(_text(wd, "Permeability_error_local_Perm.txt"))
             out[f"vel_txt_{steps}"] = np.array(_text(wd, "Velocity_ave_max.txt"))
             out[f"general_txt_{steps}"] = np.array(_text(wd, "General_Results.txt"))
@@ -276,6 +316,96 @@ def st_case(name, sp_fixture, niu_s, u_scale, c_seed, n_max, labels, freRe=5):
     print("wrote", path, "count =", u.shape[0], "ranks", counts)
 
 
+def _write_sp(wd, solid, **kw):
+    geometry.write_binary(os.path.join(wd, "geometry.bin"), solid)
+    base = dict(filename="geometry.bin", n_xyz=tuple(solid.shape), reso=1.0, ini_v=(0.0, 0.0, 0.0), wr_per=1,
+                PerDir=1, Out_Mode=0, outputfile="./", fre_backup=0, decbin=1)
+    base.update(kw)
+    inputs.write_sp_input(inputs.SPInput(**base), os.path.join(wd, "INPUT.dat"))
+
+
+def stop_case(name, kind):
+    """The time loop's three stop rules (SP:812-816, 838-839), each on a reference run.
+
+    converged      Out_Mode = 1: the run ends when Comput_Perm's mean speed changes by less than
+                   1e-7 relative between two checks; the velocity files are written for that
+                   step and the step counter jumps to n_max + 1 (SP:812-816).
+    high_velocity  a force that drives a box without solid past u_max = 10: after the fifth check
+                   at or above it the run stops (SP:712, 839).  Uniform flow along x keeps every
+                   population that crosses a rank's x-face positive, so the reference's ranks
+                   stay exact (its exchange keeps only positive values, SP:1993-1998).  Only
+                   rank 0 stops; the fixture keeps its files (see _run_until_quiet).
+    nan            a restart from a checkpoint with one non-finite node inside each rank's slab:
+                   Error's velocity change is NaN at the first check and the run stops (SP:838).
+    """
+    wd = tempfile.mkdtemp(prefix="sp_stop_")
+    out = {}
+    try:
+        if kind == "converged":
+            solid = geometry.sphere_pack((16, 12, 8), 3.0, 0.6, 7)
+            params = dict(kind=kind, shape=list(solid.shape), nu=0.1, g=[1e-5, 0.0, 0.0], freRe=10, n_max=20000,
+                          Out_Mode=1)
+            _write_sp(wd, solid, n_max=20000, g=(1e-5, 0.0, 0.0), in_vis=0.1, freRe=10, Out_Mode=1)
+            _run(os.path.join(REF_DIR, "sp_ref"), wd)
+            vel = [f for f in os.listdir(wd) if f.startswith("LBM_velocity_Vector_")]
+            assert len(vel) == 1, vel
+            n_stop = int(vel[0][len("LBM_velocity_Vector_"):-4])
+            params["n_stop"] = n_stop
+            out["vtk_velocity"] = np.array(_text(wd, vel[0]))
+            out["profile"] = np.array(_text(wd, f"LBM_velocity_{n_stop}.out"))
+            label, ranks = 20000, 2
+        elif kind == "high_velocity":
+            solid = np.zeros((8, 6, 4), np.int32)
+            params = dict(kind=kind, shape=list(solid.shape), nu=0.1, g=[0.25, 0.0, 0.0], freRe=1, n_max=80)
+            _write_sp(wd, solid, n_max=80, g=(0.25, 0.0, 0.0), in_vis=0.1, freRe=1)
+            label, ranks = 80, 1   # rank 0's files only
+            _run_until_quiet(os.path.join(REF_DIR, "sp_ref"), wd,
+                             [f"LBM_checkpoint_{k}_{label}.0.bin_input" for k in ("f", "velocity", "rho")])
+        else:
+            solid = geometry.sphere_pack((16, 12, 8), 3.0, 0.6, 7)
+            params = dict(kind=kind, shape=list(solid.shape), nu=0.1, g=[1e-5, 0.0, 0.0], freRe=10, n_max=30,
+                          start_label=9)
+            _write_sp(wd, solid, n_max=9, g=(1e-5, 0.0, 0.0), in_vis=0.1, freRe=10)
+            _run(os.path.join(REF_DIR, "sp_ref"), wd)
+            counts = _rank_counts(wd, 9)
+            # one node in the middle x-plane of each rank's slab gets non-finite populations
+            start = {k: _read_ckpt(wd, k, 9, w) for k, w in (("f", 19), ("velocity", 3), ("rho", 1))}
+            planes = np.flatnonzero(solid.ravel() == 0) // (solid.shape[1] * solid.shape[2])
+            lo = 0
+            for r, c in enumerate(counts):
+                mid = (planes[lo] + planes[lo + c - 1]) // 2
+                node = lo + int(np.flatnonzero(planes[lo:lo + c] == mid)[0])
+                start["f"][node, :] = np.nan
+                lo += c
+            lo = 0
+            for r, c in enumerate(counts):
+                for k, w in (("f", 19), ("velocity", 3), ("rho", 1)):
+                    a = start[k][lo:lo + c].reshape(c, w)
+                    np.concatenate([np.zeros((1, w)), a]).tofile(os.path.join(wd, f"LBM_checkpoint_{k}_9.{r}.bin_input"))
+                lo += c
+            for f in os.listdir(wd):
+                if f.endswith(".txt"):
+                    os.remove(os.path.join(wd, f))
+            out.update({"f_start": start["f"], "u_start": start["velocity"], "rho_start": start["rho"]})
+            _write_sp(wd, solid, n_max=30, g=(1e-5, 0.0, 0.0), in_vis=0.1, freRe=10, mode_backup_ini=9)
+            _run(os.path.join(REF_DIR, "sp_ref"), wd)
+            label, ranks = 30, 2
+        out["solid"] = solid.astype(np.int8)
+        out["f_end"] = _read_ckpt(wd, "f", label, 19, ranks)
+        out["u_end"] = _read_ckpt(wd, "velocity", label, 3, ranks)
+        out["rho_end"] = _read_ckpt(wd, "rho", label, 1, ranks)
+        params["label"] = label
+        params["rank_counts"] = _rank_counts(wd, label, ranks)
+        out["vel_txt"] = np.array(_text(wd, "Velocity_ave_max.txt"))
+        out["perm_txt"] = np.array(_text(wd, "Permeability_error_local_Perm.txt"))
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+    out["params"] = np.array(json.dumps(params))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, params)
+
+
 def c1_case(name, steps=2000, ranks=8):
     """The reference's own CPU case at full size: 64^3 spheres r = 6 to porosity 0.6 (seed
     12345, 157,141 fluid nodes), tau = 1, g = (1e-6, 0, 0), FP64, 2000 steps.  The state is too
@@ -291,6 +421,7 @@ def c1_case(name, steps=2000, ranks=8):
         inputs.write_sp_input(inp, os.path.join(wd, "INPUT.dat"))
         _run(os.path.join(REF_DIR, "sp_ref"), wd, ranks=ranks)
         f = _read_ckpt(wd, "f", steps - 1, 19, ranks)
+        counts = _rank_counts(wd, steps - 1, ranks)
         u = _read_ckpt(wd, "velocity", steps - 1, 3, ranks)
         rho = _read_ckpt(wd, "rho", steps - 1, 1, ranks)
         pick = np.arange(0, f.shape[0], 64)
@@ -299,7 +430,7 @@ def c1_case(name, steps=2000, ranks=8):
                "perm_txt": np.array(_text(wd, "Permeability_error_local_Perm.txt")),
                "params": np.array(json.dumps(dict(shape=[64, 64, 64], nu=1.0 / 6.0, g=[1e-6, 0.0, 0.0], u0=[0, 0, 0],
                                                   steps=steps, radius=6.0, porosity_target=0.6, seed=12345,
-                                                  count=int(f.shape[0]), ranks=ranks)))}
+                                                  count=int(f.shape[0]), ranks=ranks, rank_counts=counts)))}
     finally:
         shutil.rmtree(wd, ignore_errors=True)
     path = os.path.join(HERE, f"{name}.npz")
@@ -376,6 +507,10 @@ def main(only=()):
                                                         "c": [[0, 0, 0], [5, 2, 3], [1, 1, 1], [12, 4, 6]]})),
         ("sp_out_16x12x8", lambda n: sp_case(n, (16, 12, 8), 3.0, 0.6, 7, nu=0.1, g=(1e-5, 2e-6, -3e-6),
                                               u0=(1e-3, -2e-3, 5e-4), steps_list=[21], field_outputs=True)),
+        # the time loop's stop rules
+        ("sp_stop_converged_16x12x8", lambda n: stop_case(n, "converged")),
+        ("sp_stop_highvel_8x6x4", lambda n: stop_case(n, "high_velocity")),
+        ("sp_stop_nan_16x12x8", lambda n: stop_case(n, "nan")),
     ]
     for name, make in cases:
         if not only or name in only:

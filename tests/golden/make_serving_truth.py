@@ -22,11 +22,8 @@ SEED, NREQ = 7, 32
 
 
 def stream(pkg):
-    rng = np.random.default_rng(SEED)
-    edges = rng.integers(48, 161, NREQ)
-    pors = rng.uniform(0.15, 0.45, NREQ)
-    return [(int(e), float(p), pkg.geometry.grf_sample(int(e), 6.0, float(p), seed=SEED * 1000 + i))
-            for i, (e, p) in enumerate(zip(edges, pors))]
+    samples, _, edges, pors = pkg.request_stream(NREQ, 200.0, SEED)
+    return [(int(e), float(p), s) for e, p, s in zip(edges, pors, samples)]
 
 
 def main():

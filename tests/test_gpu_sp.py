@@ -242,3 +242,45 @@ def test_python_local_and_group_permeability_match_reference(pkg):
         assert len(rows) == len(ref)
         for (kk, por), (rk, rp) in zip(rows, ref):
             assert abs(kk * 1000.0 - rk) <= 2e-5 * abs(rk) + 1e-9 and abs(por - rp) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12"])
+def test_error_over_ranks_matches_reference_printed_values(pkg, oracle_mod, name):
+    """lbm_sp_error_ranks with the reference's 2 ranks gives the u_max and velocity change its
+    Velocity_ave_max.txt prints (6 digits), and the oracle's 2-rank Error to round-off."""
+    _log(f"[gpu] Error over ranks {name}")
+    s, d, p = _fixture_solver(pkg, name)
+    o = oracle_mod.SPOracle(d["solid"].astype(np.int32), p["nu"], p["g"], p["u0"])
+    last = p["steps"][-1]
+    rows = [[float(x) for x in ln.split()] for ln in str(d[f"vel_txt_{last}"]).strip().splitlines()]
+    for line_no, n in enumerate(range(0, last, p["freRe"])):
+        s.step(n + 1 - s.steps_done)
+        o.run(n + 1 - o.steps)
+        err, umax, _ = s.Error(ranks=2)
+        oerr, oumax, ossum = o.error(ranks=2)
+        assert abs(err - oerr) <= 1e-11 * oerr and abs(umax - oumax) <= 1e-13 * oumax
+        assert abs(umax - rows[line_no][1]) <= 6e-6 * rows[line_no][1]
+        assert abs(err - rows[line_no][2]) <= 6e-6 * rows[line_no][2]
+        # one rank is the plain diagnostics
+        dg = s.diagnostics()
+        e1, um1, _ = s.Error(ranks=1)
+        assert e1 == dg["err"] and um1 == dg["umax"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_error_closed_form_on_a_solid_free_box(pkg, precision):
+    """No solid, uniform body force: u_n = (n + 1/2) g at every node after n steps, so Error's
+    velocity change is 1/(n + 1/2) on any rank count and u_max = (n + 1/2)|g|."""
+    _log(f"[gpu] Error closed form {precision}")
+    solid = np.zeros((6, 5, 4), np.int32)
+    g = np.array([3e-4, -1e-4, 2e-4])
+    s = pkg.SinglePhase(solid, 0.1, g, precision=precision)
+    s.init()
+    tol = 1e-11 if precision == "fp64" else 1e-4   # FP32: u to ~1e-9 absolute
+    for n in (2, 3, 7, 20):
+        s.step(n - s.steps_done)
+        for ranks in (1, 2, 3):
+            err, umax, _ = s.Error(ranks=ranks)
+            assert abs(err - 1.0 / (n + 0.5)) < tol * 1.0 / (n + 0.5), (n, ranks, err)
+            assert abs(umax - (n + 0.5) * np.linalg.norm(g)) < tol * umax
+    s.close()

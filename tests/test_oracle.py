@@ -372,3 +372,82 @@ def test_sp_oracle_full_size_reference_case(oracle_mod):
     assert np.abs(o.u[1:][pick] - d["u_pick"]).max() < 2e-14
     np.testing.assert_allclose(o.f[1:].sum(0), d["f_sum"], rtol=1e-12)
     np.testing.assert_allclose(o.u[1:].sum(0), d["u_sum"], rtol=1e-8)
+
+
+# ----------------------------------------------------------------------------- Error over ranks
+RANK_FIXTURES = ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12", "sp_perdir3_12x10x16", "sp_bc_p1_16x12x10",
+                 "sp_bc_p3_16x12x10", "sp_nnf_16x12x8", "sp_locperm_box_20x10x12", "sp_gperm_20x10x12",
+                 "sp_c1_64cube"]
+
+
+@pytest.mark.parametrize("name", RANK_FIXTURES)
+def test_slab_split_matches_reference_rank_counts(oracle_mod, name):
+    """The reference's x-slab split (SP:1085-1108) gives each rank the node count that rank's
+    checkpoint file holds (2 ranks; 8 for the 64^3 case)."""
+    d, p = load_fixture(name)
+    counts = p["rank_counts"]
+    ends = oracle_mod.rank_ends(d["solid"].astype(np.int32), len(counts))
+    assert list(np.diff(np.concatenate([[0], ends]))) == counts
+
+
+def _vel_lines(text):
+    return [[float(x) for x in line.split()] for line in text.strip().splitlines() if line.strip()]
+
+
+def _bc_of(oracle_mod, p):
+    bc = p.get("bc")
+    return None if bc is None else oracle_mod.make_bc(bc["sub_bc"], bc.get("pre"), bc.get("vel"))
+
+
+@pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12", "sp_perdir3_12x10x16",
+                                  "sp_bc_p3_16x12x10", "sp_nnf_16x12x8"])
+def test_error_over_ranks_matches_reference_printed_values(oracle_mod, name):
+    """Velocity_ave_max.txt prints, every freRe steps, u_max and Error's velocity change from
+    the reference's 2 ranks (SP:781-782); both are reproduced from the oracle's u of the step
+    and of the step before with the 2-rank split, to the 6 digits printed.  The one-rank value
+    and a variant that forgets to skip each rank's last node are planted mistakes the printed
+    values must reject."""
+    d, p = load_fixture(name)
+    solid = d["solid"].astype(np.int32)
+    last = p["steps"][-1]
+    printed = _vel_lines(str(d[f"vel_txt_{last}"]))
+    o = oracle_mod.SPOracle(solid, p["nu"], p["g"], p["u0"], bc=_bc_of(oracle_mod, p),
+                            power_law=p.get("power_law"))
+    ends = oracle_mod.rank_ends(solid, 2)
+    assert list(np.diff(np.concatenate([[0], ends]))) == p["rank_counts"]
+    rejected_one_rank = rejected_no_skip = 0
+    for line_no, n in enumerate(range(0, last, p["freRe"])):
+        o.run(n + 1 - o.steps)          # loop step n ends with u after n + 1 steps (SP:662-684)
+        err, umax, _ = o.error(ranks=2)
+        row = printed[line_no]
+        assert abs(umax - row[1]) <= 6e-6 * abs(row[1]), (n, umax, row)
+        assert abs(err - row[2]) <= 6e-6 * abs(row[2]), (n, err, row)
+        # planted mistakes, on the CPU: one rank; every rank's last node included
+        err1, _, _ = o.error(ranks=1)
+        rejected_one_rank += abs(err1 - row[2]) > 6e-6 * abs(row[2])
+        u, u0 = o.u[1:], o.u0[1:]
+        bad = max(np.sqrt(((u[a:b] - u0[a:b]) ** 2).sum()) / (np.sqrt((u[a:b] ** 2).sum()) + 1e-30)
+                  for a, b in ((0, ends[0]), (ends[0], ends[1])))
+        rejected_no_skip += abs(bad - row[2]) > 6e-6 * abs(row[2])
+    assert len(printed) == line_no + 1
+    assert rejected_one_rank > 0 and rejected_no_skip > 0
+
+
+def test_error_closed_form_on_a_solid_free_box(oracle_mod):
+    """Body force on a periodic box with no solid: every node has u_n = (n + 1/2) g after n steps
+    (Guo forcing adds g to the momentum each step and the velocity carries half a step more),
+    so Error's velocity change is exactly 1/(n + 1/2) for n >= 2, on any rank count, and
+    u_max = (n + 1/2) |g|."""
+    solid = np.zeros((6, 5, 4), np.int32)
+    g = np.array([3e-4, -1e-4, 2e-4])
+    o = oracle_mod.SPOracle(solid, 0.1, g)
+    # u is a difference of populations ~0.05: FP64 round-off of ~1e-17 in u
+    for n in (2, 3, 7, 20):
+        o.run(n - o.steps)
+        np.testing.assert_allclose(o.u[1:], np.tile((n + 0.5) * g, (o.count, 1)), rtol=0, atol=1e-15)
+        for ranks in (1, 2, 3):
+            err, umax, ssum = o.error(ranks=ranks)
+            assert abs(err - 1.0 / (n + 0.5)) < 1e-11
+            assert abs(umax - (n + 0.5) * np.linalg.norm(g)) < 1e-11 * umax
+            # every rank skips one node
+            assert abs(ssum - (o.count - ranks) * (n + 0.5) * np.linalg.norm(g)) < 1e-11 * ssum
