@@ -307,6 +307,33 @@ int lbm_srv_get_stats(lbm_handle *h, lbm_srv_stats *out);
 /* seconds on the server clock (for latency accounting by callers) */
 double lbm_srv_clock(const lbm_handle *h);
 
+/* ------------------------------------------------------------------ launch settings
+ * How a handle launches its hot kernels: the workgroup size (nodes per tile), the order in which
+ * the tiles of consecutive nodes are dealt to the MI355X's 8 XCDs (each with its own L2), and,
+ * for the colour streaming, the variant.  No counterpart in the reference (a CPU program).  By
+ * default a handle chooses them itself: at create, on its own geometry, it times every candidate
+ * and keeps the fastest (the autotuner; choices are cached per kernel, precision and size class
+ * for the life of the process, and boxes under 2^20 fluid nodes keep the defaults).
+ * lbm_set_launch fixes a setting, e.g. to compare two; results are identical for every setting. */
+#define LBM_KERNEL_SP_STEP 1      /* single-phase step (sp_step_kernel) */
+#define LBM_KERNEL_CG_COLLIDE 2   /* colour-gradient collide (cg_collide_kernel) */
+#define LBM_KERNEL_CG_COLOUR 3    /* colour streaming (cg_colour_kernel) */
+#define LBM_KERNEL_ST_STEP 4      /* passive-solute step (st_step_kernel) */
+#define LBM_ORDER_HW 0            /* tiles in the hardware's round-robin order over the XCDs */
+#define LBM_ORDER_RUNS8 1         /* each XCD takes runs of 8 consecutive tiles */
+#define LBM_ORDER_SLAB 2          /* each XCD sweeps one contiguous slab of tiles */
+#define LBM_VARIANT_GATHER 0      /* colour streaming: 18 gathers per node through the caches */
+#define LBM_VARIANT_STAGED 1      /* colour streaming: the block's neighbourhood staged in LDS (FP32) */
+typedef struct lbm_launch {
+    int32_t block;     /* threads (nodes) per workgroup: 128, 256 or 512 */
+    int32_t order;     /* LBM_ORDER_* */
+    int32_t variant;   /* LBM_VARIANT_* (colour streaming; 0 for the other kernels) */
+    int32_t tuned;     /* 1: chosen by the autotuner on this geometry, 0: default or set */
+    double time_us;    /* the autotuner's time for one launch of the choice (0 if not tuned) */
+} lbm_launch;
+int lbm_get_launch(const lbm_handle *h, int32_t kernel, lbm_launch *out);
+int lbm_set_launch(lbm_handle *h, int32_t kernel, const lbm_launch *in);
+
 /* ------------------------------------------------------------------ common */
 const char *lbm_last_error(const lbm_handle *h);   /* NULL handle: last create() failure */
 void lbm_destroy(lbm_handle *h);

@@ -65,6 +65,31 @@ class StDiag(ctypes.Structure):
                 ("nan_count", ctypes.c_double)]
 
 
+class Launch(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("order", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("tuned", ctypes.c_int32), ("time_us", ctypes.c_double)]
+
+
+KERNELS = {"sp_step": 1, "cg_collide": 2, "cg_colour": 3, "st_step": 4}
+ORDERS = {0: "hw", 1: "runs8", 2: "slab"}
+VARIANTS = {0: "gather", 1: "staged"}
+
+
+def get_launch(h, kernel: str) -> dict:
+    ls = Launch()
+    check(lib().lbm_get_launch(h, KERNELS[kernel], ctypes.byref(ls)), h)
+    return {"block": ls.block, "order": ORDERS.get(ls.order, ls.order), "variant": VARIANTS.get(ls.variant, ls.variant),
+            "tuned": bool(ls.tuned), "time_us": round(ls.time_us, 2)}
+
+
+def set_launch(h, kernel: str, block: int = 256, order: str = "runs8", variant: str = "gather"):
+    ls = Launch()
+    ls.block = int(block)
+    ls.order = {v: k for k, v in ORDERS.items()}[order]
+    ls.variant = {v: k for k, v in VARIANTS.items()}[variant]
+    check(lib().lbm_set_launch(h, KERNELS[kernel], ctypes.byref(ls)), h)
+
+
 class SrvParams(ctypes.Structure):
     _fields_ = [("capacity_nodes", ctypes.c_int64), ("mem_fraction", ctypes.c_double),
                 ("check_interval", ctypes.c_int32), ("tol", ctypes.c_double), ("min_steps", ctypes.c_int32),
@@ -160,6 +185,8 @@ def lib():
         L.lbm_st_step_timed.argtypes = [vp, ctypes.c_int32, dp]
         L.lbm_st_get_state.argtypes = [vp, dp, dp]
         L.lbm_st_diagnostics.argtypes = [vp, ctypes.POINTER(StDiag)]
+        L.lbm_get_launch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Launch)]
+        L.lbm_set_launch.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Launch)]
         L.lbm_srv_create.argtypes = [ctypes.POINTER(SrvParams), ctypes.POINTER(vp)]
         L.lbm_srv_submit.argtypes = [vp, ctypes.POINTER(SrvRequest), ctypes.POINTER(ctypes.c_int64)]
         L.lbm_srv_poll.argtypes = [vp, ctypes.POINTER(SrvResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]

@@ -146,6 +146,22 @@ int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3])
     return s ? s->error_ranks(nranks, out) : LBM_ERR_ARG;
 }
 
+int lbm_get_launch(const lbm_handle *h, int32_t kernel, lbm_launch *out)
+{
+    if (auto *s = as_sp(h)) return s->get_launch(kernel, out);
+    if (auto *c = as_cg(h)) return c->get_launch(kernel, out);
+    if (auto *t = as_st(h)) return t->get_launch(kernel, out);
+    return LBM_ERR_ARG;
+}
+
+int lbm_set_launch(lbm_handle *h, int32_t kernel, const lbm_launch *in)
+{
+    if (auto *s = as_sp(h)) return s->set_launch(kernel, in);
+    if (auto *c = as_cg(h)) return c->set_launch(kernel, in);
+    if (auto *t = as_st(h)) return t->set_launch(kernel, in);
+    return LBM_ERR_ARG;
+}
+
 int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out)
 {
     auto *s = as_sp(h);

@@ -17,9 +17,10 @@
 // Data layout (row stride P):
 //   pops[2] 19 x P  deviations f - w rho0, canonical (SP) velocity order
 //   nbr     packed upstream node per direction or bounce back (nbrpack.h)
-//   col      2 x P  rho_r, rho_b after colour streaming (the reference's rho_r/rho_b)
+//   col      2 x P  rho_r, rho_b after colour streaming (the reference's rho_r/rho_b): doubles in
+//                   the FP64 build, int32 fixed point (units of 2^-29) in the FP32 build
 //   psi      1 x P  (rho_r - rho_b)/(rho_r + rho_b)
-//   cs       8 x P  per node, contiguous: rho_r, rho_b, ux, uy, uz, nx, ny, nz
+//   cs       8 x P  per node, contiguous: rho_r, rho_b (as in col), ux, uy, uz, nx, ny, nz
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -30,16 +31,46 @@
 #include "handle.h"
 #include "pull.h"
 #include "storage.h"
+#include "tune.h"
 
 namespace lbm {
+
+// Stored colour densities.  The FP32 build keeps rho_r and rho_b in fixed point, int32 units of
+// 2^-29 (values up to 4): the colour streaming then splits a node's colour into its 19 outgoing
+// shares w_q rho exactly in integers (cg_colour_kernel_fx), so each colour's total is kept to
+// the unit whatever the order of the sums, with FP32 arithmetic for the small terms.
+template <typename C>
+struct ColourStore {
+    using T = double;
+    static __host__ __device__ __forceinline__ double real(double v) { return v; }
+    static __host__ __device__ __forceinline__ double put(double v) { return v; }
+};
+template <>
+struct ColourStore<float> {
+    using T = int32_t;
+    static constexpr double ONE = 536870912.0;   // 2^29
+    static __host__ __device__ __forceinline__ double real(int32_t v) { return (double)v * (1.0 / ONE); }
+    static __host__ __device__ __forceinline__ int32_t put(double v) { return (int32_t)rint(v * ONE); }
+};
+
+// Per block of the LDS-staged colour kernel: up to CW_RUNS runs of consecutive node ids
+// [lo, lo + len) copied to LDS slots [off, off + len); unused runs have len 0.
+constexpr int CW_RUNS = 4;
+struct ColourWin {
+    int32_t lo[CW_RUNS];
+    int32_t len[CW_RUNS];
+    int32_t off[CW_RUNS];
+    int32_t pad[CW_RUNS];
+};
 
 template <typename ST>
 struct CgArgs {
     using C = typename Store<ST>::C;
+    using CT = typename ColourStore<C>::T;
     const ST *__restrict__ src;
     ST *__restrict__ dst;
     NbrDev nbr;
-    C *__restrict__ col;
+    CT *__restrict__ col;
     C *__restrict__ psi;
     C *__restrict__ cs;
     C *__restrict__ u_aux;
@@ -50,20 +81,19 @@ struct CgArgs {
     C ux_i, uy_i, uz_i;
     int32_t bfa;
     float scale, inv_scale;
+    int32_t order;   // tile order of the collide launch (TileOrder)
 };
 
-template <typename C>
-__device__ __forceinline__ void store_cs(C *cs, int64_t n, const C (&v)[8]);
-template <>
-__device__ __forceinline__ void store_cs<float>(float *cs, int64_t n, const float (&v)[8])
+// colour state of a node: its stored colour densities (raw) and six values of the compute type
+__device__ __forceinline__ void store_cs(float *cs, int64_t n, int32_t rr, int32_t rb, const float (&v)[6])
 {
     float4 *p = reinterpret_cast<float4 *>(cs + 8 * n);
-    p[0] = make_float4(v[0], v[1], v[2], v[3]);
-    p[1] = make_float4(v[4], v[5], v[6], v[7]);
+    p[0] = make_float4(__int_as_float(rr), __int_as_float(rb), v[0], v[1]);
+    p[1] = make_float4(v[2], v[3], v[4], v[5]);
 }
-template <>
-__device__ __forceinline__ void store_cs<double>(double *cs, int64_t n, const double (&v)[8])
+__device__ __forceinline__ void store_cs(double *cs, int64_t n, double rr, double rb, const double (&w)[6])
 {
+    const double v[8] = {rr, rb, w[0], w[1], w[2], w[3], w[4], w[5]};
     double2 *p = reinterpret_cast<double2 *>(cs + 8 * n);
     p[0] = make_double2(v[0], v[1]);
     p[1] = make_double2(v[2], v[3]);
@@ -131,17 +161,19 @@ __device__ __forceinline__ void cg_node_state(const CgArgs<ST> &a, int64_t n, ty
     }
 }
 
-template <typename ST, int MODE, bool STORE_U>
-__global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
+template <typename ST, int MODE, bool STORE_U, int NB = BLOCK>
+__global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = stream_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = tile_of(a.order, blockIdx.x, gridDim.x) * NB + threadIdx.x;
     if (n >= a.N) return;
     C h[Q], m[Q];
     uint32_t nb[Q - 1];
     Macro<C> mc;
     cg_node_state<ST, MODE>(a, n, h, m, nb, mc);
-    const C rr = a.col[n], rb = a.col[a.P + n], ps = a.psi[n];
+    using CS = ColourStore<C>;
+    const typename CS::T rr_raw = a.col[n], rb_raw = a.col[a.P + n];
+    const C rr = (C)CS::real(rr_raw), rb = (C)CS::real(rb_raw), ps = a.psi[n];
 
     // colour gradient C = sum_q 3 w_q e_q psi(x + e_q) (CG:2903-3012); x + e_q is the upstream
     // site of the opposite direction
@@ -189,8 +221,8 @@ __global__ void __launch_bounds__(BLOCK) cg_collide_kernel(CgArgs<ST> a)
     cg_relax(m, mc, gx, gy, gz, sv, a.cap_a * cc, nx, ny, nz, h);
 #pragma unroll
     for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&a.dst[(int64_t)q * a.P + n], h[q], a.scale);
-    const C st[8] = {rr, rb, mc.ux, mc.uy, mc.uz, nx, ny, nz};
-    store_cs(a.cs, n, st);
+    const C st[6] = {mc.ux, mc.uy, mc.uz, nx, ny, nz};
+    store_cs(a.cs, n, rr_raw, rb_raw, st);
 }
 
 // g_r, g_b leaving a node with colour state v in direction sgn*e_q (sgn = -1: the opposite
@@ -228,9 +260,10 @@ __device__ __forceinline__ void colour_pull_term(const int q, const C (&v)[8], C
 // the windows costing more than the L2 traffic they save.
 template <typename C>
 __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *__restrict__ cs,
-                                                          C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P)
+                                                          C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P,
+                                                          int32_t order)
 {
-    const int64_t n = xcd_slab_tile(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (n >= N) return;
     C own_s[8];
     load_cs(cs, n, own_s);
@@ -269,6 +302,161 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *_
     col[n] = (C)rr;
     col[P + n] = (C)rb;
     psi[n] = (C)((rr - rb) / (rr + rb));
+}
+
+// The FP32 build's colour streaming.  Each node's stored colour R (fixed point, units of 2^-29)
+// leaves as 19 shares: the bulk w_q R, in integers -- the nearest unit to R/18 along an axis, to
+// R/36 along a diagonal, and the remainder R - 6 R/18 - 12 R/36 on the rest population -- and
+// the small rest of g_q, w_q rho (phi_q - 1) plus the recolouring term, in FP32.  Every receiver
+// computes the bulk share of an upstream node from the same integer, so the bulk shares of a
+// node sum to its R exactly and each colour's total is conserved to the unit, in any summation
+// order; the FP32 small terms sum to zero up to their rounding (~1e-12 of the colour per
+// step).  The recolouring term of a pair of opposite directions comes out of the same
+// operations with the sign of e_q flipped, so its two shares cancel exactly.  The received
+// small terms are added to the integer sum rounded to the nearest unit.
+__device__ __forceinline__ int32_t colour_share(bool axis, int32_t R) { return axis ? (R + 9) / 18 : (R + 18) / 36; }
+
+// The colour sums of one node, direction by direction: begin() with the node's own colour state
+// (the rest population), add() the share arriving along q from an upstream state (bb: the
+// upstream site is solid and vs is the node's own state), then store().
+struct ColourSums {
+    static constexpr float UNIT = (float)(1.0 / ColourStore<float>::ONE);   // 2^-29, exact
+    static constexpr float ONE = (float)ColourStore<float>::ONE;
+    int32_t sr, sb;
+    float tr, tb;
+    __device__ __forceinline__ void begin(const float (&own)[8])
+    {
+        const int32_t R0 = __float_as_int(own[0]), B0 = __float_as_int(own[1]);
+        // rest population: the remainder of the bulk, and w0 rho (phi_0 - 1) = -1.5 w0 rho u.u
+        sr = R0 - 6 * colour_share(true, R0) - 12 * colour_share(false, R0);
+        sb = B0 - 6 * colour_share(true, B0) - 12 * colour_share(false, B0);
+        const float uu0 = own[2] * own[2] + own[3] * own[3] + own[4] * own[4];
+        const float c0 = (float)weight(0) * -1.5f * uu0;
+        tr = c0 * ((float)R0 * UNIT);
+        tb = c0 * ((float)B0 * UNIT);
+    }
+    template <int q>
+    __device__ __forceinline__ void add(bool bb, const float (&vs)[8])
+    {
+        constexpr bool axis = e2_of(q) == 1;
+        const int32_t R = __float_as_int(vs[0]), B = __float_as_int(vs[1]);
+        sr += colour_share(axis, R);
+        sb += colour_share(axis, B);
+        // a solid upstream site sends back this node's own opposite share: sign -1
+        const float sgn = bb ? -1.0f : 1.0f;
+        const float rr = (float)R * UNIT, rb = (float)B * UNIT;
+        const float ux = vs[2], uy = vs[3], uz = vs[4];
+        const float uu = ux * ux + uy * uy + uz * uz;
+        const float eu = sgn * ((float)EX[q] * ux + (float)EY[q] * uy + (float)EZ[q] * uz);
+        // phi_q - 1 formed from the small terms alone: rounding 1 + small and taking 1 off again
+        // would be biased, and would drift the colour totals
+        const float s2 = 4.5f * eu * eu - 1.5f * uu;
+        const float eps = s2 + 3.0f * eu;
+        const float phq = 1.0f + eps, php = 1.0f + (s2 - 3.0f * eu);
+        const float w = (float)weight(q);
+        float mn = w * rr * phq;
+        const float t1 = w * rr * php, t2 = w * rb * phq, t3 = w * rb * php;
+        mn = t1 < mn ? t1 : mn;
+        mn = t2 < mn ? t2 : mn;
+        mn = t3 < mn ? t3 : mn;
+        const float en = sgn * ((float)EX[q] * vs[5] + (float)EY[q] * vs[6] + (float)EZ[q] * vs[7]);
+        tr += w * rr * eps + mn * en;
+        tb += w * rb * eps - mn * en;
+    }
+    __device__ __forceinline__ void store(int64_t n, int64_t P, int32_t *__restrict__ col, float *__restrict__ psi) const
+    {
+        const int32_t Rn = sr + (int32_t)rintf(tr * ONE), Bn = sb + (int32_t)rintf(tb * ONE);
+        col[n] = Rn;
+        col[P + n] = Bn;
+        psi[n] = (float)(Rn - Bn) / (float)(Rn + Bn);
+    }
+};
+
+template <int q>
+struct ColourStep {
+    template <typename F>
+    static __device__ __forceinline__ void run(ColourSums &acc, const uint32_t (&src)[Q - 1], F &&fetch)
+    {
+        float v[8];
+        const uint32_t id = src[q - 1];
+        fetch(id, v);
+        acc.add<q>((id & BB_FLAG) != 0, v);
+        ColourStep<q + 1>::run(acc, src, fetch);
+    }
+};
+template <>
+struct ColourStep<Q> {
+    template <typename F>
+    static __device__ __forceinline__ void run(ColourSums &, const uint32_t (&)[Q - 1], F &&)
+    {
+    }
+};
+
+// gathers of the 18 upstream colour states from global memory (through the caches)
+__global__ void __launch_bounds__(BLOCK) cg_colour_kernel_fx(NbrDev nbr, const float *__restrict__ cs,
+                                                             int32_t *__restrict__ col, float *__restrict__ psi,
+                                                             int64_t N, int64_t P, int32_t order)
+{
+    const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    float own[8];
+    load_cs(cs, n, own);
+    uint32_t src[Q - 1];
+    load_nbr(nbr, P, n, src);
+    ColourSums acc;
+    acc.begin(own);
+    ColourStep<1>::run(acc, src, [&](uint32_t v, float (&out)[8]) { load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, out); });
+    acc.store(n, P, col, psi);
+}
+
+// The same with the colour states of the block's upstream neighbourhood staged in LDS first.
+// A block's 256 nodes lie on two or three z-lines of one x-plane; their upstream nodes are the
+// nodes of a few neighbouring lines in the planes x-1, x and x+1, i.e. a few runs of consecutive
+// node ids.  The host lists, per block, up to CW_RUNS such runs (ColourWin) that fit in SLOTS
+// states; the block copies them with coalesced loads, then every one of its 18 x 256 reads
+// comes from LDS, save the few upstream nodes outside the runs (across the periodic wrap of a
+// plane), which are read from global memory.  Each state is copied by the few blocks whose
+// neighbourhoods hold it, instead of being gathered by each of its 18 downstream nodes.
+template <int SLOTS>
+__global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const float *__restrict__ cs,
+                                                              const ColourWin *__restrict__ wins,
+                                                              int32_t *__restrict__ col, float *__restrict__ psi,
+                                                              int64_t N, int64_t P, int32_t order)
+{
+    __shared__ float4 lds[2 * SLOTS];
+    const int64_t b = tile_of(order, blockIdx.x, gridDim.x);
+    const ColourWin w = wins[b];
+    const float4 *__restrict__ cs4 = reinterpret_cast<const float4 *>(cs);
+#pragma unroll
+    for (int k = 0; k < CW_RUNS; ++k) {
+        const float4 *g = cs4 + 2 * (int64_t)w.lo[k];
+        float4 *l = lds + 2 * w.off[k];
+        for (int i = threadIdx.x; i < 2 * w.len[k]; i += BLOCK) l[i] = g[i];
+    }
+    __syncthreads();
+    const int64_t n = b * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    uint32_t src[Q - 1];
+    load_nbr(nbr, P, n, src);
+    auto fetch = [&](uint32_t id, float (&v)[8]) {
+        int slot = -1;
+#pragma unroll
+        for (int k = 0; k < CW_RUNS; ++k)
+            if (id - (uint32_t)w.lo[k] < (uint32_t)w.len[k]) slot = w.off[k] + (int)(id - (uint32_t)w.lo[k]);
+        if (slot >= 0) {
+            const float4 a = lds[2 * slot], c = lds[2 * slot + 1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
+        } else {
+            load_cs(cs, (int64_t)id, v);
+        }
+    };
+    float own[8];
+    fetch((uint32_t)n, own);
+    ColourSums acc;
+    acc.begin(own);
+    ColourStep<1>::run(acc, src, [&](uint32_t v, float (&out)[8]) { fetch((v & BB_FLAG) ? (uint32_t)n : v, out); });
+    acc.store(n, P, col, psi);
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -345,8 +533,8 @@ __global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, int64
                                                              double *__restrict__ rr_out, double *__restrict__ rb_out)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= n1) return;
+    const int64_t n = group_floor(n0) + (int64_t)blockIdx.x * BLOCK + threadIdx.x;   // see load_nbr
+    if (n < n0 || n >= n1) return;
     const int64_t l = n - n0;
     C h[Q], m[Q];
     uint32_t nb[Q - 1];
@@ -376,8 +564,8 @@ __global__ void __launch_bounds__(BLOCK) cg_get_state_kernel(CgArgs<ST> a, int64
     }
     if (rho_out) rho_out[l] = (double)mc.rho;
     if (psi_out) psi_out[l] = (double)a.psi[n];
-    if (rr_out) rr_out[l] = (double)a.col[n];
-    if (rb_out) rb_out[l] = (double)a.col[a.P + n];
+    if (rr_out) rr_out[l] = ColourStore<C>::real(a.col[n]);
+    if (rb_out) rb_out[l] = ColourStore<C>::real(a.col[a.P + n]);
 }
 
 // a chunk of a post-streaming state: nodes n0 <= n < n1 at local index n - n0 of the inputs
@@ -400,13 +588,14 @@ __global__ void __launch_bounds__(BLOCK) cg_load_state_kernel(CgArgs<ST> a, int6
     for (int c = 0; c < 3; ++c) a.u_aux[(int64_t)c * a.P + n] = (C)u[l * 3 + c];
     rho_aux[n] = (C)rho[l];
     a.psi[n] = (C)psi[l];
-    a.col[n] = (C)rr[l];
-    a.col[a.P + n] = (C)rb[l];
+    a.col[n] = ColourStore<C>::put(rr[l]);
+    a.col[a.P + n] = ColourStore<C>::put(rb[l]);
 }
 
 // phase of the nodes n0 <= n < n1, psi_node at local index n - n0
 template <typename C>
-__global__ void __launch_bounds__(BLOCK) cg_init_colour_kernel(const double *__restrict__ psi_node, C *col, C *psi,
+__global__ void __launch_bounds__(BLOCK) cg_init_colour_kernel(const double *__restrict__ psi_node,
+                                                               typename ColourStore<C>::T *col, C *psi,
                                                                int64_t n0, int64_t n1, int64_t P)
 {
     const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -415,8 +604,8 @@ __global__ void __launch_bounds__(BLOCK) cg_init_colour_kernel(const double *__r
     p = p > 1.0 ? 1.0 : (p < -1.0 ? -1.0 : p);
     const double rr = (p * 1.0 + 1.0) / 2;   // rho_r = (psi rho + rho)/2 with rho = 1 (CG:2531-2532)
     psi[n] = (C)p;
-    col[n] = (C)rr;
-    col[P + n] = (C)(1.0 - rr);
+    col[n] = ColourStore<C>::put(rr);
+    col[P + n] = ColourStore<C>::put(1.0 - rr);
 }
 
 // ---------------------------------------------------------------- host side
@@ -432,7 +621,7 @@ CgArgs<ST> make_cg_args(const lbm_cg_params &p, const double *visc, const double
     a.src = (const ST *)src;
     a.dst = (ST *)dst;
     a.nbr = nbr;
-    a.col = (C *)col;
+    a.col = (typename CgArgs<ST>::CT *)col;
     a.psi = (C *)psi;
     a.cs = (C *)cs;
     a.u_aux = (C *)u_aux;
@@ -461,12 +650,66 @@ CgArgs<ST> make_cg_args(const lbm_cg_params &p, const double *visc, const double
     a.bfa = p.bodyforce_apply;
     a.scale = 1.0f;
     a.inv_scale = 1.0f;
+    a.order = ORDER_RUNS8;
     return a;
 }
 
 int cg_kmode(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT ? K_DIRECT : K_PULL); }
 
 }  // namespace
+
+// The runs of node ids each block of the staged colour kernel copies to LDS: the block's own
+// nodes and their upstream nodes, sorted, split where consecutive ids are more than RUN_GAP apart
+// (ids in such a small gap are copied too), and the runs with the most reads taken first while
+// they fit in `slots` states and CW_RUNS runs.  Returns the fraction of all reads the runs serve.
+static double build_colour_windows(const SparseGeometry &g, int64_t grid, int slots, std::vector<ColourWin> &out)
+{
+    constexpr uint32_t RUN_GAP = 64;
+    out.assign((size_t)grid, ColourWin{});
+    int64_t served = 0, total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : served, total)
+    for (int64_t b = 0; b < grid; ++b) {
+        std::vector<uint32_t> ids;
+        ids.reserve((size_t)BLOCK * Q);
+        const int64_t n1 = std::min<int64_t>(g.N, (b + 1) * BLOCK);
+        for (int64_t n = b * BLOCK; n < n1; ++n) {
+            ids.push_back((uint32_t)n);
+            for (int q = 1; q < Q; ++q) {
+                const uint32_t v = g.nbr[(size_t)((q - 1) * g.P + n)];
+                ids.push_back((v & BB_FLAG) ? (uint32_t)n : v);
+            }
+        }
+        std::sort(ids.begin(), ids.end());
+        struct Run {
+            uint32_t lo, hi;
+            int64_t reads;
+        };
+        std::vector<Run> runs;
+        for (size_t i = 0; i < ids.size(); ++i) {
+            if (runs.empty() || ids[i] > runs.back().hi - 1 + RUN_GAP)
+                runs.push_back(Run{ids[i], ids[i] + 1, 0});
+            runs.back().hi = ids[i] + 1;
+            ++runs.back().reads;
+        }
+        std::sort(runs.begin(), runs.end(), [](const Run &x, const Run &y) { return x.reads > y.reads; });
+        ColourWin w{};
+        int used = 0, k = 0;
+        for (const Run &r : runs) {
+            const int len = (int)(r.hi - r.lo);
+            if (k < CW_RUNS && used + len <= slots) {
+                w.lo[k] = (int32_t)r.lo;
+                w.len[k] = len;
+                w.off[k] = used;
+                used += len;
+                ++k;
+                served += r.reads;
+            }
+            total += r.reads;
+        }
+        out[(size_t)b] = w;
+    }
+    return total ? (double)served / (double)total : 1.0;
+}
 
 CgSolver::~CgSolver() { release(); }
 
@@ -477,7 +720,7 @@ void CgSolver::release()
     for (void *&p : pops_)
         if (p) (void)hipFree(p), p = nullptr;
     nbr_.release();
-    void **bufs[] = {(void **)&d_supinv_, &d_col_, &d_psi_, &d_cs_, &d_u_, &d_rho_,
+    void **bufs[] = {(void **)&d_supinv_, &d_col_, &d_psi_, &d_cs_, &d_cwin_, &d_u_, &d_rho_,
                      (void **)&d_partial_, (void **)&d_result_};
     for (void **b : bufs)
         if (*b) (void)hipFree(*b), *b = nullptr;
@@ -532,8 +775,24 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
     LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
+    if (p.precision == LBM_FP32) {
+        // LDS runs for the staged colour kernel: the smallest staging size whose runs serve
+        // 99.5 % of the reads, else the largest
+        std::vector<ColourWin> wins;
+        for (int slots : {1024, 1536, 2048}) {
+            cwin_slots_ = slots;
+            cwin_cover_ = build_colour_windows(geo_, grid_, slots, wins);
+            if (cwin_cover_ >= 0.995) break;
+        }
+        LBM_HIP_TRY(hipMalloc(&d_cwin_, wins.size() * sizeof(ColourWin)));
+        LBM_HIP_TRY(hipMemcpyAsync(d_cwin_, wins.data(), wins.size() * sizeof(ColourWin), hipMemcpyHostToDevice, stream_));
+        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    }
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
     std::vector<uint32_t>().swap(geo_.nbr);
+    rc = autotune();
+    if (rc != LBM_OK) return rc;
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;   // the state is set by init() or load_state()
 }
 
@@ -555,7 +814,7 @@ int CgSolver::init(const double *psi_ijk, const double u0[3])
         if (prm_.precision == LBM_FP64)
             cg_init_colour_kernel<double><<<g, b, 0, stream_>>>(d, (double *)d_col_, (double *)d_psi_, n0, n0 + c, geo_.P);
         else
-            cg_init_colour_kernel<float><<<g, b, 0, stream_>>>(d, (float *)d_col_, (float *)d_psi_, n0, n0 + c, geo_.P);
+            cg_init_colour_kernel<float><<<g, b, 0, stream_>>>(d, (int32_t *)d_col_, (float *)d_psi_, n0, n0 + c, geo_.P);
         e = hipGetLastError();
         // the host chunk is refilled next round
         if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -613,16 +872,68 @@ int CgSolver::set_body_force(const double g[3])
     return LBM_OK;
 }
 
+template <typename ST, int NB>
+static void cg_launch_pull_nb(const CgArgs<ST> &a, bool store_u, hipStream_t s)
+{
+    const dim3 g((unsigned)((a.N + NB - 1) / NB)), b(NB);
+    store_u ? cg_collide_kernel<ST, K_PULL, true, NB><<<g, b, 0, s>>>(a)
+            : cg_collide_kernel<ST, K_PULL, false, NB><<<g, b, 0, s>>>(a);
+}
+
 template <typename ST>
-static hipError_t cg_launch_collide(const CgArgs<ST> &a, int mode, bool store_u, int64_t grid, hipStream_t s)
+static hipError_t cg_launch_collide(CgArgs<ST> a, int mode, bool store_u, int64_t grid, const lbm_launch &ls,
+                                    hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
+    a.order = ls.order;
     if (mode == K_INIT)
         store_u ? cg_collide_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
     else if (mode == K_DIRECT)
         cg_collide_kernel<ST, K_DIRECT, false><<<g, b, 0, s>>>(a);
+    else if (ls.block == 128)
+        cg_launch_pull_nb<ST, 128>(a, store_u, s);
+    else if (ls.block == 512)
+        cg_launch_pull_nb<ST, 512>(a, store_u, s);
     else
         store_u ? cg_collide_kernel<ST, K_PULL, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_PULL, false><<<g, b, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// the collide launch from buffer src to dst
+hipError_t CgSolver::launch_collide(int mode, bool store_u, const lbm_launch &ls, const void *src, void *dst)
+{
+    const int64_t N = geo_.N, P = geo_.P;
+    if (prm_.precision == LBM_FP64)
+        return cg_launch_collide(make_cg_args<double>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
+                                                      d_rho_, N, P),
+                                 mode, store_u, grid_, ls, stream_);
+    return cg_launch_collide(make_cg_args<float>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
+                                                 d_rho_, N, P),
+                             mode, store_u, grid_, ls, stream_);
+}
+
+// the colour-streaming launch: gathers, or (FP32) the LDS-staged variant
+hipError_t CgSolver::launch_colour(const lbm_launch &ls)
+{
+    const int64_t N = geo_.N, P = geo_.P;
+    const dim3 g((unsigned)grid_), b(BLOCK);
+    if (prm_.precision == LBM_FP64) {
+        cg_colour_kernel<double><<<g, b, 0, stream_>>>(nbr_.view(), (const double *)d_cs_, (double *)d_col_,
+                                                        (double *)d_psi_, N, P, ls.order);
+        return hipGetLastError();
+    }
+    const ColourWin *w = (const ColourWin *)d_cwin_;
+    const float *cs = (const float *)d_cs_;
+    int32_t *col = (int32_t *)d_col_;
+    float *psi = (float *)d_psi_;
+    if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1024)
+        cg_colour_kernel_win<1024><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+    else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1536)
+        cg_colour_kernel_win<1536><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+    else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 2048)
+        cg_colour_kernel_win<2048><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+    else
+        cg_colour_kernel_fx<<<g, b, 0, stream_>>>(nbr_.view(), cs, col, psi, N, P, ls.order);
     return hipGetLastError();
 }
 
@@ -630,32 +941,92 @@ int CgSolver::launch_step(bool store_u)
 {
     const int mode = cg_kmode(mode_);
     const int nxt = cur_ ^ 1;
-    const int64_t N = geo_.N, P = geo_.P;
-    dim3 g((unsigned)grid_), b(BLOCK);
-    hipError_t e;
-    if (prm_.precision == LBM_FP64) {
-        auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
-                                      d_rho_, N, P);
-        e = cg_launch_collide(a, mode, store_u, grid_, stream_);
-        if (e == hipSuccess) {
-            cg_colour_kernel<double><<<g, b, 0, stream_>>>(nbr_.view(), (const double *)d_cs_, (double *)d_col_,
-                                                            (double *)d_psi_, N, P);
-            e = hipGetLastError();
-        }
-    } else {
-        auto a = make_cg_args<float>(prm_, visc_, u_i_, pops_[cur_], pops_[nxt], nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
-                                     d_rho_, N, P);
-        e = cg_launch_collide(a, mode, store_u, grid_, stream_);
-        if (e == hipSuccess) {
-            cg_colour_kernel<float><<<g, b, 0, stream_>>>(nbr_.view(), (const float *)d_cs_, (float *)d_col_,
-                                                           (float *)d_psi_, N, P);
-            e = hipGetLastError();
-        }
-    }
-    LBM_HIP_TRY(e);
+    LBM_HIP_TRY(launch_collide(mode, store_u, launch_collide_, pops_[cur_], pops_[nxt]));
+    LBM_HIP_TRY(launch_colour(launch_colour_));
     cur_ = nxt;
     mode_ = STATE_PULL;
     ++steps_;
+    return LBM_OK;
+}
+
+// Time the collide and colour launches' settings on this geometry.  The timing runs on a
+// synthetic state (a random-looking +-1 phase field, one init launch); the handle has no state
+// yet at create, and init or load_state sets every field the timing launches touch.
+int CgSolver::autotune()
+{
+    if (geo_.N < TUNE_MIN_NODES) return LBM_OK;
+    lbm_launch c1, c2;
+    if (tune_cache_get(LBM_KERNEL_CG_COLLIDE, prm_.precision, geo_.N, c1) &&
+        tune_cache_get(LBM_KERNEL_CG_COLOUR, prm_.precision, geo_.N, c2)) {
+        launch_collide_ = c1;
+        launch_colour_ = c2;
+        return LBM_OK;
+    }
+    {
+        const int64_t N = geo_.N, chunk = std::min<int64_t>(N, stage_chunk(STAGE_BYTES, 1));
+        LBM_HIP_TRY(stage_.ensure());
+        std::vector<double> pn((size_t)chunk);
+        for (int64_t n0 = 0; n0 < N; n0 += chunk) {
+            const int64_t c = std::min(chunk, N - n0);
+            for (int64_t n = 0; n < c; ++n) pn[(size_t)n] = (((uint64_t)(n0 + n) * 2654435761u) >> 16) & 1 ? 1.0 : -1.0;
+            LBM_HIP_TRY(hipMemcpyAsync(stage_.buf, pn.data(), (size_t)c * sizeof(double), hipMemcpyHostToDevice, stream_));
+            dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
+            if (prm_.precision == LBM_FP64)
+                cg_init_colour_kernel<double><<<g, b, 0, stream_>>>((const double *)stage_.buf, (double *)d_col_,
+                                                                     (double *)d_psi_, n0, n0 + c, geo_.P);
+            else
+                cg_init_colour_kernel<float><<<g, b, 0, stream_>>>((const double *)stage_.buf, (int32_t *)d_col_,
+                                                                    (float *)d_psi_, n0, n0 + c, geo_.P);
+            LBM_HIP_TRY(hipGetLastError());
+            LBM_HIP_TRY(hipStreamSynchronize(stream_));
+        }
+    }
+    void *a = pops_[0], *b = pops_[1];
+    LBM_HIP_TRY(launch_collide(K_INIT, false, launch_collide_, b, a));
+    LBM_HIP_TRY(launch_colour(launch_colour_));
+    lbm_launch best;
+    const auto cc = tune_candidates(launch_collide_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
+    LBM_HIP_TRY(tune_launch(stream_, cc, [&](const lbm_launch &ls) { return launch_collide(K_PULL, false, ls, a, b); },
+                            best));
+    launch_collide_ = best;
+    std::vector<int> variants{LBM_VARIANT_GATHER};
+    if (prm_.precision == LBM_FP32 && cwin_slots_ > 0) variants.push_back(LBM_VARIANT_STAGED);
+    const auto cv = tune_candidates(launch_colour_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, variants);
+    LBM_HIP_TRY(tune_launch(stream_, cv, [&](const lbm_launch &ls) { return launch_colour(ls); }, best));
+    launch_colour_ = best;
+    tune_cache_put(LBM_KERNEL_CG_COLLIDE, prm_.precision, geo_.N, launch_collide_);
+    tune_cache_put(LBM_KERNEL_CG_COLOUR, prm_.precision, geo_.N, launch_colour_);
+    return LBM_OK;
+}
+
+int CgSolver::get_launch(int kernel, lbm_launch *out) const
+{
+    if (!out) return LBM_ERR_ARG;
+    if (kernel == LBM_KERNEL_CG_COLLIDE) *out = launch_collide_;
+    else if (kernel == LBM_KERNEL_CG_COLOUR) *out = launch_colour_;
+    else return LBM_ERR_ARG;
+    return LBM_OK;
+}
+
+int CgSolver::set_launch(int kernel, const lbm_launch *in)
+{
+    if (!in) return fail(LBM_ERR_ARG, "null launch settings");
+    lbm_launch l = *in;
+    l.tuned = 0;
+    l.time_us = 0.0;
+    if (kernel == LBM_KERNEL_CG_COLLIDE) {
+        if (!launch_valid(l, {128, 256, 512}, {0}))
+            return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0");
+        launch_collide_ = l;
+    } else if (kernel == LBM_KERNEL_CG_COLOUR) {
+        const std::vector<int> v = (prm_.precision == LBM_FP32 && cwin_slots_ > 0) ? std::vector<int>{0, 1}
+                                                                                    : std::vector<int>{0};
+        if (!launch_valid(l, {BLOCK}, v))
+            return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0 (or 1 in FP32)");
+        launch_colour_ = l;
+    } else {
+        return fail(LBM_ERR_ARG, "a colour-gradient handle launches LBM_KERNEL_CG_COLLIDE and LBM_KERNEL_CG_COLOUR");
+    }
     return LBM_OK;
 }
 
@@ -698,7 +1069,7 @@ int CgSolver::get_state(double *f, double *u, double *rho, double *psi, double *
         double *p = (double *)stage_.buf;
         for (int k = 0; k < 6; ++k)
             if (outs[k]) bufs[k] = p, p += c * width[k];
-        dim3 g((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
+        dim3 g((unsigned)((n1 - group_floor(n0) + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (prm_.precision == LBM_FP64) {
             auto a = make_cg_args<double>(prm_, visc_, u_i_, pops_[cur_], nullptr, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                           d_rho_, N, geo_.P);

@@ -52,6 +52,14 @@ __device__ __forceinline__ int64_t xcd_slab_tile(int64_t b, int64_t grid)
     return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Tile orders a launch can use (chosen per kernel and geometry by the autotuner, or set through
+// lbm_set_launch): the hardware order, runs of 8 tiles per XCD, or one slab per XCD.
+enum TileOrder : int32_t { ORDER_HW = 0, ORDER_RUNS8 = 1, ORDER_SLAB = 2 };
+__device__ __forceinline__ int64_t tile_of(int32_t order, int64_t b, int64_t grid)
+{
+    return order == ORDER_RUNS8 ? stream_tile(b, grid) : (order == ORDER_SLAB ? xcd_slab_tile(b, grid) : b);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {

@@ -133,6 +133,8 @@ struct BcDev {
 
 enum StateMode { STATE_EMPTY = 0, STATE_INIT = 1, STATE_DIRECT = 2, STATE_PULL = 3 };
 
+constexpr int BLOCK_DEFAULT = 256;   // workgroup size of every launch the autotuner has not changed
+
 // FP64 state crosses between host and device in chunks through one bounded device buffer,
 // allocated on a handle's first state transfer and kept until the handle is destroyed, so a
 // checkpoint of a large state (f alone is 0.9 GB in FP64 for 5.9 M nodes) needs no full-size
@@ -163,6 +165,7 @@ struct FusedSolute {
     const void *src;
     void *dst;
     double omega, w0, wq, inv_cs2;
+    lbm_launch ls;   // launch settings of the fused step (the solute handle's)
 };
 
 struct DiagBox;
@@ -186,6 +189,15 @@ public:
     int set_power_law(double m, double n);
     int get_viscosity(double *nu);
     int get_dense(double *u_ijk, double *rho_ijk);
+    int get_launch(int kernel, lbm_launch *out) const;
+    int set_launch(int kernel, const lbm_launch *in);
+
+    // one step launch from buffer src to dst (bare: without the handle's faces and power law,
+    // and without storing u -- the timing launches of the autotuners, which leave the state alone)
+    hipError_t launch_kernel(int mode, bool store_u, const FusedSolute *fs, const lbm_launch &ls, const void *src,
+                             void *dst, bool bare);
+    void *pops(int k) const { return pops_[k]; }
+    int current() const { return cur_; }
 
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
@@ -207,6 +219,7 @@ public:
 
 private:
     int launch_step(bool store_u, const FusedSolute *fs = nullptr);
+    int autotune();
     int release();
     void release_boundaries();
     BcDev bc_view(bool for_step) const;
@@ -246,6 +259,7 @@ private:
     void *d_nu_vn_ = nullptr;   // P compute type
     double pl_m_ = 0.0, pl_n_ = 0.0;
     Staging stage_;   // host transfers of the FP64 state
+    lbm_launch launch_{BLOCK_DEFAULT, 1, 0, 0, 0.0};   // the step's launch settings (runs of 8 tiles)
 };
 
 class CgSolver : public lbm_handle {
@@ -261,12 +275,17 @@ public:
     int set_body_force(const double g[3]);
     int get_state(double *f, double *u, double *rho, double *psi, double *rho_r, double *rho_b);
     int diagnostics(const int32_t *box, lbm_cg_diag *out);
+    int get_launch(int kernel, lbm_launch *out) const;
+    int set_launch(int kernel, const lbm_launch *in);
 
     int64_t count() const { return geo_.N; }
     int64_t steps_done() const { return steps_; }
 
 private:
     int launch_step(bool store_u);
+    hipError_t launch_collide(int mode, bool store_u, const lbm_launch &ls, const void *src, void *dst);
+    hipError_t launch_colour(const lbm_launch &ls);
+    int autotune();
     void release();
 
     lbm_cg_params prm_{};
@@ -283,6 +302,11 @@ private:
     void *d_col_ = nullptr;   // 2 x P: rho_r, rho_b after colour streaming
     void *d_psi_ = nullptr;   // P: psi = (rho_r - rho_b)/(rho_r + rho_b)
     void *d_cs_ = nullptr;    // 8 x P (AoS per node): rho_r, rho_b, u, n of the last collision
+    void *d_cwin_ = nullptr;  // FP32: per block, the runs of upstream colour states staged in LDS
+    int cwin_slots_ = 0;      // LDS states per block of the staged colour kernel (0: gathers)
+    double cwin_cover_ = 0;   // fraction of the colour reads the staged runs serve
+    lbm_launch launch_collide_{BLOCK_DEFAULT, 1, 0, 0, 0.0};   // runs of 8 tiles
+    lbm_launch launch_colour_{BLOCK_DEFAULT, 2, 0, 0, 0.0};    // gathers, one slab per XCD
     void *d_u_ = nullptr;     // 3 x P: u of the previous step
     void *d_rho_ = nullptr;   // P: rho of a loaded checkpoint
     double *d_partial_ = nullptr;
@@ -308,10 +332,13 @@ public:
     int step(int nsteps, double *elapsed_ms);
     int get_state(double *g, double *conc);
     int diagnostics(lbm_st_diag *out);
+    int get_launch(int kernel, lbm_launch *out) const;
+    int set_launch(int kernel, const lbm_launch *in);
     int64_t count() const { return N_; }
 
 private:
     int setup(const lbm_st_params &p);
+    int autotune();
     int upload_velocity(const double *u);
     int launch_step(bool last);
     // the velocity a launch reads: the frozen (or, coupled, initial) field until the first
@@ -340,6 +367,8 @@ private:
     StateMode mode_ = STATE_EMPTY;
     int64_t steps_ = 0;
     Staging stage_;
+    lbm_launch launch_{BLOCK_DEFAULT, 0, 0, 0, 0.0};       // coupled: the fused flow + solute step
+    lbm_launch launch_own_{BLOCK_DEFAULT, 0, 0, 0, 0.0};   // standalone solute step
 };
 
 }  // namespace lbm

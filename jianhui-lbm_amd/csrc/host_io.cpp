@@ -163,15 +163,16 @@ bool read_geometry(const std::string &path, int nx, int ny, int nz, int decbin, 
         err = "the pore geometry file (" + path + ") does not exist";
         return false;
     }
-    // tokens other than 0 and 1 are skipped, x runs fastest
+    // tokens other than the numbers 0 and 1 are skipped, x runs fastest
     std::string t;
     for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 int v = -1;
                 while (f >> t) {
-                    const double d = std::strtod(t.c_str(), nullptr);
-                    if (d == 0.0 || d == 1.0) {
+                    char *end = nullptr;
+                    const double d = std::strtod(t.c_str(), &end);
+                    if (end != t.c_str() && *end == '\0' && (d == 0.0 || d == 1.0)) {
                         v = (int)d;
                         break;
                     }

