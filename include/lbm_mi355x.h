@@ -124,6 +124,13 @@ int lbm_sp_step_timed(lbm_handle *h, int32_t nsteps, double *elapsed_ms);
 /* post-streaming f, u and rho in the reference's layout -- the content of Backup (SP:4507-4562)
  * without the padding record.  Any pointer may be NULL. */
 int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho);
+/* lbm_sp_get_state with no device allocation: no counterpart in the reference (whose Backup writes
+ * from host arrays).  For callers that cannot allocate device memory between steps -- serving
+ * loops that checkpoint a live state, harnesses that forbid hipMalloc after set-up.  The state
+ * crosses through a fixed 16 MiB device array of the library (one per GPU, held until the process
+ * ends) in chunks of 91,136 nodes, so any node count is accepted.  Calls on one GPU share that
+ * array: calls that overlap in time on one GPU race (one call at a time per GPU). */
+int lbm_sp_get_state_noalloc(lbm_handle *h, double *f, double *u, double *rho);
 /* FP64 device reductions for Error (SP:3625-3704) and Comput_Perm (SP:4565-4712).
  * box = {xn, xp, yn, yp, zn, zp} inclusive voxel bounds (par_per_* / per_*) or NULL. */
 int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out);

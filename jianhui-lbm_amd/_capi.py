@@ -161,6 +161,7 @@ def lib():
         L.lbm_sp_step.argtypes = [vp, ctypes.c_int32]
         L.lbm_sp_step_timed.argtypes = [vp, ctypes.c_int32, dp]
         L.lbm_sp_get_state.argtypes = [vp, dp, dp, dp]
+        L.lbm_sp_get_state_noalloc.argtypes = [vp, dp, dp, dp]
         L.lbm_sp_get_dense.argtypes = [vp, dp, dp]
         L.lbm_sp_diagnostics.argtypes = [vp, ip, ctypes.POINTER(SpDiag)]
         L.lbm_sp_error_ranks.argtypes = [vp, ctypes.c_int32, dp]

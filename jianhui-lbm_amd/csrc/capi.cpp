@@ -140,6 +140,12 @@ int lbm_sp_get_state(lbm_handle *h, double *f, double *u, double *rho)
     return s ? s->get_state(f, u, rho) : LBM_ERR_ARG;
 }
 
+int lbm_sp_get_state_noalloc(lbm_handle *h, double *f, double *u, double *rho)
+{
+    auto *s = as_sp(h);
+    return s ? s->get_state_noalloc(f, u, rho) : LBM_ERR_ARG;
+}
+
 int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3])
 {
     auto *s = as_sp(h);

@@ -182,6 +182,8 @@ public:
     int get_state(double *f, double *u, double *rho);
     // get_state through a caller-chosen device buffer of the given size (no allocation)
     int get_state_via(double *f, double *u, double *rho, void *buf, size_t bytes);
+    // get_state through the library's fixed device array (lbm_sp_get_state_noalloc)
+    int get_state_noalloc(double *f, double *u, double *rho);
     int diagnostics(const int32_t *box, lbm_sp_diag *out);
     // Error over the reference's nranks x-slab ranks: err, u_max, sum of |u|
     int error_ranks(int nranks, double out[3]);

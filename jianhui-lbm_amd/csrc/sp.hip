@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "boundary.h"
 #include "collide.h"
@@ -978,6 +979,49 @@ int SpSolver::error_ranks(int nranks, double out[3])
     out[1] = umax;
     out[2] = speed;
     return LBM_OK;
+}
+
+// ---------------------------------------------------------------- state read without allocation
+// lbm_sp_get_state_noalloc stages the state through this fixed array of the code object instead
+// of a buffer of the handle.  Every GPU the library runs on holds its own copy until the process
+// ends.  In host code the array's name is not a device address: the address of each device's copy
+// is taken once per device with hipGetSymbolAddress (with that device current) and only that
+// address reaches kernels and copies.  Calls on one GPU share its copy: calls that overlap in
+// time on one GPU race on it (a single-threaded caller, one call at a time per GPU, is safe).
+}  // namespace lbm
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace lbm {
+namespace {
+constexpr int MAX_SCRATCH_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[MAX_SCRATCH_DEVICES] = {nullptr};
+}  // namespace
+
+int SpSolver::get_state_noalloc(double *f, double *u, double *rho)
+{
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    if (device_ < 0 || device_ >= MAX_SCRATCH_DEVICES)
+        return fail(LBM_ERR_ARG, "lbm_sp_get_state_noalloc: device ordinal beyond the 64 the library tracks");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    void *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (!scratch_ptr[device_]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fail(LBM_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            scratch_ptr[device_] = p;
+        }
+        ptr = scratch_ptr[device_];
+    }
+    // the state streams through the array in chunks (91,136 nodes of f, u and rho per chunk), so
+    // no node count is too large for it; nothing in it needs clearing, each chunk is written
+    // whole before it is copied out
+    return get_state_via(f, u, rho, ptr, NOALLOC_SCRATCH_BYTES);
 }
 
 }  // namespace lbm

@@ -160,6 +160,14 @@ class SinglePhase:
         _capi.check(_capi.lib().lbm_sp_get_state(self._h, _dp(f), _dp(u), _dp(rho)), self._h)
         return f, u, rho
 
+    def state_noalloc(self):
+        """state() through lbm_sp_get_state_noalloc: the library's fixed device array, no allocation."""
+        f = np.empty((self.count, 19))
+        u = np.empty((self.count, 3))
+        rho = np.empty(self.count)
+        _capi.check(_capi.lib().lbm_sp_get_state_noalloc(self._h, _dp(f), _dp(u), _dp(rho)), self._h)
+        return f, u, rho
+
     def dense(self):
         """(u[nx,ny,nz,3], rho[nx,ny,nz]) on the box, solid voxels u = 0 and rho = 1: the arrays
         the reference's field writers gather (output_velocity_b SP:4265-4390)."""
