@@ -329,12 +329,13 @@ double lbm_srv_clock(const lbm_handle *h);
 #define LBM_ORDER_HW 0            /* tiles in the hardware's round-robin order over the XCDs */
 #define LBM_ORDER_RUNS8 1         /* each XCD takes runs of 8 consecutive tiles */
 #define LBM_ORDER_SLAB 2          /* each XCD sweeps one contiguous slab of tiles */
-#define LBM_VARIANT_GATHER 0      /* colour streaming: 18 gathers per node through the caches */
-#define LBM_VARIANT_STAGED 1      /* colour streaming: the block's neighbourhood staged in LDS (FP32) */
+#define LBM_VARIANT_GATHER 0      /* colour-gradient kernels: 18 neighbour gathers per node through the caches */
+#define LBM_VARIANT_STAGED 1      /* the block's neighbourhood staged in LDS first (block 256; the colour
+                                     streaming in FP32 only, the collide's phase field in both) */
 typedef struct lbm_launch {
     int32_t block;     /* threads (nodes) per workgroup: 128, 256 or 512 */
     int32_t order;     /* LBM_ORDER_* */
-    int32_t variant;   /* LBM_VARIANT_* (colour streaming; 0 for the other kernels) */
+    int32_t variant;   /* LBM_VARIANT_* (colour-gradient kernels; 0 for the others) */
     int32_t tuned;     /* 1: chosen by the autotuner on this geometry, 0: default or set */
     double time_us;    /* the autotuner's time for one launch of the choice (0 if not tuned) */
 } lbm_launch;

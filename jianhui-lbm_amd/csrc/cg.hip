@@ -81,17 +81,20 @@ struct CgArgs {
     C ux_i, uy_i, uz_i;
     int32_t bfa;
     float scale, inv_scale;
-    int32_t order;   // tile order of the collide launch (TileOrder)
+    int32_t order;                            // tile order of the collide launch (TileOrder)
+    const ColourWin *__restrict__ wins;       // the staged variant's runs per 256-node block
 };
 
-// colour state of a node: its stored colour densities (raw) and six values of the compute type
-__device__ __forceinline__ void store_cs(float *cs, int64_t n, int32_t rr, int32_t rb, const float (&v)[6])
+// colour state of a node: its stored colour densities (raw) and six values of the compute type.
+// FP32: two rows of 16-byte records, (rho_r, rho_b, ux, uy) at [n] and (uz, nx, ny, nz) at
+// [P + n], so that a wavefront's loads of either half are 1 KB contiguous.
+__device__ __forceinline__ void store_cs(float *cs, int64_t n, int64_t P, int32_t rr, int32_t rb, const float (&v)[6])
 {
-    float4 *p = reinterpret_cast<float4 *>(cs + 8 * n);
-    p[0] = make_float4(__int_as_float(rr), __int_as_float(rb), v[0], v[1]);
-    p[1] = make_float4(v[2], v[3], v[4], v[5]);
+    float4 *p = reinterpret_cast<float4 *>(cs);
+    p[n] = make_float4(__int_as_float(rr), __int_as_float(rb), v[0], v[1]);
+    p[P + n] = make_float4(v[2], v[3], v[4], v[5]);
 }
-__device__ __forceinline__ void store_cs(double *cs, int64_t n, double rr, double rb, const double (&w)[6])
+__device__ __forceinline__ void store_cs(double *cs, int64_t n, int64_t, double rr, double rb, const double (&w)[6])
 {
     const double v[8] = {rr, rb, w[0], w[1], w[2], w[3], w[4], w[5]};
     double2 *p = reinterpret_cast<double2 *>(cs + 8 * n);
@@ -101,17 +104,17 @@ __device__ __forceinline__ void store_cs(double *cs, int64_t n, double rr, doubl
     p[3] = make_double2(v[6], v[7]);
 }
 template <typename C>
-__device__ __forceinline__ void load_cs(const C *cs, int64_t n, C (&v)[8]);
+__device__ __forceinline__ void load_cs(const C *cs, int64_t n, int64_t P, C (&v)[8]);
 template <>
-__device__ __forceinline__ void load_cs<float>(const float *cs, int64_t n, float (&v)[8])
+__device__ __forceinline__ void load_cs<float>(const float *cs, int64_t n, int64_t P, float (&v)[8])
 {
-    const float4 *p = reinterpret_cast<const float4 *>(cs + 8 * n);
-    const float4 a = p[0], b = p[1];
+    const float4 *p = reinterpret_cast<const float4 *>(cs);
+    const float4 a = p[n], b = p[P + n];
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 template <>
-__device__ __forceinline__ void load_cs<double>(const double *cs, int64_t n, double (&v)[8])
+__device__ __forceinline__ void load_cs<double>(const double *cs, int64_t n, int64_t, double (&v)[8])
 {
     const double2 *p = reinterpret_cast<const double2 *>(cs + 8 * n);
 #pragma unroll
@@ -161,12 +164,35 @@ __device__ __forceinline__ void cg_node_state(const CgArgs<ST> &a, int64_t n, ty
     }
 }
 
-template <typename ST, int MODE, bool STORE_U, int NB = BLOCK>
+// SLOTS > 0: the staged variant (NB = 256).  The phase field of the block's neighbourhood -- the
+// runs of node ids of the colour kernel's staging (ColourWin) -- is copied to LDS first, and the
+// colour gradient reads its 18 neighbour values from there.
+template <typename ST, int MODE, bool STORE_U, int NB = BLOCK, int SLOTS = 0>
 __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
 {
     using C = typename Store<ST>::C;
-    const int64_t n = tile_of(a.order, blockIdx.x, gridDim.x) * NB + threadIdx.x;
+    const int64_t b = tile_of(a.order, blockIdx.x, gridDim.x);
+    const int64_t n = b * NB + threadIdx.x;
+    __shared__ C lpsi[SLOTS > 0 ? SLOTS : 1];
+    ColourWin w{};
+    if (SLOTS > 0) {
+        w = a.wins[b];
+#pragma unroll
+        for (int k = 0; k < CW_RUNS; ++k)
+            for (int i = threadIdx.x; i < w.len[k]; i += NB) lpsi[w.off[k] + i] = a.psi[(int64_t)w.lo[k] + i];
+        __syncthreads();
+    }
     if (n >= a.N) return;
+    auto psi_at = [&](uint32_t id) -> C {
+        if (SLOTS > 0) {
+            int slot = -1;
+#pragma unroll
+            for (int k = 0; k < CW_RUNS; ++k)
+                if (id - (uint32_t)w.lo[k] < (uint32_t)w.len[k]) slot = w.off[k] + (int)(id - (uint32_t)w.lo[k]);
+            if (slot >= 0) return lpsi[slot];
+        }
+        return a.psi[id];
+    };
     C h[Q], m[Q];
     uint32_t nb[Q - 1];
     Macro<C> mc;
@@ -185,7 +211,7 @@ __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
         const bool solid = (v & BB_FLAG) != 0;
         wall |= solid;
         // unconditional load (a solid site's index is the node itself), then select
-        const C pv = a.psi[v & ~BB_FLAG];
+        const C pv = psi_at(v & ~BB_FLAG);
         const C pq = solid ? a.psi_solid : pv;
         const C wq = C(3.0 * weight(q));
         if (EX[q] != 0) cx += C(EX[q]) * wq * pq;
@@ -222,7 +248,7 @@ __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
 #pragma unroll
     for (int q = 0; q < Q; ++q) Store<ST>::store_nt(&a.dst[(int64_t)q * a.P + n], h[q], a.scale);
     const C st[6] = {mc.ux, mc.uy, mc.uz, nx, ny, nz};
-    store_cs(a.cs, n, rr_raw, rb_raw, st);
+    store_cs(a.cs, n, a.P, rr_raw, rb_raw, st);
 }
 
 // g_r, g_b leaving a node with colour state v in direction sgn*e_q (sgn = -1: the opposite
@@ -266,7 +292,7 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *_
     const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (n >= N) return;
     C own_s[8];
-    load_cs(cs, n, own_s);
+    load_cs(cs, n, P, own_s);
     double own[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) own[k] = (double)own_s[k];
@@ -286,7 +312,7 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *_
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             const uint32_t v = src[q0 + j - 1];
-            load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, up_s[j]);
+            load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, up_s[j]);
         }
 #pragma unroll
         for (int j = 0; j < G; ++j) {
@@ -400,12 +426,13 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_fx(NbrDev nbr, const f
     const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (n >= N) return;
     float own[8];
-    load_cs(cs, n, own);
+    load_cs(cs, n, P, own);
     uint32_t src[Q - 1];
     load_nbr(nbr, P, n, src);
     ColourSums acc;
     acc.begin(own);
-    ColourStep<1>::run(acc, src, [&](uint32_t v, float (&out)[8]) { load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, out); });
+    ColourStep<1>::run(acc, src,
+                       [&](uint32_t v, float (&out)[8]) { load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, out); });
     acc.store(n, P, col, psi);
 }
 
@@ -423,15 +450,17 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const 
                                                               int32_t *__restrict__ col, float *__restrict__ psi,
                                                               int64_t N, int64_t P, int32_t order)
 {
-    __shared__ float4 lds[2 * SLOTS];
+    __shared__ float4 lds[2 * SLOTS];   // two rows of SLOTS records, as in global memory
     const int64_t b = tile_of(order, blockIdx.x, gridDim.x);
     const ColourWin w = wins[b];
     const float4 *__restrict__ cs4 = reinterpret_cast<const float4 *>(cs);
 #pragma unroll
     for (int k = 0; k < CW_RUNS; ++k) {
-        const float4 *g = cs4 + 2 * (int64_t)w.lo[k];
-        float4 *l = lds + 2 * w.off[k];
-        for (int i = threadIdx.x; i < 2 * w.len[k]; i += BLOCK) l[i] = g[i];
+        const float4 *g = cs4 + (int64_t)w.lo[k];
+        for (int i = threadIdx.x; i < w.len[k]; i += BLOCK) {
+            lds[w.off[k] + i] = g[i];
+            lds[SLOTS + w.off[k] + i] = g[P + i];
+        }
     }
     __syncthreads();
     const int64_t n = b * BLOCK + threadIdx.x;
@@ -444,11 +473,11 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const 
         for (int k = 0; k < CW_RUNS; ++k)
             if (id - (uint32_t)w.lo[k] < (uint32_t)w.len[k]) slot = w.off[k] + (int)(id - (uint32_t)w.lo[k]);
         if (slot >= 0) {
-            const float4 a = lds[2 * slot], c = lds[2 * slot + 1];
+            const float4 a = lds[slot], c = lds[SLOTS + slot];
             v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
             v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
         } else {
-            load_cs(cs, (int64_t)id, v);
+            load_cs(cs, (int64_t)id, P, v);
         }
     };
     float own[8];
@@ -651,6 +680,7 @@ CgArgs<ST> make_cg_args(const lbm_cg_params &p, const double *visc, const double
     a.scale = 1.0f;
     a.inv_scale = 1.0f;
     a.order = ORDER_RUNS8;
+    a.wins = nullptr;
     return a;
 }
 
@@ -775,9 +805,9 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
     LBM_HIP_TRY(hipMemcpyAsync(d_supinv_, geo_.supinv.data(), geo_.supinv.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream_));
     LBM_HIP_TRY(hipMemsetAsync(d_u_, 0, (size_t)3 * P * ct_bytes_, stream_));
-    if (p.precision == LBM_FP32) {
-        // LDS runs for the staged colour kernel: the smallest staging size whose runs serve
-        // 99.5 % of the reads, else the largest
+    {
+        // LDS runs for the staged kernels: the smallest staging size whose runs serve 99.5 % of
+        // the reads, else the largest
         std::vector<ColourWin> wins;
         for (int slots : {1024, 1536, 2048}) {
             cwin_slots_ = slots;
@@ -880,13 +910,28 @@ static void cg_launch_pull_nb(const CgArgs<ST> &a, bool store_u, hipStream_t s)
             : cg_collide_kernel<ST, K_PULL, false, NB><<<g, b, 0, s>>>(a);
 }
 
+template <typename ST, int SLOTS>
+static void cg_launch_pull_staged(const CgArgs<ST> &a, bool store_u, int64_t grid, hipStream_t s)
+{
+    const dim3 g((unsigned)grid), b(BLOCK);
+    store_u ? cg_collide_kernel<ST, K_PULL, true, BLOCK, SLOTS><<<g, b, 0, s>>>(a)
+            : cg_collide_kernel<ST, K_PULL, false, BLOCK, SLOTS><<<g, b, 0, s>>>(a);
+}
+
 template <typename ST>
 static hipError_t cg_launch_collide(CgArgs<ST> a, int mode, bool store_u, int64_t grid, const lbm_launch &ls,
-                                    hipStream_t s)
+                                    const void *wins, int slots, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
     a.order = ls.order;
-    if (mode == K_INIT)
+    a.wins = (const ColourWin *)wins;
+    if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 1024)
+        cg_launch_pull_staged<ST, 1024>(a, store_u, grid, s);
+    else if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 1536)
+        cg_launch_pull_staged<ST, 1536>(a, store_u, grid, s);
+    else if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 2048)
+        cg_launch_pull_staged<ST, 2048>(a, store_u, grid, s);
+    else if (mode == K_INIT)
         store_u ? cg_collide_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
     else if (mode == K_DIRECT)
         cg_collide_kernel<ST, K_DIRECT, false><<<g, b, 0, s>>>(a);
@@ -906,10 +951,10 @@ hipError_t CgSolver::launch_collide(int mode, bool store_u, const lbm_launch &ls
     if (prm_.precision == LBM_FP64)
         return cg_launch_collide(make_cg_args<double>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                                       d_rho_, N, P),
-                                 mode, store_u, grid_, ls, stream_);
+                                 mode, store_u, grid_, ls, d_cwin_, cwin_slots_, stream_);
     return cg_launch_collide(make_cg_args<float>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                                  d_rho_, N, P),
-                             mode, store_u, grid_, ls, stream_);
+                             mode, store_u, grid_, ls, d_cwin_, cwin_slots_, stream_);
 }
 
 // the colour-streaming launch: gathers, or (FP32) the LDS-staged variant
@@ -981,18 +1026,33 @@ int CgSolver::autotune()
             LBM_HIP_TRY(hipStreamSynchronize(stream_));
         }
     }
+    // each candidate is timed inside a step, collide then colour launch, as the two share the
+    // caches in the run: timed alone, a kernel finds its inputs still cached from its own last
+    // launch
     void *a = pops_[0], *b = pops_[1];
     LBM_HIP_TRY(launch_collide(K_INIT, false, launch_collide_, b, a));
     LBM_HIP_TRY(launch_colour(launch_colour_));
     lbm_launch best;
-    const auto cc = tune_candidates(launch_collide_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
-    LBM_HIP_TRY(tune_launch(stream_, cc, [&](const lbm_launch &ls) { return launch_collide(K_PULL, false, ls, a, b); },
+    const std::vector<int> staged{LBM_VARIANT_GATHER, LBM_VARIANT_STAGED};
+    auto cc = tune_candidates(launch_collide_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
+    for (const lbm_launch &l : tune_candidates(launch_collide_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
+        if (l.variant == LBM_VARIANT_STAGED) cc.push_back(l);
+    LBM_HIP_TRY(tune_launch(stream_, cc,
+                            [&](const lbm_launch &ls) {
+                                hipError_t e = launch_collide(K_PULL, false, ls, a, b);
+                                return e == hipSuccess ? launch_colour(launch_colour_) : e;
+                            },
                             best));
     launch_collide_ = best;
     std::vector<int> variants{LBM_VARIANT_GATHER};
-    if (prm_.precision == LBM_FP32 && cwin_slots_ > 0) variants.push_back(LBM_VARIANT_STAGED);
+    if (prm_.precision == LBM_FP32) variants.push_back(LBM_VARIANT_STAGED);
     const auto cv = tune_candidates(launch_colour_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, variants);
-    LBM_HIP_TRY(tune_launch(stream_, cv, [&](const lbm_launch &ls) { return launch_colour(ls); }, best));
+    LBM_HIP_TRY(tune_launch(stream_, cv,
+                            [&](const lbm_launch &ls) {
+                                hipError_t e = launch_collide(K_PULL, false, launch_collide_, a, b);
+                                return e == hipSuccess ? launch_colour(ls) : e;
+                            },
+                            best));
     launch_colour_ = best;
     tune_cache_put(LBM_KERNEL_CG_COLLIDE, prm_.precision, geo_.N, launch_collide_);
     tune_cache_put(LBM_KERNEL_CG_COLOUR, prm_.precision, geo_.N, launch_colour_);
@@ -1015,12 +1075,11 @@ int CgSolver::set_launch(int kernel, const lbm_launch *in)
     l.tuned = 0;
     l.time_us = 0.0;
     if (kernel == LBM_KERNEL_CG_COLLIDE) {
-        if (!launch_valid(l, {128, 256, 512}, {0}))
-            return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0");
+        if (!(launch_valid(l, {128, 256, 512}, {0}) || launch_valid(l, {BLOCK}, {1})))
+            return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0; or block 256, variant 1");
         launch_collide_ = l;
     } else if (kernel == LBM_KERNEL_CG_COLOUR) {
-        const std::vector<int> v = (prm_.precision == LBM_FP32 && cwin_slots_ > 0) ? std::vector<int>{0, 1}
-                                                                                    : std::vector<int>{0};
+        const std::vector<int> v = prm_.precision == LBM_FP32 ? std::vector<int>{0, 1} : std::vector<int>{0};
         if (!launch_valid(l, {BLOCK}, v))
             return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0 (or 1 in FP32)");
         launch_colour_ = l;
