@@ -6,8 +6,16 @@ tau = 1 (nu = 1/6), Guo body force g = (1e-6, 0, 0), FP32 shifted populations wi
 device reductions.  A step is one fused pull-stream + MRT collide launch over every fluid
 node (collision + comput_macro_variables of the reference, SP:662-684).
 
-Prints one JSON line: fluid MLUPS (whole job), the HBM roofline of the step kernel, and the
-FP64 CPU oracle timed on this host's cores on a bounded sample of the same workload.
+Prints one JSON line: fluid MLUPS (whole job), the HBM roofline of the step kernel, the launch
+settings the autotuner chose, and the reference program (FP64, under mpirun) and the FP64 C
+oracle timed on this host's cores on a bounded sample of the same workload.
+
+The default line also carries sub-records, each with its own roofline and launch settings:
+"fp64" and "fp16" (the same workload in the other storage precisions), "accuracy" (the FP64,
+FP32 and FP16 runs to steady state: k and velocity-field errors against FP64), "cg" (the
+colour-gradient step on a 192^3 pack, its FP64 twin, the reference colour-gradient program as
+CPU baseline and the FP32-vs-FP64 run over 20,000 steps) and "solute" (the coupled flow +
+D3Q7 solute step on a 384^3 fibre pack).
 
 The same line carries the serving figures ("serving"): a seeded Poisson stream of single-phase
 permeability requests on thresholded Gaussian-random-field subsamples (edge 48..160 lu,
@@ -345,7 +353,8 @@ def cg_bench(pkg, steps, warmup, precision, edge=192, cpu_s=15.0, no_cpu=False):
                         "traffic": read_traffic(f"cg_step_{precision}_{edge}cube"), "bytes_per_node": b_alg,
                         "kernels": "cg_collide_kernel + cg_colour_kernel per step",
                         "avg_step_us": round(ms * 1e3 / steps, 2)},
-           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"], "cpu_baseline": None}
+           "saturation": d["wet_count"] / d["count_in_box"], "nan_count": d["nan_count"],
+           "launch": {"cg_collide": s.launch("cg_collide"), "cg_colour": s.launch("cg_colour")}, "cpu_baseline": None}
     s.close()
     if not no_cpu:
         out["cpu_baseline"] = cg_reference_cpu_baseline(solid, psi, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), cpu_s)
@@ -424,7 +433,7 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
                         "kernels": "one sp_step_kernel launch per step with the solute step fused in",
                         "avg_step_us": round(ms * 1e3 / steps, 2)},
            "mass_rel_drift": abs(d["mass"] - m0) / m0, "pore_velocity_at_end": u_pore,
-           "pe_at_end": u_pore * 8.0 / D, "cpu_baseline": None}
+           "pe_at_end": u_pore * 8.0 / D, "launch": {"sp_step_fused": st.launch()}, "cpu_baseline": None}
     st.close()
     flow.close()
     # the solute kernel alone on the same pack with a frozen velocity field: the reference's
@@ -444,7 +453,7 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
                             "unit": "MLUPS", "kernel": "st_step_kernel", "bytes_per_node": B_ALG_ST[precision],
                             "avg_launch_us": round(avg * 1e6, 2), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                             "frac": round(ach / HBM_PEAK_GBS, 4),
-                            "traffic": read_traffic(f"st_step_{precision}_{edge}cube")}
+                            "traffic": read_traffic(f"st_step_{precision}_{edge}cube"), "launch": fs.launch()}
     fs.close()
     if not no_cpu:
         out["cpu_baseline"] = solute_cpu_baseline(solid, nu, g, D, w0, cpu_s)
@@ -555,6 +564,49 @@ def accuracy_study(pkg, edge=256, max_steps=100000):
     return out
 
 
+SUB_STEPS, SUB_WARMUP = 500, 20     # timed and untimed steps of each sub-record
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sp_record(pkg, solid, precision, steps, warmup, nu, g, faces=None, power_law=None):
+    """The single-phase step on the given pack: MLUPS over the wall clock of `steps` steps, and
+    the step kernel's roofline from HIP events on the handle stream around the same launches."""
+    edge = solid.shape[0]
+    s = pkg.SinglePhase(solid, nu, g, precision=precision, device=0)
+    if faces:
+        s.set_boundaries(faces["sub_bc"], faces["pre"])
+        s.init(1.0, (0.0, 0.0, 0.0))
+    if power_law:
+        s.set_power_law(*power_law)
+        s.init(1.0, (0.0, 0.0, 0.0))
+    N = s.count
+    s.step(warmup)
+    s.diagnostics()                        # synchronises the handle stream
+    t0 = time.perf_counter()
+    kernel_ms = s.step_timed(steps)        # HIP events on the handle stream around the K launches
+    wall = time.perf_counter() - t0
+    diag = s.diagnostics()
+    if diag["nan_count"] != 0:
+        raise RuntimeError(f"non-finite velocity in the {precision} benchmark run")
+    avg_launch_s = kernel_ms / 1e3 / steps
+    # the power-law step also reads and writes the node's viscosity
+    b_node = B_ALG[precision] + (2 * (8 if precision == "fp64" else 4) if power_law else 0)
+    achieved = b_node * N / avg_launch_s / 1e9
+    tag = f"sp_step_{precision}_{edge}cube" + ("_xfaces" if faces else "") + ("_powerlaw" if power_law else "")
+    rec = {"value": round(N * steps / wall / 1e6, 1), "unit": "MLUPS", "steps": steps, "warmup": warmup,
+           "ms_per_step": round(wall * 1e3 / steps, 5), "dtype": DTYPE[precision], "fluid_nodes": N,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(tag),
+                        "kernel": "sp_step_kernel", "bytes_per_node": b_node,
+                        "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+           "launch": s.launch()}
+    s.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -577,6 +629,8 @@ def main():
     ap.add_argument("--vtk", action="store_true", help="time the VTK field writers of the lbm_sp driver")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="the single-phase FP32 line alone, without the fp64 / fp16 / accuracy / cg / solute records")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
@@ -612,41 +666,20 @@ def main():
         # the same mean driving as g = 1e-6: dp = |p+ - p-| c_s^2 / L = 1e-6 over 256 lu
         g = (0.0, 0.0, 0.0)
         faces = {"sub_bc": 3, "pre": {"xp": 1.0, "xn": 1.0 + 3e-6 * edge}}
-    s = pkg.SinglePhase(solid, nu, g, precision=args.precision, device=0)
-    if faces:
-        s.set_boundaries(faces["sub_bc"], faces["pre"])
-        s.init(1.0, (0.0, 0.0, 0.0))
     power_law = None
     if args.power_law:
         # m chosen so that nu = 1/6 at |S| = 1e-7, the strain rate scale of this flow
         power_law = (1.0 / 6.0 * 1e-7 ** 0.2, 0.8)
-        s.set_power_law(*power_law)
-        s.init(1.0, (0.0, 0.0, 0.0))
-    N = s.count
-    s.step(args.warmup)
-    s.diagnostics()                      # synchronises the handle stream
-    t0 = time.perf_counter()
-    kernel_ms = s.step_timed(args.steps)   # HIP events on the handle stream around the K launches
-    wall = time.perf_counter() - t0
-    diag = s.diagnostics()
-    if diag["nan_count"] != 0:
-        raise RuntimeError("non-finite velocity in the benchmark run")
-    ms_per_step = wall * 1e3 / args.steps
-    value = N * args.steps / wall / 1e6
-    avg_launch_s = kernel_ms / 1e3 / args.steps
-    # the power-law step also reads and writes the node's viscosity
-    b_node = B_ALG[args.precision] + (2 * (8 if args.precision == "fp64" else 4) if power_law else 0)
-    achieved = b_node * N / avg_launch_s / 1e9
-    tag = f"sp_step_{args.precision}_256cube" + ("_xfaces" if faces else "") + ("_powerlaw" if power_law else "")
-    traffic = read_traffic(tag) if edge == 256 else None
+    t_start = time.perf_counter()
+    rec = sp_record(pkg, solid, args.precision, args.steps, args.warmup, nu, g, faces, power_law)
     out = {
         "metric": "fluid MLUPS",
-        "value": round(value, 1),
+        "value": rec["value"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": rec["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -654,23 +687,38 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"sp_d3q19_mrt_{edge}cube_spheres_r8_phi0.35" + ("_xpressure_faces_subbc3" if faces else "")
                    + ("_power_law_n0.8" if power_law else ""), "power_law": power_law,
-                   "fluid_nodes": N, "box": [edge, edge, edge], "precision": args.precision, "tau": 1.0, "g": list(g),
-                   "faces": faces, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "sp_step_kernel", "bytes_per_node": b_node,
-                     "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+                   "fluid_nodes": rec["fluid_nodes"], "box": [edge, edge, edge], "precision": args.precision, "tau": 1.0,
+                   "g": list(g), "faces": faces, "parallelism": "single GPU"},
+        "roofline": rec["roofline"],
+        "launch": rec["launch"],
         "cpu_baseline": None,
     }
-    s.close()
     if not args.no_cpu:
         ref = reference_cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
         port = cpu_baseline(solid, nu, g, args.cpu_seconds, faces, power_law)
         out["cpu_baseline"] = ref if ref is not None else port
         if ref is not None:
             out["cpu_port"] = port   # the FP64 C restatement on the same cores, for comparison
+    default_run = not (faces or power_law) and edge == 256 and args.precision == "fp32"
+    if default_run and not args.no_sub:
+        # the other storage precisions of the same workload, their accuracy at steady state,
+        # then the colour-gradient and coupled-solute workloads
+        for prec in ("fp64", "fp16"):
+            out[prec] = sp_record(pkg, solid, prec, SUB_STEPS, SUB_WARMUP, nu, g)
+            _progress(f"sp {prec}: {out[prec]['value']} MLUPS")
+        out["accuracy"] = accuracy_study(pkg, edge)
+        _progress("accuracy done")
+        cg = cg_bench(pkg, SUB_STEPS, SUB_WARMUP, "fp32", cpu_s=args.cpu_seconds, no_cpu=args.no_cpu)
+        cg["fp64"] = {k: v for k, v in cg_bench(pkg, SUB_STEPS, SUB_WARMUP, "fp64", no_cpu=True).items()
+                      if k in ("value", "unit", "ms_per_step", "roofline", "launch", "steps", "warmup")}
+        cg["accuracy"] = cg_accuracy(pkg)
+        out["cg"] = cg
+        _progress(f"cg: {cg['value']} MLUPS")
+        out["solute"] = solute_bench(pkg, SUB_STEPS, SUB_WARMUP, "fp32", cpu_s=args.cpu_seconds, no_cpu=args.no_cpu)
+        _progress(f"solute: {out['solute']['value']} MLUPS")
     if not args.no_serve:
         out["serving"] = serving_bench(pkg, args.serve_requests, args.serve_rate)
+    out["bench_seconds"] = round(time.perf_counter() - t_start, 1)
     print(json.dumps(out))
 
 

@@ -140,6 +140,14 @@ int lbm_sp_diagnostics(lbm_handle *h, const int32_t *box, lbm_sp_diag *out);
  * out[0] = the largest err_r, out[1] = the largest per-rank u_max, out[2] = the sum of |u| over
  * the ranks (SP:3677-3685).  nranks = 1 is lbm_sp_diagnostics' err, umax and sum_speed. */
 int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3]);
+/* Comput_Grop_Perm (SP:4886-5090), one cube grid in one pass over its nodes: the m^3 cubes of
+ * edge `size` voxels, cube (ix, iy, iz) spanning origin + (ix, iy, iz) * size to that + size - 1
+ * inclusive, at index c = (ix * m + iy) * m + iz (the order of the reference's loops and of the
+ * lines of Perm_<size>.output).  out[4c .. 4c+2] = the sum of u over the cube's fluid nodes,
+ * out[4c+3] = their number; voxels outside the box hold none.  out holds 4 m^3 doubles;
+ * 1 <= size, 1 <= m, m^3 <= 2^24.  A grid's node lists are built on the first call and kept
+ * (the four most recent grids). */
+int lbm_sp_cube_sums(lbm_handle *h, const int32_t origin[3], int32_t size, int32_t m, double *out);
 /* number of time steps taken since create/init/load */
 int lbm_sp_steps_done(const lbm_handle *h, int64_t *steps);
 

@@ -165,6 +165,7 @@ def lib():
         L.lbm_sp_get_dense.argtypes = [vp, dp, dp]
         L.lbm_sp_diagnostics.argtypes = [vp, ip, ctypes.POINTER(SpDiag)]
         L.lbm_sp_error_ranks.argtypes = [vp, ctypes.c_int32, dp]
+        L.lbm_sp_cube_sums.argtypes = [vp, ip, ctypes.c_int32, ctypes.c_int32, dp]
         L.lbm_sp_steps_done.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.lbm_cg_create.argtypes = [ctypes.POINTER(CgParams), ip, ctypes.POINTER(vp)]
         L.lbm_cg_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]

@@ -152,6 +152,12 @@ int lbm_sp_error_ranks(lbm_handle *h, int32_t nranks, double out[3])
     return s ? s->error_ranks(nranks, out) : LBM_ERR_ARG;
 }
 
+int lbm_sp_cube_sums(lbm_handle *h, const int32_t origin[3], int32_t size, int32_t m, double *out)
+{
+    auto *s = as_sp(h);
+    return s ? s->cube_sums(origin, size, m, out) : LBM_ERR_ARG;
+}
+
 int lbm_get_launch(const lbm_handle *h, int32_t kernel, lbm_launch *out)
 {
     if (auto *s = as_sp(h)) return s->get_launch(kernel, out);

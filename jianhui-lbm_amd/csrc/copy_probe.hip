@@ -1,8 +1,10 @@
 // copy_probe.hip -- a plain device-to-device copy kernel of known size.
 //
-// Used two ways: (1) its rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate the counters
-// (known bytes per dispatch / counter reading), so the step kernel's counter readings can be
-// turned into HBM bytes; (2) its bandwidth is the practical copy ceiling of the device.
+// Its rocprofv3 FETCH_SIZE / WRITE_SIZE readings calibrate the counters (known bytes per
+// dispatch / counter reading), so the step kernels' counter readings can be turned into HBM
+// bytes.  It also prints its own bandwidth, for information only: it is a plain grid-stride
+// float4 copy, not tuned, and no ceiling (the single-phase step kernel moves more bytes per
+// second than it does).  A calibration tool, built by `make copy_probe`, not by `make all`.
 //   copy_probe [MiB per buffer, default 1024] [repetitions, default 20]
 #include <hip/hip_runtime.h>
 

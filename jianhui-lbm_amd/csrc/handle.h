@@ -187,6 +187,8 @@ public:
     int diagnostics(const int32_t *box, lbm_sp_diag *out);
     // Error over the reference's nranks x-slab ranks: err, u_max, sum of |u|
     int error_ranks(int nranks, double out[3]);
+    // per cube of a cube grid: sum of u and node count (lbm_sp_cube_sums)
+    int cube_sums(const int32_t origin[3], int32_t size, int32_t m, double *out);
     int set_boundaries(const lbm_sp_bc *bc);
     int set_power_law(double m, double n);
     int get_viscosity(double *nu);
@@ -262,6 +264,20 @@ private:
     double pl_m_ = 0.0, pl_n_ = 0.0;
     Staging stage_;   // host transfers of the FP64 state
     lbm_launch launch_{BLOCK_DEFAULT, 1, 0, 0, 0.0};   // the step's launch settings (runs of 8 tiles)
+    // node lists of cube grids (cube_sums): per cube, entries (64-node group, lane mask) of its
+    // nodes, cube c's entries at off[c] .. off[c+1]
+    struct CubeGrid {
+        int32_t key[5] = {0, 0, 0, 0, 0};   // origin, size, m
+        int64_t ncubes = 0;
+        uint32_t *off = nullptr, *grp = nullptr;
+        uint64_t *mask = nullptr;
+        double *sums = nullptr;             // 4 x ncubes
+        uint64_t used = 0;                  // last use, for replacing the oldest
+    };
+    std::vector<CubeGrid> cube_grids_;
+    uint64_t cube_clock_ = 0;
+    int build_cube_grid(const int32_t origin[3], int32_t size, int32_t m, CubeGrid &g);
+    static void free_cube_grid(CubeGrid &g);
 };
 
 class CgSolver : public lbm_handle {

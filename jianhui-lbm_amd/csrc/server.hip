@@ -17,6 +17,11 @@
 //   nbr      18 x CAP  int16 upstream delta: upstream - node in sample-local indices, taken
 //                      modulo the sample's node count (so the periodic wraps are short too);
 //                      SRV_BB = bounce back.  36 bytes a node instead of 72 for full indices.
+//   nbr_hi   18 x CAP  int16, read only for "wide" samples, whose deltas need 32 bits (more
+//                      than 32,767 fluid nodes between a node and an upstream node, e.g. x-planes
+//                      of more than 32,767 pores): the delta is (nbr_hi << 16) | (uint16) nbr and
+//                      bounce back is INT32_MIN.  A tile belongs to one sample, so the choice is
+//                      uniform over a workgroup.
 // Device tables: slots[] (per-sample parameters), tiles[] (slot, node range of one workgroup).
 // Every sample is cut into tiles from its own first node, and its flux is summed over its own
 // tiles in tile order, so its result is bitwise the same whatever else shares the launches.
@@ -46,12 +51,22 @@ struct SlotDev {
     double gx, gy, gz, sv, sq, rho0;
     int64_t first_step;   // global step of the sample's first (equilibrium) collision
     int64_t count;        // fluid nodes of the sample (the modulus of its neighbour deltas)
+    int32_t wide;         // 1: 32-bit deltas (nbr_hi holds their high halves)
+    int32_t pad;
 };
 
 constexpr int16_t SRV_BB = -32768;
+constexpr int32_t SRV_BB32 = INT32_MIN;
+
+// The delta of a link from its low half (and, for a wide sample, its high half); bounce back
+// comes out as SRV_BB (narrow) or SRV_BB32 (wide).
+__device__ __forceinline__ int32_t srv_delta(int16_t lo, int16_t hi, bool wide)
+{
+    return wide ? (int32_t)(((uint32_t)(uint16_t)hi << 16) | (uint32_t)(uint16_t)lo) : (int32_t)lo;
+}
 
 // Upstream arena node of direction q for node n of a sample whose nodes start at base.
-__device__ __forceinline__ int64_t srv_upstream(int16_t d, int64_t n, int64_t base, int64_t count)
+__device__ __forceinline__ int64_t srv_upstream(int32_t d, int64_t n, int64_t base, int64_t count)
 {
     int64_t s = (n - base) + (int64_t)d;
     s += s < 0 ? count : 0;
@@ -82,14 +97,19 @@ __device__ __forceinline__ const T *byte_at(const T *base, uint32_t off)
     return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
 
-template <typename ST, int BLK>
+// WIDE: the launch covers the tiles of wide samples only (tiles tile0 .. tile0 + gridDim.x - 1;
+// the tile table lists narrow samples' tiles first), so a launch over narrow samples carries no
+// code or registers for the high halves.
+template <typename ST, int BLK, bool WIDE>
 __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
                                                        const ST *__restrict__ src, ST *__restrict__ dst,
-                                                       const int16_t *__restrict__ nbr, int64_t CAP, int64_t step)
+                                                       const int16_t *__restrict__ nbr,
+                                                       const int16_t *__restrict__ nbr_hi, int64_t CAP, int64_t step,
+                                                       int64_t tile0)
 {
     using C = typename Store<ST>::C;
     constexpr uint32_t SZ = sizeof(ST);
-    const TileDev t = tiles[stream_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[tile0 + stream_tile(blockIdx.x, gridDim.x)];
     const int64_t n = t.begin + threadIdx.x;
     if (n >= t.end) return;
     const SlotDev sl = slots[t.slot];
@@ -106,17 +126,24 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict
     } else {
         const int16_t *__restrict__ NB = nbr + t.base;
         const int32_t cnt = (int32_t)sl.count;
+        constexpr int32_t bbv = WIDE ? SRV_BB32 : (int32_t)SRV_BB;
         h[0] = Store<ST>::load(*byte_at(S, ln * SZ), 1.0f);
-        int16_t d[Q - 1];
+        int32_t d[Q - 1];
 #pragma unroll
         for (int q = 1; q < Q; ++q) d[q - 1] = *byte_at(NB, ((uint32_t)(q - 1) * cap + ln) * 2u);
+        if (WIDE) {
+            const int16_t *__restrict__ NH = nbr_hi + t.base;
+#pragma unroll
+            for (int q = 1; q < Q; ++q)
+                d[q - 1] = srv_delta((int16_t)d[q - 1], *byte_at(NH, ((uint32_t)(q - 1) * cap + ln) * 2u), true);
+        }
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
             // upstream node, sample-local: ln + d reduced modulo the node count (|d| <= count/2)
             int32_t u = (int32_t)ln + d[q - 1];
             u += u < 0 ? cnt : 0;
             u -= u >= cnt ? cnt : 0;
-            const uint32_t e = d[q - 1] == SRV_BB ? (uint32_t)opp(q) * cap + ln : (uint32_t)q * cap + (uint32_t)u;
+            const uint32_t e = d[q - 1] == bbv ? (uint32_t)opp(q) * cap + ln : (uint32_t)q * cap + (uint32_t)u;
             h[q] = Store<ST>::load(*byte_at(S, e * SZ), 1.0f);
         }
         moments_fwd(h, m);
@@ -130,13 +157,15 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel(const TileDev *__restrict
 
 // FP64 arena: 64-bit element indices, which keep this kernel at 70 VGPRs (7 waves per SIMD)
 // where the byte-offset form below needs 84 (5 waves)
-template <typename ST, int BLK>
+template <typename ST, int BLK, bool WIDE>
 __global__ void __launch_bounds__(BLK) srv_step_kernel_wide(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
                                                        const ST *__restrict__ src, ST *__restrict__ dst,
-                                                       const int16_t *__restrict__ nbr, int64_t CAP, int64_t step)
+                                                       const int16_t *__restrict__ nbr,
+                                                       const int16_t *__restrict__ nbr_hi, int64_t CAP, int64_t step,
+                                                       int64_t tile0)
 {
     using C = typename Store<ST>::C;
-    const TileDev t = tiles[stream_tile(blockIdx.x, gridDim.x)];
+    const TileDev t = tiles[tile0 + stream_tile(blockIdx.x, gridDim.x)];
     const int64_t n = t.begin + threadIdx.x;
     if (n >= t.end) return;
     const SlotDev sl = slots[t.slot];
@@ -150,12 +179,17 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel_wide(const TileDev *__res
         sp_eq_moments(mc, m);
     } else {
         h[0] = Store<ST>::load(src[n], 1.0f);
-        int16_t d[Q - 1];
+        constexpr int32_t bbv = WIDE ? SRV_BB32 : (int32_t)SRV_BB;
+        int32_t d[Q - 1];
 #pragma unroll
         for (int q = 1; q < Q; ++q) d[q - 1] = nbr[(int64_t)(q - 1) * CAP + n];
+        if (WIDE) {
+#pragma unroll
+            for (int q = 1; q < Q; ++q) d[q - 1] = srv_delta((int16_t)d[q - 1], nbr_hi[(int64_t)(q - 1) * CAP + n], true);
+        }
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
-            const bool bb = d[q - 1] == SRV_BB;
+            const bool bb = d[q - 1] == bbv;
             const int64_t up = srv_upstream(d[q - 1], n, t.base, sl.count);
             h[q] = Store<ST>::load(src[(bb ? (int64_t)opp(q) : (int64_t)q) * CAP + (bb ? n : up)], 1.0f);
         }
@@ -171,7 +205,8 @@ __global__ void __launch_bounds__(BLK) srv_step_kernel_wide(const TileDev *__res
 template <typename ST, int BLK>
 __global__ void __launch_bounds__(BLK) srv_flux_kernel(const TileDev *__restrict__ tiles, const SlotDev *__restrict__ slots,
                                                        const ST *__restrict__ src, const int16_t *__restrict__ nbr,
-                                                       int64_t CAP, double *__restrict__ partial)
+                                                       const int16_t *__restrict__ nbr_hi, int64_t CAP,
+                                                       double *__restrict__ partial)
 {
     using C = typename Store<ST>::C;
     const TileDev t = tiles[blockIdx.x];
@@ -181,10 +216,12 @@ __global__ void __launch_bounds__(BLK) srv_flux_kernel(const TileDev *__restrict
         const SlotDev sl = slots[t.slot];
         C h[Q], m[Q];
         h[0] = Store<ST>::load(src[n], 1.0f);
+        const bool wide = sl.wide != 0;
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
-            const int16_t d = nbr[(int64_t)(q - 1) * CAP + n];
-            const bool bb = d == SRV_BB;
+            const int64_t i = (int64_t)(q - 1) * CAP + n;
+            const int32_t d = srv_delta(nbr[i], wide ? nbr_hi[i] : (int16_t)0, wide);
+            const bool bb = d == (wide ? SRV_BB32 : (int32_t)SRV_BB);
             const int64_t up = srv_upstream(d, n, t.base, sl.count);
             h[q] = Store<ST>::load(src[(bb ? (int64_t)opp(q) : (int64_t)q) * CAP + (bb ? n : up)], 1.0f);
         }
@@ -356,6 +393,9 @@ struct Prepared {
     // 18 x npad upstream deltas (SRV_BB = bounce back); shared with the preparation worker's
     // connectivity test, which may still read them after the sample is queued
     std::shared_ptr<int16_t[]> nbr;
+    // wide sample: the high halves of its 32-bit deltas, 18 x npad (null for a narrow one)
+    std::shared_ptr<int16_t[]> nbr_hi;
+    bool wide = false;
     double nu = 0, g[3] = {0, 0, 0};
     int dir = 1;
     int64_t volume = 0;
@@ -429,37 +469,13 @@ struct PrepJob {
     int dir = 1;
 };
 
-// Largest number of pore voxels in any cyclic run of L consecutive voxels (i-major order).  A
-// pull link from voxel v reaches back at most ny*nz + nz voxels (the xy diagonals), and its
-// node-index distance, taken modulo the node count, is the number of pores in between.
-int64_t max_window_pores(const uint8_t *pore, int64_t V, int64_t L)
-{
-    int64_t c = 0;
-    if (L >= V) {
-        for (int64_t v = 0; v < V; ++v) c += pore[v];
-        return c;
-    }
-    // the run ending at v starts at v - L + 1 (cyclically): the voxel leaving it is v - L + V
-    // while v < L, and v - L after that
-    for (int64_t v = V - L; v < V; ++v) c += pore[v];
-    int64_t best = c;
-    for (int64_t v = 0; v < L; ++v) {
-        c += (int64_t)pore[v] - pore[v - L + V];
-        best = std::max(best, c);
-    }
-    for (int64_t v = L; v < V; ++v) {
-        c += (int64_t)pore[v] - pore[v - L];
-        best = std::max(best, c);
-    }
-    return best;
-}
-
 // The sample's pore numbering (i-major, x slowest: the same as build_sparse_geometry) and its
-// pull links as 16-bit deltas, built in one pass without the 32-bit table: for each link the
-// shortest of up - n, up - n - N, up - n + N (the device adds it to n and reduces modulo N).
-// False if the box has no pores or a link does not fit, which needs an x-plane span of more
-// than 32767 fluid nodes (refused in submit).
-bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64_t &P, std::shared_ptr<int16_t[]> &out)
+// pull links as deltas: for each link the shortest of up - n, up - n - N, up - n + N (the device
+// adds it to n and reduces modulo N).  16-bit deltas when every one fits (out_hi left null);
+// otherwise a second pass writes 32-bit deltas as low halves in out and high halves in out_hi.
+// False if the box has no pores.
+bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64_t &P, std::shared_ptr<int16_t[]> &out,
+                  std::shared_ptr<int16_t[]> &out_hi)
 {
     const int64_t nyz = (int64_t)ny * nz;
     // numbering: pores per x-plane, their prefix, then each plane's pores in order
@@ -483,8 +499,23 @@ bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64
     // every entry is written below (a delta, or SRV_BB for a solid site and for the padding), so
     // the rows are allocated uninitialised and first touched by the threads that fill them
     out = std::shared_ptr<int16_t[]>(new int16_t[(size_t)(18 * P)]);
+    out_hi.reset();
     int16_t *o = out.get();
     const int64_t NN = N;
+    // the shortest delta of each link of node n at (i, j, k), or SRV_BB32 for bounce back
+    auto link = [&](int i, int j, int k, int32_t n, int q) -> int32_t {
+        int iu = i - EX[q], ju = j - EY[q], ku = k - EZ[q];
+        iu = iu < 0 ? iu + nx : (iu >= nx ? iu - nx : iu);
+        ju = ju < 0 ? ju + ny : (ju >= ny ? ju - ny : ju);
+        ku = ku < 0 ? ku + nz : (ku >= nz ? ku - nz : ku);
+        const int32_t up = idx[(size_t)(iu * nyz + (int64_t)ju * nz + ku)];
+        if (up < 0) return SRV_BB32;
+        const int64_t d = (int64_t)up - n;
+        int64_t best = d;
+        if (std::llabs(d - NN) < std::llabs(best)) best = d - NN;
+        if (std::llabs(d + NN) < std::llabs(best)) best = d + NN;
+        return (int32_t)best;   // |best| <= N / 2 < 2^30
+    };
     int bad = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad)
     for (int i = 0; i < nx; ++i)
@@ -493,36 +524,58 @@ bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64
                 const int32_t n = idx[(size_t)(i * nyz + (int64_t)j * nz + k)];
                 if (n < 0) continue;
                 for (int q = 1; q < Q; ++q) {
-                    int iu = i - EX[q], ju = j - EY[q], ku = k - EZ[q];
-                    iu = iu < 0 ? iu + nx : (iu >= nx ? iu - nx : iu);
-                    ju = ju < 0 ? ju + ny : (ju >= ny ? ju - ny : ju);
-                    ku = ku < 0 ? ku + nz : (ku >= nz ? ku - nz : ku);
-                    const int32_t up = idx[(size_t)(iu * nyz + (int64_t)ju * nz + ku)];
-                    int16_t w = SRV_BB;   // bounce back
-                    if (up >= 0) {
-                        const int64_t d = (int64_t)up - n;
-                        int64_t best = d;
-                        if (std::llabs(d - NN) < std::llabs(best)) best = d - NN;
-                        if (std::llabs(d + NN) < std::llabs(best)) best = d + NN;
-                        if (best < -32767 || best > 32767)
+                    const int32_t d = link(i, j, k, n, q);
+                    int16_t w = SRV_BB;
+                    if (d != SRV_BB32) {
+                        if (d < -32767 || d > 32767)
                             bad = 1;
                         else
-                            w = (int16_t)best;
+                            w = (int16_t)d;
                     }
                     o[(size_t)((q - 1) * P + n)] = w;
                 }
             }
-    for (int q = 0; q < 18; ++q)
-        for (int64_t n = N; n < P; ++n) o[(size_t)(q * P + n)] = SRV_BB;
-    return bad == 0;
+    if (bad) {
+        // a wide sample: every link as a 32-bit delta split into halves
+        out_hi = std::shared_ptr<int16_t[]>(new int16_t[(size_t)(18 * P)]);
+        int16_t *oh = out_hi.get();
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const int32_t n = idx[(size_t)(i * nyz + (int64_t)j * nz + k)];
+                    if (n < 0) continue;
+                    for (int q = 1; q < Q; ++q) {
+                        const uint32_t d = (uint32_t)link(i, j, k, n, q);
+                        o[(size_t)((q - 1) * P + n)] = (int16_t)(uint16_t)(d & 0xFFFFu);
+                        oh[(size_t)((q - 1) * P + n)] = (int16_t)(uint16_t)(d >> 16);
+                    }
+                }
+        for (int q = 0; q < 18; ++q)
+            for (int64_t n = N; n < P; ++n) {
+                o[(size_t)(q * P + n)] = 0;
+                oh[(size_t)(q * P + n)] = (int16_t)(uint16_t)((uint32_t)SRV_BB32 >> 16);
+            }
+    } else {
+        for (int q = 0; q < 18; ++q)
+            for (int64_t n = N; n < P; ++n) o[(size_t)(q * P + n)] = SRV_BB;
+    }
+    return true;
+}
+
+// The 32-bit delta of entry i (narrow samples: hi null); bounce back as SRV_BB32.
+inline int32_t host_delta(const int16_t *lo, const int16_t *hi, size_t i)
+{
+    if (!hi) return lo[i] == SRV_BB ? SRV_BB32 : (int32_t)lo[i];
+    return (int32_t)(((uint32_t)(uint16_t)hi[i] << 16) | (uint32_t)(uint16_t)lo[i]);
 }
 
 // Whether the sample's pores wrap round the box along axis, from its deltas (connectivity.h).
-bool deltas_wrap_along(const int16_t *nbr, int64_t N, int64_t P, int axis)
+bool deltas_wrap_along(const int16_t *nbr, const int16_t *nbr_hi, int64_t N, int64_t P, int axis)
 {
-    return lattice_loop_wraps(N, axis, [nbr, N, P](int64_t n, int q) -> int64_t {
-        const int16_t d = nbr[(size_t)((q - 1) * P + n)];
-        if (d == SRV_BB) return -1;
+    return lattice_loop_wraps(N, axis, [nbr, nbr_hi, N, P](int64_t n, int q) -> int64_t {
+        const int32_t d = host_delta(nbr, nbr_hi, (size_t)((q - 1) * P + n));
+        if (d == SRV_BB32) return -1;
         int64_t m = n + d;
         m += m < 0 ? N : 0;
         m -= m >= N ? N : 0;
@@ -555,7 +608,7 @@ private:
     bool make_fallback(int64_t min_nodes);
     int run_chunk(std::unique_lock<std::mutex> &lk);
     bool try_admit_locked();
-    int rebuild_tables();
+    int rebuild_tables(std::string &msg);
     int64_t alloc_segment(int64_t len);
     void free_segment(int64_t start, int64_t len);
     template <int BLK>
@@ -578,6 +631,7 @@ private:
     void *pops_[2] = {nullptr, nullptr};
     int cur_ = 0;
     int16_t *d_nbr_ = nullptr;
+    int16_t *d_nbr_hi_ = nullptr;   // high halves of wide samples' deltas
     // Anderson acceleration: X, R_prev, G_prev, then m dR and m dG vectors, each 19 x CAP
     int aa_m_ = 0, aa_period_ = 0, aa_warm_ = 0;
     void *d_aa_vec_ = nullptr;
@@ -590,6 +644,7 @@ private:
     double *d_partial_ = nullptr, *d_sums_ = nullptr, *h_sums_ = nullptr;
     int64_t max_slots_ = 0, max_tiles_ = 0;
     int64_t ntiles_ = 0;
+    int64_t ntiles_narrow_ = 0;   // tiles 0 .. ntiles_narrow_ - 1 belong to narrow samples
     int64_t global_step_ = 0;
 
     std::mutex mu_;
@@ -659,6 +714,12 @@ bool Server::escalate_locked(Active &a, bool restart)
     hipError_t e = hipMemcpy2DAsync(a.req.nbr.get(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_ + a.base,
                                     (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                     hipMemcpyDeviceToHost, stream_);
+    if (a.req.wide && e == hipSuccess) {
+        a.req.nbr_hi = std::shared_ptr<int16_t[]>(new int16_t[(size_t)18 * a.req.npad]);
+        e = hipMemcpy2DAsync(a.req.nbr_hi.get(), (size_t)a.req.npad * sizeof(int16_t), d_nbr_hi_ + a.base,
+                             (size_t)cap_ * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
+                             hipMemcpyDeviceToHost, stream_);
+    }
     // otherwise the FP64 solve continues from the FP32 state rather than from rest: the
     // transient so far (the pressure build-up of the slow mode) is kept
     if (!restart) {
@@ -738,7 +799,7 @@ int Server::create(const lbm_srv_params &p)
         LBM_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
         const double frac = p.mem_fraction > 0 && p.mem_fraction < 0.95 ? p.mem_fraction : 0.5;
         const double bytes = std::min(frac * (double)free_b, 64e9);
-        cap = (int64_t)(bytes / (double)(2 * Q * st_ + 18 * sizeof(int16_t) + (aa_m_ ? (3 + 2 * aa_m_) * Q * st_ : 0)));
+        cap = (int64_t)(bytes / (double)(2 * Q * st_ + 2 * 18 * sizeof(int16_t) + (aa_m_ ? (3 + 2 * aa_m_) * Q * st_ : 0)));
     }
     // the step kernel's 32-bit byte offsets reach (19 * CAP - 1) * sizeof(value)
     cap = std::min<int64_t>(cap, (int64_t)(0xffffffffu / (Q * st_)) - 511);
@@ -746,6 +807,7 @@ int Server::create(const lbm_srv_params &p)
     if (cap_ < 512) return fail(LBM_ERR_ARG, "arena too small");
     for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * cap_ * st_));
     LBM_HIP_TRY(hipMalloc(&d_nbr_, (size_t)18 * cap_ * sizeof(int16_t)));
+    LBM_HIP_TRY(hipMalloc(&d_nbr_hi_, (size_t)18 * cap_ * sizeof(int16_t)));
     max_slots_ = 1 << 16;
     max_tiles_ = cap_ / 128 + 1;
     LBM_HIP_TRY(hipMalloc(&d_slots_, (size_t)max_slots_ * sizeof(SlotDev)));
@@ -796,7 +858,7 @@ void Server::shutdown()
     (void)hipSetDevice(device_);
     for (void *&b : pops_)
         if (b) (void)hipFree(b), b = nullptr;
-    void **bufs[] = {(void **)&d_nbr_, (void **)&d_slots_, (void **)&d_tiles_, (void **)&d_tlo_, (void **)&d_thi_,
+    void **bufs[] = {(void **)&d_nbr_, (void **)&d_nbr_hi_, (void **)&d_slots_, (void **)&d_tiles_, (void **)&d_tlo_, (void **)&d_thi_,
                      (void **)&d_partial_, (void **)&d_sums_, &d_aa_vec_, (void **)&d_aa_, (void **)&d_aa_partial_,
                      (void **)&d_aa_sums_};
     for (void **b : bufs)
@@ -830,8 +892,6 @@ int Server::submit(const lbm_srv_request &r, int64_t *ticket)
     }
     if (N == 0) return fail(LBM_ERR_ARG, "geometry has no pore voxels");
     if (padded_stride(N) > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
-    if (max_window_pores(job.pore.data(), V, (int64_t)r.ny * r.nz + r.nz) > 32767)
-        return fail(LBM_ERR_ARG, "sample has more than 32767 fluid nodes within one x-plane span (solve it with lbm_sp_*)");
     job.nx = r.nx;
     job.ny = r.ny;
     job.nz = r.nz;
@@ -868,10 +928,12 @@ void Server::prep_worker()
         omp_set_num_threads(std::max(1, std::min(8, prep_threads_ / std::max(1, sharing))));
         Prepared pr;
         int64_t N = 0, P = 0;
-        std::shared_ptr<int16_t[]> nbr;
-        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.pore.data(), N, P, nbr);   // fits: checked in submit
+        std::shared_ptr<int16_t[]> nbr, nbr_hi;
+        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.pore.data(), N, P, nbr, nbr_hi);
         std::vector<uint8_t>().swap(job.pore);
         pr.nbr = nbr;
+        pr.nbr_hi = nbr_hi;
+        pr.wide = nbr_hi != nullptr;
         pr.ticket = job.ticket;
         pr.t_submit = job.t_submit;
         pr.N = N;
@@ -904,7 +966,7 @@ void Server::prep_worker()
         // below what FP32 deviations resolve.  With escalation on it goes to the FP64 server:
         // at once if it is still queued, at its next check if it has been admitted meanwhile.
         // The test runs after queueing so that it does not delay admission.
-        if (test_wrap && !deltas_wrap_along(nbr.get(), N, P, job.dir - 1)) {
+        if (test_wrap && !deltas_wrap_along(nbr.get(), nbr_hi.get(), N, P, job.dir - 1)) {
             std::lock_guard<std::mutex> lk(mu_);
             for (auto it = pending_.begin(); it != pending_.end(); ++it)
                 if (it->ticket == job.ticket) {
@@ -1017,6 +1079,10 @@ bool Server::try_admit_locked()
         hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.get(),
                                         (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                         hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess && a.req.wide)
+            e = hipMemcpy2DAsync(d_nbr_hi_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr_hi.get(),
+                                 (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
+                                 hipMemcpyHostToDevice, stream_);
         if (e == hipSuccess && !a.req.carried.empty()) {
             // an escalated sample resumes from its FP32 state: no equilibrium step
             std::vector<double> st(a.req.carried.begin(), a.req.carried.end());
@@ -1044,12 +1110,17 @@ bool Server::try_admit_locked()
             loop_error_ = true;
             loop_msg_ = std::string("neighbour upload failed: ") + hipGetErrorString(e);
         }
-        for (Active &a : active_) a.req.nbr.reset();
+        for (Active &a : active_) {
+            a.req.nbr.reset();
+            a.req.nbr_hi.reset();
+        }
     }
     return any;
 }
 
-int Server::rebuild_tables()
+// Called on the loop thread (and by autotune at create): errors are returned with their
+// message in msg, not through fail(), so the loop thread never writes last_error.
+int Server::rebuild_tables(std::string &msg)
 {
     // order active samples by arena position (keeps neighbouring tiles adjacent in memory)
     std::sort(active_.begin(), active_.end(), [](const Active &a, const Active &b) { return a.base < b.base; });
@@ -1069,27 +1140,47 @@ int Server::rebuild_tables()
         d.rho0 = 1.0;
         d.first_step = a.first_step;
         d.count = a.req.N;
+        d.wide = a.req.wide ? 1 : 0;
         slots[s] = d;
-        tlo[s] = (int32_t)tiles.size();
-        for (int64_t b = 0; b < a.req.N; b += blk_) {
-            TileDev t{};
-            t.slot = (int32_t)s;
-            t.begin = a.base + b;
-            t.end = a.base + std::min<int64_t>(b + blk_, a.req.N);
-            t.base = a.base;
-            tiles.push_back(t);
+    }
+    // the tiles of narrow samples first, then those of wide ones (each launched by its own
+    // kernel); a sample's tiles stay consecutive and in node order
+    for (int pass = 0; pass < 2; ++pass) {
+        for (size_t s = 0; s < active_.size(); ++s) {
+            const Active &a = active_[s];
+            if (a.req.wide != (pass == 1)) continue;
+            tlo[s] = (int32_t)tiles.size();
+            for (int64_t b = 0; b < a.req.N; b += blk_) {
+                TileDev t{};
+                t.slot = (int32_t)s;
+                t.begin = a.base + b;
+                t.end = a.base + std::min<int64_t>(b + blk_, a.req.N);
+                t.base = a.base;
+                tiles.push_back(t);
+            }
+            thi[s] = (int32_t)tiles.size();
         }
-        thi[s] = (int32_t)tiles.size();
+        if (pass == 0) ntiles_narrow_ = (int64_t)tiles.size();
     }
     ntiles_ = (int64_t)tiles.size();
-    if (ntiles_ > max_tiles_) return fail(LBM_ERR_ARG, "tile table overflow");
+    if (ntiles_ > max_tiles_) {
+        msg = "tile table overflow";
+        return LBM_ERR_ARG;
+    }
     if (!slots.empty()) {
-        LBM_HIP_TRY(hipMemcpyAsync(d_slots_, slots.data(), slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice, stream_));
-        LBM_HIP_TRY(hipMemcpyAsync(d_tiles_, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, stream_));
-        LBM_HIP_TRY(hipMemcpyAsync(d_tlo_, tlo.data(), tlo.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
-        LBM_HIP_TRY(hipMemcpyAsync(d_thi_, thi.data(), thi.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+        hipError_t e = hipMemcpyAsync(d_slots_, slots.data(), slots.size() * sizeof(SlotDev), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tiles_, tiles.data(), tiles.size() * sizeof(TileDev), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_tlo_, tlo.data(), tlo.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_thi_, thi.data(), thi.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_);
         // pageable sources: make sure the copies have consumed them before the vectors go away
-        LBM_HIP_TRY(hipStreamSynchronize(stream_));
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        if (e != hipSuccess) {
+            msg = std::string("server table upload failed: ") + hipGetErrorString(e);
+            return LBM_ERR_HIP;
+        }
     }
     return LBM_OK;
 }
@@ -1097,13 +1188,27 @@ int Server::rebuild_tables()
 template <int BLK>
 hipError_t Server::launch_steps(int nsteps)
 {
+    const int64_t nn = ntiles_narrow_, nw = ntiles_ - ntiles_narrow_;
     for (int s = 0; s < nsteps; ++s) {
-        if (prm_.precision == LBM_FP64)
-            srv_step_kernel_wide<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
-                d_tiles_, d_slots_, (const double *)pops_[cur_], (double *)pops_[cur_ ^ 1], d_nbr_, cap_, global_step_);
-        else
-            srv_step_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(
-                d_tiles_, d_slots_, (const float *)pops_[cur_], (float *)pops_[cur_ ^ 1], d_nbr_, cap_, global_step_);
+        if (prm_.precision == LBM_FP64) {
+            const double *src = (const double *)pops_[cur_];
+            double *dst = (double *)pops_[cur_ ^ 1];
+            if (nn > 0)
+                srv_step_kernel_wide<double, BLK, false><<<(unsigned)nn, BLK, 0, stream_>>>(
+                    d_tiles_, d_slots_, src, dst, d_nbr_, d_nbr_hi_, cap_, global_step_, 0);
+            if (nw > 0)
+                srv_step_kernel_wide<double, BLK, true><<<(unsigned)nw, BLK, 0, stream_>>>(
+                    d_tiles_, d_slots_, src, dst, d_nbr_, d_nbr_hi_, cap_, global_step_, nn);
+        } else {
+            const float *src = (const float *)pops_[cur_];
+            float *dst = (float *)pops_[cur_ ^ 1];
+            if (nn > 0)
+                srv_step_kernel<float, BLK, false><<<(unsigned)nn, BLK, 0, stream_>>>(
+                    d_tiles_, d_slots_, src, dst, d_nbr_, d_nbr_hi_, cap_, global_step_, 0);
+            if (nw > 0)
+                srv_step_kernel<float, BLK, true><<<(unsigned)nw, BLK, 0, stream_>>>(
+                    d_tiles_, d_slots_, src, dst, d_nbr_, d_nbr_hi_, cap_, global_step_, nn);
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         cur_ ^= 1;
@@ -1126,10 +1231,10 @@ hipError_t Server::launch_flux()
 {
     if (prm_.precision == LBM_FP64)
         srv_flux_kernel<double, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(d_tiles_, d_slots_, (const double *)pops_[cur_],
-                                                                             d_nbr_, cap_, d_partial_);
+                                                                             d_nbr_, d_nbr_hi_, cap_, d_partial_);
     else
         srv_flux_kernel<float, BLK><<<(unsigned)ntiles_, BLK, 0, stream_>>>(d_tiles_, d_slots_, (const float *)pops_[cur_],
-                                                                            d_nbr_, cap_, d_partial_);
+                                                                            d_nbr_, d_nbr_hi_, cap_, d_partial_);
     return hipGetLastError();
 }
 
@@ -1357,9 +1462,10 @@ void Server::loop()
             continue;
         }
         if (tables_dirty) {
-            if (rebuild_tables() != LBM_OK) {
+            std::string msg;
+            if (rebuild_tables(msg) != LBM_OK) {
                 loop_error_ = true;
-                loop_msg_ = last_error;
+                loop_msg_ = msg;
                 break;
             }
             tables_dirty = false;
@@ -1386,8 +1492,8 @@ int Server::autotune()
                 pore[((size_t)i * L + j) * L + k] = v > -0.05 ? 0 : 1;
             }
     int64_t N = 0, P = 0;
-    std::shared_ptr<int16_t[]> d16;
-    if (!build_deltas(L, L, L, pore.data(), N, P, d16) || P > cap_) {
+    std::shared_ptr<int16_t[]> d16, d16_hi;
+    if (!build_deltas(L, L, L, pore.data(), N, P, d16, d16_hi) || d16_hi || P > cap_) {
         blk_ = 256;
         return LBM_OK;
     }
@@ -1408,7 +1514,8 @@ int Server::autotune()
     for (int blk : {128, 256, 512}) {
         blk_ = blk;
         global_step_ = 0;
-        if (rebuild_tables() != LBM_OK) break;
+        std::string msg;
+        if (rebuild_tables(msg) != LBM_OK) break;
         LBM_HIP_TRY(launch_steps_dispatch(5));   // warm-up (first step is the equilibrium init)
         LBM_HIP_TRY(hipEventRecord(ev0_, stream_));
         LBM_HIP_TRY(launch_steps_dispatch(40));

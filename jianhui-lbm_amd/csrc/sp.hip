@@ -253,6 +253,44 @@ __global__ void __launch_bounds__(BLOCK) sp_diag_kernel(SpArgs<ST> a, const int3
     block_reduce_store<NDIAG + 1>(v, is_max, partial + (int64_t)blockIdx.x * (NDIAG + 1));
 }
 
+// Sums of u and node counts per cube of a cube grid (Comput_Grop_Perm, SP:4886-5090), one
+// workgroup per cube.  A cube's nodes are listed as entries (64-node group, lane mask); a wave
+// takes one entry at a time, so its active lanes always lie in one group (as load_nbr and
+// bc_patch require).  Each thread sums its nodes in a fixed order and the block reduction has a
+// fixed tree: the sums are the same on every call.
+template <typename ST, int MODE>
+__global__ void __launch_bounds__(BLOCK) sp_cube_kernel(SpArgs<ST> a, const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ grp,
+                                                        const uint64_t *__restrict__ mask, double *__restrict__ out)
+{
+    using C = typename Store<ST>::C;
+    const int64_t c = blockIdx.x;
+    const uint32_t e0 = off[c], e1 = off[c + 1];
+    const uint32_t lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t e = e0 + wid; e < e1; e += BLOCK / WAVE) {
+        const uint64_t mk = mask[e];
+        const int64_t n = (int64_t)grp[e] * WAVE + lane;
+        if (!((mk >> lane) & 1ull)) continue;
+        C h[Q], m[Q];
+        Macro<C> mc;
+        if (MODE == K_INIT) {
+            mc = node_macro<ST, MODE>(a, n, m);
+        } else {
+            gather<ST, MODE>(a, n, h);
+            if (MODE == K_PULL && a.bc.wave) bc_patch(a.bc, n, a.rho0, h);
+            moments_fwd(h, m);
+            mc = node_macro<ST, MODE>(a, n, m);
+        }
+        v[0] += (double)mc.ux;
+        v[1] += (double)mc.uy;
+        v[2] += (double)mc.uz;
+        v[3] += 1.0;
+    }
+    const bool is_max[4] = {false, false, false, false};
+    block_reduce_store<4>(v, is_max, out + c * 4);
+}
+
 // ---------------------------------------------------------------- state I/O
 // Post-streaming f, u and rho of the nodes n0 <= n < n1.  Without a voxel map the node's values
 // land at local index n - n0 of the outputs; with one (dense output) at supinv[n] - v0.  The
@@ -435,6 +473,21 @@ hipError_t launch_diag_t(SpArgs<ST> a, const BcDev &bc, int mode, const int32_t 
 }
 
 template <typename ST>
+hipError_t launch_cube_t(SpArgs<ST> a, const BcDev &bc, int mode, const uint32_t *off, const uint32_t *grp,
+                         const uint64_t *mask, int64_t ncubes, double *out, hipStream_t s)
+{
+    dim3 g((unsigned)ncubes), b(BLOCK);
+    a.bc = bc;
+    if (mode == K_INIT)
+        sp_cube_kernel<ST, K_INIT><<<g, b, 0, s>>>(a, off, grp, mask, out);
+    else if (mode == K_DIRECT)
+        sp_cube_kernel<ST, K_DIRECT><<<g, b, 0, s>>>(a, off, grp, mask, out);
+    else
+        sp_cube_kernel<ST, K_PULL><<<g, b, 0, s>>>(a, off, grp, mask, out);
+    return hipGetLastError();
+}
+
+template <typename ST>
 hipError_t launch_get_t(SpArgs<ST> a, const BcDev &bc, int mode, int64_t n0, int64_t n1, const int32_t *supinv,
                         int64_t v0, double *f, double *u, double *rho, hipStream_t s)
 {
@@ -462,6 +515,8 @@ int SpSolver::release()
     for (void *&p : pops_)
         if (p) (void)hipFree(p), p = nullptr;
     nbr_.release();
+    for (CubeGrid &g : cube_grids_) free_cube_grid(g);
+    cube_grids_.clear();
     if (d_supinv_) (void)hipFree(d_supinv_), d_supinv_ = nullptr;
     if (d_u_) (void)hipFree(d_u_), d_u_ = nullptr;
     if (d_rho_) (void)hipFree(d_rho_), d_rho_ = nullptr;
@@ -978,6 +1033,113 @@ int SpSolver::error_ranks(int nranks, double out[3])
     out[0] = err;
     out[1] = umax;
     out[2] = speed;
+    return LBM_OK;
+}
+
+void SpSolver::free_cube_grid(CubeGrid &g)
+{
+    for (void *p : {(void *)g.off, (void *)g.grp, (void *)g.mask, (void *)g.sums})
+        if (p) (void)hipFree(p);
+    g = CubeGrid{};
+}
+
+// The entries of every cube of the grid: for each (i, j) line of a cube (x, then y, ascending)
+// the nodes with k in the cube's z range are one run of consecutive ids (found by binary search
+// in supinv, which is ascending), so a cube's nodes come in ascending id order and a group shared
+// by two of its runs is the last entry when the second run reaches it.
+int SpSolver::build_cube_grid(const int32_t origin[3], int32_t size, int32_t m, CubeGrid &g)
+{
+    const int64_t nx = geo_.nx, ny = geo_.ny, nz = geo_.nz;
+    const std::vector<int32_t> &sup = geo_.supinv;
+    const int64_t ncubes = (int64_t)m * m * m;
+    std::vector<uint32_t> off((size_t)ncubes + 1, 0u), grp;
+    std::vector<uint64_t> mask;
+    auto first_at_or_after = [&](int64_t voxel) {
+        return (int64_t)(std::lower_bound(sup.begin(), sup.begin() + geo_.N, voxel) - sup.begin());
+    };
+    int64_t c = 0;
+    for (int64_t ix = 0; ix < m; ++ix)
+        for (int64_t iy = 0; iy < m; ++iy)
+            for (int64_t iz = 0; iz < m; ++iz, ++c) {
+                const int64_t x0 = origin[0] + ix * size, y0 = origin[1] + iy * size, z0 = origin[2] + iz * size;
+                const int64_t zlo = std::max<int64_t>(z0, 0), zhi = std::min<int64_t>(z0 + size - 1, nz - 1);
+                const size_t first = grp.size();
+                if (zlo <= zhi)
+                    for (int64_t i = std::max<int64_t>(x0, 0); i <= std::min<int64_t>(x0 + size - 1, nx - 1); ++i)
+                        for (int64_t j = std::max<int64_t>(y0, 0); j <= std::min<int64_t>(y0 + size - 1, ny - 1); ++j) {
+                            const int64_t base = (i * ny + j) * nz;
+                            const int64_t lo = first_at_or_after(base + zlo), hi = first_at_or_after(base + zhi + 1);
+                            for (int64_t n = lo; n < hi; ++n) {
+                                const uint32_t w = (uint32_t)(n >> 6);
+                                if (grp.size() == first || grp.back() != w) {
+                                    grp.push_back(w);
+                                    mask.push_back(0ull);
+                                }
+                                mask.back() |= 1ull << (n & 63);
+                            }
+                        }
+                if (grp.size() > 0xFFFFFFF0ull) return fail(LBM_ERR_ARG, "cube grid: too many node groups");
+                off[(size_t)c + 1] = (uint32_t)grp.size();
+            }
+    const size_t ne = std::max<size_t>(grp.size(), 1);
+    grp.resize(ne, 0u);
+    mask.resize(ne, 0ull);
+    LBM_HIP_TRY(hipMalloc(&g.off, off.size() * sizeof(uint32_t)));
+    LBM_HIP_TRY(hipMalloc(&g.grp, ne * sizeof(uint32_t)));
+    LBM_HIP_TRY(hipMalloc(&g.mask, ne * sizeof(uint64_t)));
+    LBM_HIP_TRY(hipMalloc(&g.sums, (size_t)ncubes * 4 * sizeof(double)));
+    LBM_HIP_TRY(hipMemcpyAsync(g.off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemcpyAsync(g.grp, grp.data(), ne * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipMemcpyAsync(g.mask, mask.data(), ne * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));   // the host vectors go out of scope
+    g.ncubes = ncubes;
+    return LBM_OK;
+}
+
+int SpSolver::cube_sums(const int32_t origin[3], int32_t size, int32_t m, double *out)
+{
+    if (!origin || !out) return fail(LBM_ERR_ARG, "null argument");
+    if (size < 1 || m < 1 || m > 256) return fail(LBM_ERR_ARG, "cube grid: size >= 1 and 1 <= m <= 256");
+    if (mode_ == STATE_EMPTY) return fail(LBM_ERR_STATE, "state not initialised");
+    LBM_HIP_TRY(hipSetDevice(device_));
+    const int32_t key[5] = {origin[0], origin[1], origin[2], size, m};
+    CubeGrid *g = nullptr;
+    for (CubeGrid &cg : cube_grids_)
+        if (std::equal(key, key + 5, cg.key)) g = &cg;
+    if (!g) {
+        if (cube_grids_.size() < 4) {
+            cube_grids_.emplace_back();
+            g = &cube_grids_.back();
+        } else {
+            g = &*std::min_element(cube_grids_.begin(), cube_grids_.end(),
+                                   [](const CubeGrid &a, const CubeGrid &b) { return a.used < b.used; });
+            free_cube_grid(*g);
+        }
+        const int rc = build_cube_grid(origin, size, m, *g);
+        if (rc != LBM_OK) {
+            free_cube_grid(*g);
+            return rc;
+        }
+        std::copy(key, key + 5, g->key);
+    }
+    g->used = ++cube_clock_;
+    const int mode = kmode_of(mode_);
+    hipError_t e;
+    if (prm_.precision == LBM_FP64)
+        e = launch_cube_t(make_args<double>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, g->off, g->grp, g->mask, g->ncubes, g->sums, stream_);
+    else if (prm_.precision == LBM_FP32)
+        e = launch_cube_t(make_args<float>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                           rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, g->off, g->grp, g->mask, g->ncubes, g->sums, stream_);
+    else
+        e = launch_cube_t(make_args<__half>(pops_[cur_], nullptr, nbr_.view(), d_u_, d_rho_, geo_.N, geo_.P, prm_, sv_, sq_,
+                                            rho_i_, u_i_, scale_, inv_scale_),
+                          bc_view(false), mode, g->off, g->grp, g->mask, g->ncubes, g->sums, stream_);
+    LBM_HIP_TRY(e);
+    LBM_HIP_TRY(hipMemcpyAsync(out, g->sums, (size_t)g->ncubes * 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    LBM_HIP_TRY(hipStreamSynchronize(stream_));
     return LBM_OK;
 }
 

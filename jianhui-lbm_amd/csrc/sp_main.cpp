@@ -278,21 +278,18 @@ int main(int argc, char **argv)
             // Comput_Grop_Perm (SP:4886-5090): for each of the four cube grids, the permeability
             // along PerDir and the porosity of every cube (origin c + i*size, edge size, x then y
             // then z), one line per cube in Perm_<size>.output, rewritten at each call
+            // (all cubes of a grid in one launch, lbm_sp_cube_sums)
             for (int gi = 0; gi < 4; ++gi) {
                 const int L = in.size_gperm[gi], m = in.n_gperm[gi];
                 std::ofstream fo(out + "Perm_" + std::to_string(L) + ".output", std::ios::out);
+                if (m < 1 || L < 1) continue;
                 const double vol = (double)L * L * L;
-                for (int ix = 0; ix < m; ++ix)
-                    for (int iy = 0; iy < m; ++iy)
-                        for (int iz = 0; iz < m; ++iz) {
-                            const int32_t b[6] = {in.c_gperm[gi][0] + ix * L, in.c_gperm[gi][0] + ix * L + L - 1,
-                                                  in.c_gperm[gi][1] + iy * L, in.c_gperm[gi][1] + iy * L + L - 1,
-                                                  in.c_gperm[gi][2] + iz * L, in.c_gperm[gi][2] + iz * L + L - 1};
-                            lbm_sp_diag dg{};
-                            if (check(lbm_sp_diagnostics(h, b, &dg), h, "lbm_sp_diagnostics")) return 1;
-                            const double k = dg.Q[dir] / vol * in.in_vis / (in.g[dir] + dp);
-                            fo << k * rr << " " << dg.count_in_box / vol << std::endl;
-                        }
+                std::vector<double> cs((size_t)4 * m * m * m);
+                if (check(lbm_sp_cube_sums(h, in.c_gperm[gi], L, m, cs.data()), h, "lbm_sp_cube_sums")) return 1;
+                for (size_t c = 0; c < (size_t)m * m * m; ++c) {
+                    const double k = cs[4 * c + dir] / vol * in.in_vis / (in.g[dir] + dp);
+                    fo << k * rr << " " << cs[4 * c + 3] / vol << std::endl;
+                }
             }
         }
         {

@@ -20,6 +20,10 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
 class SinglePhase:
     """Sparse D3Q19 MRT single-phase solver on one GPU.
 
@@ -245,15 +249,16 @@ class SinglePhase:
         out = []
         for m, L, c in zip(n, size, origins):
             vol = float(L) ** 3
-            rows = []
-            for ix in range(m):
-                for iy in range(m):
-                    for iz in range(m):
-                        b = (c[0] + ix * L, c[0] + ix * L + L - 1, c[1] + iy * L, c[1] + iy * L + L - 1,
-                             c[2] + iz * L, c[2] + iz * L + L - 1)
-                        d = self.diagnostics(b)
-                        rows.append((d["Q"][i] / vol * self.nu / (self.g[i] + dp), d["count_in_box"] / vol))
-            out.append(rows)
+            sums = self.cube_sums(c, L, m)
+            out.append([(q[i] / vol * self.nu / (self.g[i] + dp), q[3] / vol) for q in sums])
+        return out
+
+    def cube_sums(self, origin: Sequence[int], size: int, m: int) -> np.ndarray:
+        """(m^3, 4): per cube of the grid (index (ix*m + iy)*m + iz), the sum of u over its fluid
+        nodes and their number, in one launch (lbm_sp_cube_sums)."""
+        out = np.zeros((int(m) ** 3, 4))
+        o = np.ascontiguousarray(origin, np.int32)
+        _capi.check(_capi.lib().lbm_sp_cube_sums(self._h, _ip(o), int(size), int(m), _dp(out)), self._h)
         return out
 
 

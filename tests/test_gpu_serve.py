@@ -1,5 +1,6 @@
 """Continuous-batching permeability server: results independent of batch mates, admission by
 HBM footprint, and agreement with a plain single-sample solve."""
+import os
 import sys
 import time
 
@@ -213,15 +214,75 @@ def test_rejects_bad_requests(pkg):
             srv.submit(np.zeros((8, 8, 8), np.int32), 0.1, (0.0, 1e-5, 0))    # no force along x
         with pytest.raises(pkg.LBMError):
             srv.submit(np.zeros((64, 64, 64), np.int32), 0.1, (1e-5, 0, 0))   # larger than the arena
-    with pkg.PermeabilityServer(capacity_nodes=200_000, block=256) as srv:
-        with pytest.raises(pkg.LBMError):
-            srv.submit(np.zeros((3, 190, 190), np.int32), 0.1, (1e-5, 0, 0))  # x-plane of 36100 nodes
     with pytest.raises(pkg.LBMError):
         pkg.PermeabilityServer(capacity_nodes=50_000, block=256, precision="fp16")
     with pytest.raises(pkg.LBMError):
         pkg.PermeabilityServer(capacity_nodes=50_000, block=256, accel_depth=9)
     with pytest.raises(pkg.LBMError):
         pkg.PermeabilityServer(capacity_nodes=50_000, block=256, check_interval=100, accel_depth=3, accel_period=150)
+
+
+def test_fp64_arena_matches_the_oracle(pkg, oracle_mod):
+    """The FP64 arena's k on a 32^3 Gaussian-random-field sample against the CPU oracle stepped
+    the same number of steps (Comput_Perm, SP:4617-4677), to 1e-12."""
+    _log("[gpu] serve: fp64 arena vs oracle")
+    s = pkg.geometry.grf_sample(32, 3.0, 0.35, seed=102)
+    nu, g = 1.0 / 6.0, (1e-5, 0.0, 0.0)
+    with pkg.PermeabilityServer(capacity_nodes=200_000, check_interval=100, tol=1e-6, block=256,
+                                precision="fp64") as srv:
+        r = _run(srv, [(s, nu, g)])[0]
+    assert r["converged"] == 1 and r["precision"] == "fp64"
+    o = oracle_mod.SPOracle(s.astype(np.int32), nu, g, nthreads=max(1, min(16, os.cpu_count() or 1)))
+    o.run(r["steps"])
+    k = o.flux()[0] / s.size * nu / g[0]
+    _log(f"[gpu] fp64 arena k {r['k'][0]:.15e}, oracle {k:.15e} after {r['steps']} steps")
+    assert abs(r["k"][0] - k) <= 1e-12 * abs(k)
+
+
+def test_diverging_request_is_isolated(pkg):
+    """A request whose body force makes it blow up retires with converged = -1 (non-finite
+    velocity) while its batch mates run on: their results are bit for bit those they get
+    alone.  (The reference stops the whole run on a NaN, SP:838-839.)"""
+    _log("[gpu] serve: diverging request isolated")
+    samples = _samples(pkg)[:3]
+    good = [(s, 1.0 / 6.0, (1e-5, 0.0, 0.0)) for s in samples]
+    bad = (samples[1], 1.0 / 6.0, (50.0, 0.0, 0.0))
+    kw = dict(capacity_nodes=400_000, check_interval=100, tol=1e-6, block=256, **AA)
+    with pkg.PermeabilityServer(**kw) as srv:
+        alone = _run(srv, good)
+    with pkg.PermeabilityServer(**kw) as srv:
+        mixed = _run(srv, [good[0], bad, good[1], good[2]])
+    assert mixed[1]["converged"] == -1
+    for a, m in zip(alone, [mixed[0], mixed[2], mixed[3]]):
+        assert a["converged"] == 1 and m["converged"] == 1
+        assert a["steps"] == m["steps"]
+        np.testing.assert_array_equal(a["k"], m["k"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_wide_sample_matches_plain_solver(pkg, precision):
+    """x-planes of more than 32,767 pores put some links' upstream nodes beyond a 16-bit delta:
+    such a sample runs on 32-bit deltas (its own launch).  Its k is the plain solver's, and a
+    narrow sample sharing the arena with it gives the same k as alone."""
+    _log(f"[gpu] serve: wide sample, {precision}")
+    rng = np.random.default_rng(77)
+    wide = (rng.uniform(size=(4, 200, 200)) < 0.1).astype(np.int32)
+    assert np.count_nonzero(wide[0] == 0) > 32767
+    narrow = pkg.geometry.grf_sample(24, 3.0, 0.4, seed=101)
+    kw = dict(capacity_nodes=400_000, check_interval=50, tol=1e-6, block=256, precision=precision)
+    with pkg.PermeabilityServer(**kw) as srv:
+        alone = _run(srv, [(narrow, 1.0 / 6.0, (1e-5, 0.0, 0.0))])[0]
+    with pkg.PermeabilityServer(**kw) as srv:
+        r, n = _run(srv, [(wide, 1.0 / 6.0, (1e-5, 0.0, 0.0)), (narrow, 1.0 / 6.0, (1e-5, 0.0, 0.0))])
+    assert r["converged"] == 1 and n["converged"] == 1
+    np.testing.assert_array_equal(n["k"], alone["k"])
+    sp = pkg.SinglePhase(wide, 1.0 / 6.0, (1e-5, 0.0, 0.0), precision=precision)
+    sp.step(r["steps"])
+    k, _, _ = sp.Comput_Perm(1)
+    sp.close()
+    _log(f"[gpu] wide sample k {r['k'][0]:.10e} after {r['steps']} steps, plain solver {k[0]:.10e}")
+    # two FP64 kernels (arena and plain solver) differ by round-off only
+    assert abs(r["k"][0] - k[0]) <= (1e-11 if precision == "fp64" else 1e-6) * abs(k[0])
 
 
 def test_autotuned_block(pkg):

@@ -244,6 +244,38 @@ def test_python_local_and_group_permeability_match_reference(pkg):
             assert abs(kk * 1000.0 - rk) <= 2e-5 * abs(rk) + 1e-9 and abs(por - rp) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_cube_sums_match_box_diagnostics(pkg, precision):
+    """lbm_sp_cube_sums (all cubes of a grid in one launch) against one box diagnostic per cube:
+    the same flux sums to round-off and the same node counts, for a grid that reaches past the
+    box on every side, after pull steps and before the first step; the grid's node lists are
+    kept, so a second call gives the same sums bit for bit."""
+    _log(f"[gpu] cube sums {precision}")
+    solid = pkg.geometry.sphere_pack((48, 40, 36), 4.0, 0.4, seed=11)
+    s = pkg.SinglePhase(solid, 1.0 / 6.0, (1e-5, 2e-6, -1e-6), precision=precision)
+    for steps in (0, 30):
+        s.step(steps)
+        for origin, size, m in (((-3, 2, -5), 9, 6), ((0, 0, 0), 12, 3), ((5, 7, 1), 1, 4)):
+            got = s.cube_sums(origin, size, m)
+            assert np.array_equal(got, s.cube_sums(origin, size, m))
+            c = 0
+            for ix in range(m):
+                for iy in range(m):
+                    for iz in range(m):
+                        lo = [origin[0] + ix * size, origin[1] + iy * size, origin[2] + iz * size]
+                        box = (lo[0], lo[0] + size - 1, lo[1], lo[1] + size - 1, lo[2], lo[2] + size - 1)
+                        d = s.diagnostics(box)
+                        assert got[c, 3] == d["count_in_box"], (origin, size, m, c)
+                        scale = np.abs(d["Q"]).max() + 1e-300
+                        assert np.abs(got[c, :3] - d["Q"]).max() <= 1e-12 * scale, (origin, size, m, c)
+                        c += 1
+    with pytest.raises(pkg.LBMError):
+        s.cube_sums((0, 0, 0), 0, 2)
+    with pytest.raises(pkg.LBMError):
+        s.cube_sums((0, 0, 0), 4, 257)
+    s.close()
+
+
 @pytest.mark.parametrize("name", ["sp_mrt_16x12x8", "sp_tau1_box_20x10x12"])
 def test_error_over_ranks_matches_reference_printed_values(pkg, oracle_mod, name):
     """lbm_sp_error_ranks with the reference's 2 ranks gives the u_max and velocity change its
