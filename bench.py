@@ -333,6 +333,8 @@ def cg_bench(pkg, steps, warmup, precision, edge=192, cpu_s=15.0, no_cpu=False):
     solid = geo.sphere_pack((edge, edge, edge), 6.0, 0.45, seed=2025)
     psi = np.where(np.random.default_rng(2026).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
     s = pkg.ColorGradient(solid, 1 / 6, 1 / 6, 0.0, 0.01, (1e-5, 0.0, 0.0), precision=precision)
+    pin(s, "cg_collide")
+    pin(s, "cg_colour")
     s.init(psi)
     s.step(warmup)
     s.diagnostics()
@@ -406,6 +408,7 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
     conc[: edge // 8] = 1.0
     flow = pkg.SinglePhase(solid, nu, g, precision=precision)
     st = pkg.Solute(D, flow=flow, w0=w0, precision=precision)
+    pin(st, "st_step")
     st.init(conc)
     N = flow.count
     st.step(warmup)
@@ -439,6 +442,7 @@ def solute_bench(pkg, steps, warmup, precision, edge=384, cpu_s=15.0, no_cpu=Fal
     # the solute kernel alone on the same pack with a frozen velocity field: the reference's
     # own solute program (velocity read from the flow run's files)
     fs = pkg.Solute(D, solid=solid, w0=w0, precision=precision)
+    pin(fs, "st_step")
     rng = np.random.default_rng(2028)
     fs.set_velocity(rng.uniform(-0.01, 0.01, (fs.count, 3)))
     fs.init(conc)
@@ -565,6 +569,21 @@ def accuracy_study(pkg, edge=256, max_steps=100000):
 
 
 SUB_STEPS, SUB_WARMUP = 500, 20     # timed and untimed steps of each sub-record
+LAUNCH = {}                          # --launch settings: kernel -> (block, order, variant)
+
+
+def pin(handle, kernel):
+    """Apply a --launch setting (the profiler runs pin the settings the autotuner chose in a
+    normal run: under --pmc every dispatch is serialised and its timings mean nothing)."""
+    if kernel not in LAUNCH:
+        return
+    block, order, variant = LAUNCH[kernel]
+    if kernel == "sp_step":
+        handle.set_launch(block, order)
+    elif kernel == "st_step":
+        handle.set_launch(block, order)
+    else:
+        handle.set_launch(kernel, block, order, variant)
 
 
 def _progress(msg):
@@ -576,6 +595,7 @@ def sp_record(pkg, solid, precision, steps, warmup, nu, g, faces=None, power_law
     the step kernel's roofline from HIP events on the handle stream around the same launches."""
     edge = solid.shape[0]
     s = pkg.SinglePhase(solid, nu, g, precision=precision, device=0)
+    pin(s, "sp_step")
     if faces:
         s.set_boundaries(faces["sub_bc"], faces["pre"])
         s.init(1.0, (0.0, 0.0, 0.0))
@@ -629,9 +649,16 @@ def main():
     ap.add_argument("--vtk", action="store_true", help="time the VTK field writers of the lbm_sp driver")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the single-phase workload to steady state in FP64/FP32/FP16 and compare")
+    ap.add_argument("--launch", action="append", default=[], metavar="KERNEL=BLOCK:ORDER:VARIANT",
+                    help="fix a kernel's launch settings instead of the autotuner's choice (sp_step, cg_collide, "
+                         "cg_colour, st_step; order hw|runs8|slab; variant gather|staged)")
     ap.add_argument("--no-sub", action="store_true",
                     help="the single-phase FP32 line alone, without the fp64 / fp16 / accuracy / cg / solute records")
     args = ap.parse_args()
+    for item in args.launch:
+        kernel, spec = item.split("=", 1)
+        block, order, variant = (spec.split(":") + ["gather"])[:3]
+        LAUNCH[kernel] = (int(block), order, variant)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only "
                                    "(x slabs with 5-population halos), no figure at several GPUs"}))
