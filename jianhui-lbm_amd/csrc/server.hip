@@ -380,6 +380,130 @@ __global__ void __launch_bounds__(BLK) srv_aa_update_kernel(const TileDev *__res
     }
 }
 
+// ---------------------------------------------------------------------------- sample preparation
+// A submitted sample's pore numbering (i-major, x slowest: build_sparse_geometry's) and its pull
+// deltas, built on the device from its pore flags when it is admitted: a three-pass scan numbers
+// the pores (count per tile of PREP_TILE voxels, scan of the tile counts, numbering inside each
+// tile), then one thread per voxel writes each pore's 18 deltas into the sample's arena rows --
+// the shortest of up - n, up - n - N, up - n + N, as build_deltas does on the host.
+constexpr int PREP_TILE = 2048;   // voxels per scan tile: 256 threads x 8
+constexpr int PREP_PER = PREP_TILE / 256;
+constexpr int32_t PREP_WIDE = 1;  // a delta does not fit 16 bits
+constexpr int32_t PREP_COUNT = 2; // the pore count differs from the host's
+
+// inclusive scan of one int per thread over a 256-thread block; returns the block total
+__device__ __forceinline__ int32_t block_scan_256(int32_t &v)
+{
+    __shared__ int32_t part[256 / WAVE];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int32_t o = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += o;
+    }
+    if (lane == WAVE - 1) part[wid] = v;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 256 / WAVE; ++w) {
+        before += w < wid ? part[w] : 0;
+        total += part[w];
+    }
+    v += before;
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(256) srv_prep_count_kernel(const uint8_t *__restrict__ pore, int64_t V,
+                                                             int32_t *__restrict__ tile_cnt)
+{
+    const int64_t v0 = (int64_t)blockIdx.x * PREP_TILE + (int64_t)threadIdx.x * PREP_PER;
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < PREP_PER; ++k) c += (v0 + k < V) ? (int32_t)pore[v0 + k] : 0;
+    const int32_t total = block_scan_256(c);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile counts in place (one block, any number of tiles), and the total
+__global__ void __launch_bounds__(256) srv_prep_scan_kernel(int32_t *__restrict__ tile, int64_t ntiles,
+                                                            int64_t N_host, int32_t *__restrict__ flag)
+{
+    int32_t carry = 0;
+    for (int64_t t0 = 0; t0 < ntiles; t0 += 256) {
+        const int64_t t = t0 + threadIdx.x;
+        const int32_t c = t < ntiles ? tile[t] : 0;
+        int32_t v = c;
+        const int32_t total = block_scan_256(v);
+        if (t < ntiles) tile[t] = carry + v - c;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && (int64_t)carry != N_host) atomicOr(flag, PREP_COUNT);
+}
+
+__global__ void __launch_bounds__(256) srv_prep_number_kernel(const uint8_t *__restrict__ pore, int64_t V,
+                                                              const int32_t *__restrict__ tile_off,
+                                                              int32_t *__restrict__ idx)
+{
+    const int64_t v0 = (int64_t)blockIdx.x * PREP_TILE + (int64_t)threadIdx.x * PREP_PER;
+    uint8_t p[PREP_PER];
+    int32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < PREP_PER; ++k) {
+        p[k] = (v0 + k < V) ? pore[v0 + k] : 0;
+        c += p[k];
+    }
+    int32_t incl = c;
+    (void)block_scan_256(incl);
+    int32_t n = tile_off[blockIdx.x] + incl - c;
+#pragma unroll
+    for (int k = 0; k < PREP_PER; ++k)
+        if (v0 + k < V) idx[v0 + k] = p[k] ? n++ : -1;
+}
+
+// WIDE = false: 16-bit deltas into nbr (SRV_BB for bounce back), PREP_WIDE raised if one does
+// not fit; WIDE = true: 32-bit deltas, low halves into nbr and high halves into nbr_hi
+template <bool WIDE>
+__global__ void __launch_bounds__(256) srv_prep_links_kernel(const int32_t *__restrict__ idx, int32_t nx, int32_t ny,
+                                                             int32_t nz, int64_t N, int16_t *__restrict__ nbr,
+                                                             int16_t *__restrict__ nbr_hi, int64_t CAP, int64_t base,
+                                                             int32_t *__restrict__ flag)
+{
+    const int64_t nyz = (int64_t)ny * nz, V = (int64_t)nx * nyz;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= V) return;
+    const int32_t n = idx[v];
+    if (n < 0) return;
+    const int32_t i = (int32_t)(v / nyz), j = (int32_t)((v / nz) % ny), k = (int32_t)(v % nz);
+    bool bad = false;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        int iu = i - EX[q], ju = j - EY[q], ku = k - EZ[q];
+        iu = iu < 0 ? iu + nx : (iu >= nx ? iu - nx : iu);
+        ju = ju < 0 ? ju + ny : (ju >= ny ? ju - ny : ju);
+        ku = ku < 0 ? ku + nz : (ku >= nz ? ku - nz : ku);
+        const int32_t up = idx[((int64_t)iu * ny + ju) * nz + ku];
+        int32_t d = SRV_BB32;
+        if (up >= 0) {
+            const int64_t dd = (int64_t)up - n;
+            int64_t best = dd;
+            if (llabs(dd - N) < llabs(best)) best = dd - N;
+            if (llabs(dd + N) < llabs(best)) best = dd + N;
+            d = (int32_t)best;
+        }
+        const int64_t e = (int64_t)(q - 1) * CAP + base + n;
+        if (WIDE) {
+            nbr[e] = (int16_t)(uint16_t)((uint32_t)d & 0xFFFFu);
+            nbr_hi[e] = (int16_t)(uint16_t)((uint32_t)d >> 16);
+        } else {
+            const bool fits = d == SRV_BB32 || (d >= -32767 && d <= 32767);
+            bad |= !fits;
+            nbr[e] = d == SRV_BB32 ? SRV_BB : (fits ? (int16_t)d : (int16_t)0);
+        }
+    }
+    if (!WIDE && bad) atomicOr(flag, PREP_WIDE);
+}
+
 // ---------------------------------------------------------------------------- host
 namespace {
 using Clock = std::chrono::steady_clock;
@@ -390,8 +514,13 @@ struct Prepared {
     int prior_acc = 0;
     std::vector<float> carried;  // 19 x npad FP32 state to continue from (escalation), else start at rest
     int64_t N = 0, npad = 0;
-    // 18 x npad upstream deltas (SRV_BB = bounce back); shared with the preparation worker's
-    // connectivity test, which may still read them after the sample is queued
+    // a submitted sample: its pore flags ([i][j][k], 1 = pore), numbered and linked on the device
+    // at admission; shared with the connectivity test, which may still read them then
+    std::shared_ptr<const std::vector<uint8_t>> pore;
+    int nx = 0, ny = 0, nz = 0;
+    bool wrap_pending = false;   // its connectivity test has not reported yet: it does not retire
+    // a sample handed over from another arena (escalation): its 18 x npad upstream deltas
+    // (SRV_BB = bounce back), read back from that arena
     std::shared_ptr<int16_t[]> nbr;
     // wide sample: the high halves of its 32-bit deltas, 18 x npad (null for a narrow one)
     std::shared_ptr<int16_t[]> nbr_hi;
@@ -459,15 +588,37 @@ struct Segment {
     int64_t start, len;
 };
 
-// a submitted request waiting for its numbering and neighbour deltas (built on a worker)
+// a queued sample's connectivity test (run on a host worker, beside the device work)
 struct PrepJob {
     int64_t ticket = 0;
-    double t_submit = 0;
     int nx = 0, ny = 0, nz = 0;
-    std::vector<uint8_t> pore;   // 1 = pore voxel, [i][j][k]
-    double nu = 0, g[3] = {0, 0, 0};
+    std::shared_ptr<const std::vector<uint8_t>> pore;   // 1 = pore voxel, [i][j][k]
     int dir = 1;
 };
+
+// Whether the pores wrap round the periodic box along axis (connectivity.h), from the pore flags:
+// the numbering in one pass, the links computed as the union-find asks for them.
+bool pore_flags_wrap(int nx, int ny, int nz, const uint8_t *pore, int axis)
+{
+    const int64_t nyz = (int64_t)ny * nz, V = (int64_t)nx * nyz;
+    std::vector<int32_t> idx((size_t)V), sup;
+    int32_t N = 0;
+    for (int64_t v = 0; v < V; ++v) {
+        idx[(size_t)v] = pore[v] ? N++ : -1;
+        if (pore[v]) sup.push_back((int32_t)v);
+    }
+    return lattice_loop_wraps(N, axis, [&](int64_t n, int q) -> int64_t {
+        const int64_t s = sup[(size_t)n];
+        int i = (int)(s / nyz), j = (int)((s / nz) % ny), k = (int)(s % nz);
+        i -= EX[q];
+        j -= EY[q];
+        k -= EZ[q];
+        i = i < 0 ? i + nx : (i >= nx ? i - nx : i);
+        j = j < 0 ? j + ny : (j >= ny ? j - ny : j);
+        k = k < 0 ? k + nz : (k >= nz ? k - nz : k);
+        return idx[(size_t)(((int64_t)i * ny + j) * nz + k)];
+    });
+}
 
 // The sample's pore numbering (i-major, x slowest: the same as build_sparse_geometry) and its
 // pull links as deltas: for each link the shortest of up - n, up - n - N, up - n + N (the device
@@ -563,25 +714,6 @@ bool build_deltas(int nx, int ny, int nz, const uint8_t *pore, int64_t &N, int64
     return true;
 }
 
-// The 32-bit delta of entry i (narrow samples: hi null); bounce back as SRV_BB32.
-inline int32_t host_delta(const int16_t *lo, const int16_t *hi, size_t i)
-{
-    if (!hi) return lo[i] == SRV_BB ? SRV_BB32 : (int32_t)lo[i];
-    return (int32_t)(((uint32_t)(uint16_t)hi[i] << 16) | (uint32_t)(uint16_t)lo[i]);
-}
-
-// Whether the sample's pores wrap round the box along axis, from its deltas (connectivity.h).
-bool deltas_wrap_along(const int16_t *nbr, const int16_t *nbr_hi, int64_t N, int64_t P, int axis)
-{
-    return lattice_loop_wraps(N, axis, [nbr, nbr_hi, N, P](int64_t n, int q) -> int64_t {
-        const int32_t d = host_delta(nbr, nbr_hi, (size_t)((q - 1) * P + n));
-        if (d == SRV_BB32) return -1;
-        int64_t m = n + d;
-        m += m < 0 ? N : 0;
-        m -= m >= N ? N : 0;
-        return m;
-    });
-}
 }  // namespace
 
 class Server : public lbm_handle {
@@ -632,6 +764,12 @@ private:
     int cur_ = 0;
     int16_t *d_nbr_ = nullptr;
     int16_t *d_nbr_hi_ = nullptr;   // high halves of wide samples' deltas
+    // device preparation of submitted samples (loop thread only): pore flags, voxel -> node
+    // numbering and the scan's tile counts, for samples of up to prep_cap_ voxels
+    uint8_t *d_prep_pore_ = nullptr;
+    int32_t *d_prep_idx_ = nullptr, *d_prep_tile_ = nullptr, *d_prep_flag_ = nullptr, *h_prep_flag_ = nullptr;
+    int64_t prep_cap_ = 0;
+    hipError_t prepare_on_device(Active &a);
     // Anderson acceleration: X, R_prev, G_prev, then m dR and m dG vectors, each 19 x CAP
     int aa_m_ = 0, aa_period_ = 0, aa_warm_ = 0;
     void *d_aa_vec_ = nullptr;
@@ -657,8 +795,7 @@ private:
     std::deque<Prepared> pending_;
     std::deque<PrepJob> prep_q_;   // submitted, geometry not built yet
     int prep_busy_ = 0;
-    int nprep_ = 1;          // preparation workers
-    int prep_threads_ = 2;   // host threads the preparation jobs share
+    int nprep_ = 1;          // connectivity-test workers
     std::vector<std::thread> prep_thr_;
     std::condition_variable prep_cv_;
     std::vector<Active> active_;   // in tile order
@@ -735,6 +872,8 @@ bool Server::escalate_locked(Active &a, bool restart)
         a.req.prior_steps += a.steps;
         a.req.prior_acc += a.aa_updates;
     }
+    // the FP64 server is the answer whatever the connectivity test says
+    a.req.wrap_pending = false;
     fallback_->enqueue(std::move(a.req));
     ++escalated_;
     return true;
@@ -838,7 +977,6 @@ int Server::create(const lbm_srv_params &p)
     // its neighbour deltas while the device steps the admitted ones
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     nprep_ = (int)std::min(8u, std::max(1u, hw / 2));
-    prep_threads_ = (int)std::min(16u, hw);
     for (int i = 0; i < nprep_; ++i) prep_thr_.emplace_back([this] { prep_worker(); });
     return LBM_OK;
 }
@@ -863,6 +1001,11 @@ void Server::shutdown()
                      (void **)&d_aa_sums_};
     for (void **b : bufs)
         if (*b) (void)hipFree(*b), *b = nullptr;
+    for (void *b : {(void *)d_prep_pore_, (void *)d_prep_idx_, (void *)d_prep_tile_, (void *)d_prep_flag_})
+        if (b) (void)hipFree(b);
+    d_prep_pore_ = nullptr;
+    d_prep_idx_ = d_prep_tile_ = d_prep_flag_ = nullptr;
+    if (h_prep_flag_) (void)hipHostFree(h_prep_flag_), h_prep_flag_ = nullptr;
     if (h_sums_) (void)hipHostFree(h_sums_), h_sums_ = nullptr;
     if (h_aa_) (void)hipHostFree(h_aa_), h_aa_ = nullptr;
     if (h_aa_sums_) (void)hipHostFree(h_aa_sums_), h_aa_sums_ = nullptr;
@@ -880,40 +1023,63 @@ int Server::submit(const lbm_srv_request &r, int64_t *ticket)
     const int64_t V = (int64_t)r.nx * r.ny * r.nz;
     if (V >= ((int64_t)1 << 31)) return fail(LBM_ERR_ARG, "box has 2^31 or more voxels");
     // the checks that decide admissibility are O(voxels) and done here, on a byte copy of the
-    // pore flags made in the same pass as the pore count; the numbering and the neighbour deltas
-    // are built by a preparation worker
-    PrepJob job;
-    job.pore.resize((size_t)V);
+    // pore flags made in the same pass as the pore count; the sample is queued at once, and the
+    // numbering and neighbour deltas are built on the device when it is admitted
+    auto pore = std::make_shared<std::vector<uint8_t>>((size_t)V);
     int64_t N = 0;
     for (int64_t v = 0; v < V; ++v) {
         const uint8_t p = r.solid_ijk[v] == 0;
-        job.pore[(size_t)v] = p;
+        (*pore)[(size_t)v] = p;
         N += p;
     }
     if (N == 0) return fail(LBM_ERR_ARG, "geometry has no pore voxels");
-    if (padded_stride(N) > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
-    job.nx = r.nx;
-    job.ny = r.ny;
-    job.nz = r.nz;
-    job.nu = r.nu;
-    for (int c = 0; c < 3; ++c) job.g[c] = r.g[c];
-    job.dir = r.perm_dir;
+    const int64_t P = padded_stride(N);
+    if (P > cap_) return fail(LBM_ERR_ARG, "sample larger than the whole arena");
+    Prepared pr;
+    pr.pore = pore;
+    pr.nx = r.nx;
+    pr.ny = r.ny;
+    pr.nz = r.nz;
+    pr.N = N;
+    pr.npad = P;
+    pr.nu = r.nu;
+    for (int c = 0; c < 3; ++c) pr.g[c] = r.g[c];
+    pr.dir = r.perm_dir;
+    pr.volume = V;
+    // the connectivity test (escalation on): a worker runs it beside the device work
+    const bool test_wrap = prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && fallback_ && fallback_->fits(P);
     {
         std::lock_guard<std::mutex> lk(mu_);
-        job.ticket = next_ticket_++;
-        job.t_submit = clock();
-        if (ticket) *ticket = job.ticket;
-        prep_q_.push_back(std::move(job));
+        pr.ticket = next_ticket_++;
+        pr.t_submit = clock();
+        if (ticket) *ticket = pr.ticket;
+        pr.wrap_pending = test_wrap;
+        if (test_wrap) {
+            PrepJob job;
+            job.ticket = pr.ticket;
+            job.nx = r.nx;
+            job.ny = r.ny;
+            job.nz = r.nz;
+            job.pore = pore;
+            job.dir = r.perm_dir;
+            prep_q_.push_back(std::move(job));
+        }
+        pending_.push_back(std::move(pr));
     }
-    prep_cv_.notify_one();
+    cv_.notify_all();
+    if (test_wrap) prep_cv_.notify_one();
     return LBM_OK;
 }
 
+// A sample whose pores do not wrap round the box along the permeability direction has no
+// throughflow: its k is the lattice's spurious-current residue, orders of magnitude below what
+// FP32 deviations resolve.  With escalation on it goes to the FP64 server: at once if it is still
+// queued, at its next check if it has been admitted meanwhile.  The test runs on these workers,
+// beside the device work, so that it does not delay admission.
 void Server::prep_worker()
 {
     for (;;) {
         PrepJob job;
-        int sharing = 1;
         {
             std::unique_lock<std::mutex> lk(mu_);
             prep_cv_.wait(lk, [this] { return stop_ || !prep_q_.empty(); });
@@ -921,62 +1087,30 @@ void Server::prep_worker()
             job = std::move(prep_q_.front());
             prep_q_.pop_front();
             ++prep_busy_;
-            sharing = std::min(nprep_, prep_busy_ + (int)prep_q_.size());
         }
-        // the host threads are shared by the jobs in flight: a lone job (a request arriving on
-        // an idle server) gets up to 8 of them for its parallel loops
-        omp_set_num_threads(std::max(1, std::min(8, prep_threads_ / std::max(1, sharing))));
-        Prepared pr;
-        int64_t N = 0, P = 0;
-        std::shared_ptr<int16_t[]> nbr, nbr_hi;
-        const bool ok = build_deltas(job.nx, job.ny, job.nz, job.pore.data(), N, P, nbr, nbr_hi);
-        std::vector<uint8_t>().swap(job.pore);
-        pr.nbr = nbr;
-        pr.nbr_hi = nbr_hi;
-        pr.wide = nbr_hi != nullptr;
-        pr.ticket = job.ticket;
-        pr.t_submit = job.t_submit;
-        pr.N = N;
-        pr.npad = P;
-        pr.nu = job.nu;
-        for (int c = 0; c < 3; ++c) pr.g[c] = job.g[c];
-        pr.dir = job.dir;
-        pr.volume = (int64_t)job.nx * job.ny * job.nz;
-        const bool test_wrap = ok && prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 && fallback_ && fallback_->fits(P);
+        const bool wraps = pore_flags_wrap(job.nx, job.ny, job.nz, job.pore->data(), job.dir - 1);
+        job.pore.reset();
         {
             std::lock_guard<std::mutex> lk(mu_);
             --prep_busy_;
-            if (ok) {
-                pending_.push_back(std::move(pr));
-            } else {
-                lbm_srv_result r{};
-                r.ticket = job.ticket;
-                r.converged = -2;   // could not be prepared
-                r.precision = prm_.precision;
-                r.t_submit = r.t_admit = job.t_submit;
-                r.t_done = clock();
-                done_.push_back(r);
-                ++n_done_;
-            }
-        }
-        cv_.notify_all();
-        idle_cv_.notify_all();
-        // A sample whose pores do not wrap round the box along the permeability direction has no
-        // throughflow: its k is the lattice's spurious-current residue, orders of magnitude
-        // below what FP32 deviations resolve.  With escalation on it goes to the FP64 server:
-        // at once if it is still queued, at its next check if it has been admitted meanwhile.
-        // The test runs after queueing so that it does not delay admission.
-        if (test_wrap && !deltas_wrap_along(nbr.get(), nbr_hi.get(), N, P, job.dir - 1)) {
-            std::lock_guard<std::mutex> lk(mu_);
-            for (auto it = pending_.begin(); it != pending_.end(); ++it)
-                if (it->ticket == job.ticket) {
-                    fallback_->enqueue(std::move(*it));
-                    pending_.erase(it);
-                    ++escalated_;
-                    break;
-                }
+            for (Prepared &p : pending_)
+                if (p.ticket == job.ticket) p.wrap_pending = false;
             for (Active &a : active_)
-                if (a.req.ticket == job.ticket) a.no_wrap = true;
+                if (a.req.ticket == job.ticket) a.req.wrap_pending = false;
+            if (!wraps) {
+                bool moved = false;
+                for (auto it = pending_.begin(); it != pending_.end(); ++it)
+                    if (it->ticket == job.ticket) {
+                        fallback_->enqueue(std::move(*it));
+                        pending_.erase(it);
+                        ++escalated_;
+                        moved = true;
+                        break;
+                    }
+                if (!moved)
+                    for (Active &a : active_)
+                        if (a.req.ticket == job.ticket) a.no_wrap = true;
+            }
         }
         cv_.notify_all();
         idle_cv_.notify_all();
@@ -1075,14 +1209,20 @@ bool Server::try_admit_locked()
         a.first_step = global_step_;
         a.t_admit = clock();
         a.aa_next = aa_warm_;
-        // sample-local neighbour rows go straight into the arena rows of the segment
-        hipError_t e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.get(),
-                                        (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
-                                        hipMemcpyHostToDevice, stream_);
-        if (e == hipSuccess && a.req.wide)
-            e = hipMemcpy2DAsync(d_nbr_hi_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr_hi.get(),
+        hipError_t e = hipSuccess;
+        if (a.req.pore) {
+            // a submitted sample: numbered and linked on the device, into the segment's rows
+            e = prepare_on_device(a);
+        } else {
+            // a handed-over sample: its neighbour rows go straight into the arena rows
+            e = hipMemcpy2DAsync(d_nbr_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr.get(),
                                  (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
                                  hipMemcpyHostToDevice, stream_);
+            if (e == hipSuccess && a.req.wide)
+                e = hipMemcpy2DAsync(d_nbr_hi_ + base, (size_t)cap_ * sizeof(int16_t), a.req.nbr_hi.get(),
+                                     (size_t)a.req.npad * sizeof(int16_t), (size_t)a.req.npad * sizeof(int16_t), 18,
+                                     hipMemcpyHostToDevice, stream_);
+        }
         if (e == hipSuccess && !a.req.carried.empty()) {
             // an escalated sample resumes from its FP32 state: no equilibrium step
             std::vector<double> st(a.req.carried.begin(), a.req.carried.end());
@@ -1113,6 +1253,7 @@ bool Server::try_admit_locked()
         for (Active &a : active_) {
             a.req.nbr.reset();
             a.req.nbr_hi.reset();
+            a.req.pore.reset();
         }
     }
     return any;
@@ -1120,6 +1261,52 @@ bool Server::try_admit_locked()
 
 // Called on the loop thread (and by autotune at create): errors are returned with their
 // message in msg, not through fail(), so the loop thread never writes last_error.
+// Number and link a submitted sample on the device (the preparation kernels above), into the
+// arena rows of its segment a.base; sets a.req.wide.  Loop thread only.  The pore flags are read
+// from pageable memory: the caller synchronises the stream before they go.
+hipError_t Server::prepare_on_device(Active &a)
+{
+    const int64_t V = (int64_t)a.req.nx * a.req.ny * a.req.nz;
+    hipError_t e = hipSuccess;
+    if (V > prep_cap_) {
+        for (void *b : {(void *)d_prep_pore_, (void *)d_prep_idx_, (void *)d_prep_tile_})
+            if (b) (void)hipFree(b);
+        d_prep_pore_ = nullptr;
+        d_prep_idx_ = d_prep_tile_ = nullptr;
+        prep_cap_ = 0;
+        const int64_t cap = std::max<int64_t>(V, int64_t(1) << 22);
+        e = hipMalloc(&d_prep_pore_, (size_t)cap);
+        if (e == hipSuccess) e = hipMalloc(&d_prep_idx_, (size_t)cap * sizeof(int32_t));
+        if (e == hipSuccess) e = hipMalloc(&d_prep_tile_, (size_t)(cap / PREP_TILE + 1) * sizeof(int32_t));
+        if (e == hipSuccess && !d_prep_flag_) e = hipMalloc(&d_prep_flag_, sizeof(int32_t));
+        if (e == hipSuccess && !h_prep_flag_) e = hipHostMalloc((void **)&h_prep_flag_, sizeof(int32_t), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        prep_cap_ = cap;
+    }
+    const int64_t ntiles = (V + PREP_TILE - 1) / PREP_TILE;
+    e = hipMemsetAsync(d_prep_flag_, 0, sizeof(int32_t), stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_prep_pore_, a.req.pore->data(), (size_t)V, hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess) return e;
+    srv_prep_count_kernel<<<(unsigned)ntiles, 256, 0, stream_>>>(d_prep_pore_, V, d_prep_tile_);
+    srv_prep_scan_kernel<<<1, 256, 0, stream_>>>(d_prep_tile_, ntiles, a.req.N, d_prep_flag_);
+    srv_prep_number_kernel<<<(unsigned)ntiles, 256, 0, stream_>>>(d_prep_pore_, V, d_prep_tile_, d_prep_idx_);
+    const unsigned gl = (unsigned)((V + 255) / 256);
+    srv_prep_links_kernel<false><<<gl, 256, 0, stream_>>>(d_prep_idx_, a.req.nx, a.req.ny, a.req.nz, a.req.N, d_nbr_,
+                                                           d_nbr_hi_, cap_, a.base, d_prep_flag_);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h_prep_flag_, d_prep_flag_, sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return e;
+    if (*h_prep_flag_ & PREP_COUNT) return hipErrorInvalidValue;   // cannot happen: the counts are the host's
+    a.req.wide = (*h_prep_flag_ & PREP_WIDE) != 0;
+    if (a.req.wide) {
+        srv_prep_links_kernel<true><<<gl, 256, 0, stream_>>>(d_prep_idx_, a.req.nx, a.req.ny, a.req.nz, a.req.N, d_nbr_,
+                                                              d_nbr_hi_, cap_, a.base, d_prep_flag_);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
 int Server::rebuild_tables(std::string &msg)
 {
     // order active samples by arena position (keeps neighbouring tiles adjacent in memory)
@@ -1363,6 +1550,8 @@ int Server::run_chunk(std::unique_lock<std::mutex> &lk)
             status[s] = 1;
         else if (a.steps >= prm_.max_steps)
             status[s] = 0;
+        // a sample whose connectivity test is still running keeps stepping (unless it blew up)
+        if (a.req.wrap_pending && status[s] != -1) status[s] = 2;
         // precision escalation: a flux too small for FP32 deviations, or no convergence
         if (prm_.precision == LBM_FP32 && prm_.escalate_k > 0.0 &&
             (status[s] == 0 || (status[s] == 2 && a.no_wrap) ||
