@@ -33,7 +33,10 @@ const char *lbm_version(void) { return "lbm-mi355x 0.1 (gfx950, sparse D3Q19 MRT
 const char *lbm_last_error(const lbm_handle *h)
 {
     if (!h) return g_create_error.c_str();
-    return h->last_error.c_str();
+    // a copy per calling thread: the handle's message may be replaced by another thread's call
+    thread_local std::string msg;
+    msg = h->error();
+    return msg.c_str();
 }
 
 void lbm_destroy(lbm_handle *h) { delete h; }
@@ -69,7 +72,7 @@ int lbm_sp_create(const lbm_sp_params *p, const int32_t *solid_ijk, lbm_handle *
     }
     int rc = s->create(*p, solid_ijk);
     if (rc != LBM_OK) {
-        g_create_error = s->last_error;
+        g_create_error = s->error();
         delete s;
         return rc;
     }
@@ -202,7 +205,7 @@ int lbm_cg_create(const lbm_cg_params *p, const int32_t *solid_ijk, lbm_handle *
     }
     int rc = s->create(*p, solid_ijk);
     if (rc != LBM_OK) {
-        g_create_error = s->last_error;
+        g_create_error = s->error();
         delete s;
         return rc;
     }
@@ -284,7 +287,7 @@ int lbm_st_create(const lbm_st_params *p, const int32_t *solid_ijk, lbm_handle *
     }
     int rc = s->create(*p, solid_ijk);
     if (rc != LBM_OK) {
-        g_create_error = s->last_error;
+        g_create_error = s->error();
         delete s;
         return rc;
     }
@@ -307,7 +310,7 @@ int lbm_st_create_coupled(const lbm_st_params *p, lbm_handle *flow, lbm_handle *
     }
     int rc = s->create_coupled(*p, f);
     if (rc != LBM_OK) {
-        g_create_error = s->last_error;
+        g_create_error = s->error();
         delete s;
         return rc;
     }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,12 +14,23 @@ struct lbm_handle {
     explicit lbm_handle(Kind k) : kind(k) {}
     virtual ~lbm_handle() = default;
     Kind kind;
-    std::string last_error;
+    // the message of the last failed call; a server handle's calls may come from several threads,
+    // so it is written and read under err_mu
     int fail(int code, const std::string &msg)
     {
+        std::lock_guard<std::mutex> lk(err_mu);
         last_error = msg;
         return code;
     }
+    std::string error() const
+    {
+        std::lock_guard<std::mutex> lk(err_mu);
+        return last_error;
+    }
+
+private:
+    mutable std::mutex err_mu;
+    std::string last_error;
 };
 
 namespace lbm {

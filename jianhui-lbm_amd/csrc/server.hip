@@ -1130,7 +1130,7 @@ int Server::poll(lbm_srv_result *out, int max, int *n_out)
         int m = 0;
         rc = fallback_->poll(out + n, max - n, &m);
         n += m;
-        if (rc != LBM_OK) fail(rc, fallback_->last_error);
+        if (rc != LBM_OK) fail(rc, fallback_->error());
     }
     if (n_out) *n_out = n;
     if (loop_error_) return fail(LBM_ERR_HIP, loop_msg_);
@@ -1158,7 +1158,7 @@ int Server::wait_idle(double timeout_s)
             if (rest <= 0) return LBM_ERR_BUSY;
         }
         const int rc = fallback_->wait_idle(rest);
-        if (rc != LBM_OK) return rc == LBM_ERR_BUSY ? rc : fail(rc, fallback_->last_error);
+        if (rc != LBM_OK) return rc == LBM_ERR_BUSY ? rc : fail(rc, fallback_->error());
     }
     return LBM_OK;
 }
@@ -1740,7 +1740,7 @@ int lbm_srv_create(const lbm_srv_params *p, lbm_handle **out)
     if (!s) return LBM_ERR_NOMEM;
     const int rc = s->create(*p);
     if (rc != LBM_OK) {
-        lbm::set_create_error(s->last_error);
+        lbm::set_create_error(s->error());
         delete s;
         return rc;
     }

@@ -425,7 +425,7 @@ int StSolver::init(const double *conc_ijk)
         if (!flow_->initialised()) return fail(LBM_ERR_STATE, "flow handle not initialised");
         std::vector<double> u((size_t)N_ * 3);
         const int rc = flow_->get_state(nullptr, u.data(), nullptr);
-        if (rc != LBM_OK) return fail(rc, "flow velocity: " + flow_->last_error);
+        if (rc != LBM_OK) return fail(rc, "flow velocity: " + flow_->error());
         const int rc2 = upload_velocity(u.data());
         if (rc2 != LBM_OK) return rc2;
     }
@@ -484,14 +484,14 @@ int StSolver::launch_step(bool last)
         FusedSolute fs{pops_[cur_], pops_[nxt], 1.0 / (prm_.D / cs2 + 0.5), prm_.w0, (1.0 - prm_.w0) / 6.0, 1.0 / cs2,
                        launch_};
         const int rc = flow_->step_fused(fs, last);
-        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
+        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->error());
         cur_ = nxt;
         ++steps_;
         return LBM_OK;
     }
     if (flow_) {
         const int rc = flow_->step_storing_velocity();
-        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->last_error);
+        if (rc != LBM_OK) return fail(rc, "coupled flow step failed: " + flow_->error());
     }
     const void *u = velocity();
     hipError_t e;
