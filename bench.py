@@ -263,8 +263,9 @@ def serving_bench(pkg, nreq, rate, seed=7, max_steps=60000, quick=False):
            "accuracy_target": "k within 1e-4 of the FP64 reference (tests/golden/serving_stream_k.json)"}
     out.update(served)
     out["server"] = ("FP32 arena, Anderson acceleration m=8 every 100 steps after 200, retire at 2 x |dk/k| <= 1e-6, "
-                     "samples whose pores do not wrap round the box along x sent to FP64 (from rest) when their connectivity test reports, samples with "
-                     "k < 1e-2 continued in FP64")
+                     "samples whose pores do not wrap round the box along x sent to FP64 (from rest) when their "
+                     "connectivity test reports, samples with k < 1e-2 continued in FP64; samples numbered and "
+                     "linked on the device at admission")
     # the same accuracy: every k within 1e-4 of the FP64 reference, which only the oracle-stopped
     # sequential solves reach (the 1e-7 and 1e-6 relative-change rules miss it)
     out["speedup_vs_sequential_same_accuracy"] = round(span_orc / span_srv, 2) if span_orc else None
