@@ -334,7 +334,7 @@ private:
     void *d_cs_ = nullptr;    // 8 x P (AoS per node): rho_r, rho_b, u, n of the last collision
     void *d_cwin_ = nullptr;  // FP32: per block, the runs of upstream colour states staged in LDS
     int cwin_slots_ = 0;      // LDS states per block of the staged colour kernel (0: gathers)
-    double cwin_cover_ = 0;   // fraction of the colour reads the staged runs serve
+    double cwin_fast_ = 0;    // fraction of the staged blocks that read without a search
     lbm_launch launch_collide_{BLOCK_DEFAULT, 1, 0, 0, 0.0};   // runs of 8 tiles
     lbm_launch launch_colour_{BLOCK_DEFAULT, 2, 0, 0, 0.0};    // gathers, one slab per XCD
     void *d_u_ = nullptr;     // 3 x P: u of the previous step
