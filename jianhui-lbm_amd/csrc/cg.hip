@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "collide.h"
@@ -70,6 +71,16 @@ struct ColourWin {
     int32_t d[Q - 1];
 };
 static_assert(sizeof(ColourWin) == 128, "one 128-byte record per block");
+
+// node id held by LDS slot s (the runs are packed from slot 0), or -1 past the last run
+__device__ __forceinline__ int32_t cw_slot_node(const ColourWin &w, int32_t s)
+{
+    int32_t id = -1;
+#pragma unroll
+    for (int k = 0; k < CW_RUNS; ++k)
+        if ((uint32_t)(s - w.off[k]) < (uint32_t)w.len[k]) id = w.lo[k] + (s - w.off[k]);
+    return id;
+}
 
 template <typename ST>
 struct CgArgs {
@@ -262,9 +273,18 @@ __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
     ColourWin w{};
     if (SLOTS > 0) w = a.wins[b];
     if (SLOTS > 0 && w.fast == CW_ALL) {   // the same for the whole block
+        // slots threadIdx.x + i NB: all loads in flight before the first LDS write (a slot past
+        // the runs, which nothing reads, gets run 0's first node)
+        constexpr int NS = SLOTS > 0 ? SLOTS / NB : 1;
+        static_assert(SLOTS % NB == 0, "whole slots per lane");
+        C v[NS];
 #pragma unroll
-        for (int k = 0; k < CW_RUNS; ++k)
-            for (int i = threadIdx.x; i < w.len[k]; i += NB) lpsi[w.off[k] + i] = a.psi[(int64_t)w.lo[k] + i];
+        for (int i = 0; i < NS; ++i) {
+            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * NB);
+            v[i] = a.psi[id >= 0 ? id : w.lo[0]];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) lpsi[threadIdx.x + i * NB] = v[i];
         __syncthreads();
     }
     if (n >= a.N) return;
@@ -274,27 +294,53 @@ __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
         cg_collide_node<ST, MODE, STORE_U, SLOTS, false>(a, n, lpsi, w);
 }
 
-// g_r, g_b leaving a node with colour state v in direction sgn*e_q (sgn = -1: the opposite
+// e_q . (x, y, z) with the zero components left out (0 * v is not folded away in IEEE arithmetic)
+// and the others added in the order x, y, z: the same value as the full sum
+template <int q, typename T>
+__device__ __forceinline__ T e_dot(T x, T y, T z)
+{
+    constexpr bool hx = EX[q] != 0, hy = EY[q] != 0, hz = EZ[q] != 0;
+    T s = T(0);
+    if constexpr (hx) s = EX[q] > 0 ? x : -x;
+    if constexpr (hy) {
+        const T t = EY[q] > 0 ? y : -y;
+        s = hx ? s + t : t;
+    }
+    if constexpr (hz) {
+        const T t = EZ[q] > 0 ? z : -z;
+        s = (hx || hy) ? s + t : t;
+    }
+    return s;
+}
+
+// f(std::integral_constant<int, I>) for I = B .. E - 1, unrolled at compile time
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// g_r, g_b leaving a node with colour state v in direction e_q, or -e_q when bb (the opposite
 // direction, used for the bounce-back term of the pull): the equilibrium split
 // g_c = w rho_c phi_q, phi_q = 1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u (CG:3477-3485), then the
 // recolouring g_r += d, g_b -= d, d = min(g_r,q, g_r,opp, g_b,q, g_b,opp) e.C/|C| (CG:3493-3508)
-template <typename C>
-__device__ __forceinline__ void colour_pull_term(const int q, const C (&v)[8], C sgn, C &gr, C &gb)
+template <int q, typename C>
+__device__ __forceinline__ void colour_pull_term(const C (&v)[8], bool bb, C &gr, C &gb)
 {
     const C rr = v[0], rb = v[1], ux = v[2], uy = v[3], uz = v[4];
     const C uu = ux * ux + uy * uy + uz * uz;
-    const C eu = sgn * (C(EX[q]) * ux + C(EY[q]) * uy + C(EZ[q]) * uz);
+    const C du = e_dot<q>(ux, uy, uz), eu = bb ? -du : du;
     const C base = C(1) - C(1.5) * uu + C(4.5) * eu * eu;
     const C phq = base + C(3) * eu, php = base - C(3) * eu;
     const C w = C(weight(q));
-    gr = w * rr * phq;
-    gb = w * rb * phq;
-    C mn = gr;
-    const C t1 = w * rr * php, t2 = gb, t3 = w * rb * php;
-    mn = t1 < mn ? t1 : mn;
-    mn = t2 < mn ? t2 : mn;
-    mn = t3 < mn ? t3 : mn;
-    const C en = sgn * (C(EX[q]) * v[5] + C(EY[q]) * v[6] + C(EZ[q]) * v[7]);
+    const C wr = w * rr, wb = w * rb;
+    gr = wr * phq;
+    gb = wb * phq;
+    const C mn = fmin(fmin(fmin(gr, wr * php), gb), wb * php);
+    const C dn = e_dot<q>(v[5], v[6], v[7]), en = bb ? -dn : dn;
     gr += mn * en;
     gb -= mn * en;
 }
@@ -329,25 +375,25 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *_
     uint32_t src[Q - 1];
     load_nbr(nbr, P, n, src);
     constexpr int G = sizeof(C) == 4 ? 18 : 3;   // measured best per type (FP64: 351 us at 3, 402 us at 6)
-#pragma unroll
-    for (int q0 = 1; q0 < Q; q0 += G) {
+    static_for<0, (Q - 1) / G>([&](auto g) {
+        constexpr int q0 = 1 + decltype(g)::value * G;
         C up_s[G][8];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             const uint32_t v = src[q0 + j - 1];
             load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, up_s[j]);
         }
-#pragma unroll
-        for (int j = 0; j < G; ++j) {
+        static_for<0, G>([&](auto jj) {
+            constexpr int j = decltype(jj)::value;
             double up[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) up[k] = (double)up_s[j][k];
             double gr, gb;
-            colour_pull_term<double>(q0 + j, up, (src[q0 + j - 1] & BB_FLAG) ? -1.0 : 1.0, gr, gb);
+            colour_pull_term<q0 + j>(up, (src[q0 + j - 1] & BB_FLAG) != 0, gr, gb);
             rr += gr;
             rb += gb;
-        }
-    }
+        });
+    });
     col[n] = (C)rr;
     col[P + n] = (C)rb;
     psi[n] = (C)((rr - rb) / (rr + rb));
@@ -373,24 +419,6 @@ __device__ __forceinline__ int32_t colour_share(bool axis, int32_t R)
     return axis ? h >> 2 : h >> 3;
 }
 
-// e_q . (x, y, z) with the zero components left out (0 * v is not folded away in IEEE arithmetic)
-// and the others added in the order x, y, z: the same value as the full sum
-template <int q>
-__device__ __forceinline__ float e_dot(float x, float y, float z)
-{
-    constexpr bool hx = EX[q] != 0, hy = EY[q] != 0, hz = EZ[q] != 0;
-    float s = 0.0f;
-    if constexpr (hx) s = EX[q] > 0 ? x : -x;
-    if constexpr (hy) {
-        const float t = EY[q] > 0 ? y : -y;
-        s = hx ? s + t : t;
-    }
-    if constexpr (hz) {
-        const float t = EZ[q] > 0 ? z : -z;
-        s = (hx || hy) ? s + t : t;
-    }
-    return s;
-}
 
 // The colour sums of one node, direction by direction: begin() with the node's own colour state
 // (the rest population), add() the share arriving along q from an upstream state (bb: the
@@ -534,27 +562,41 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const 
                                                               int32_t *__restrict__ col, float *__restrict__ psi,
                                                               int64_t N, int64_t P, int32_t order)
 {
-    __shared__ float4 lds[2 * SLOTS];   // two rows of SLOTS records, as in global memory
+    // a plain vector type, which the compiler keeps in registers (HIP's float4 is a struct whose
+    // arrays it left in scratch memory here)
+    using f4 = float __attribute__((ext_vector_type(4)));
+    __shared__ f4 lds[2 * SLOTS];   // two rows of SLOTS records, as in global memory
     const int64_t b = tile_of(order, blockIdx.x, gridDim.x);
     const ColourWin w = wins[b];
-    const float4 *__restrict__ cs4 = reinterpret_cast<const float4 *>(cs);
+    const f4 *__restrict__ cs4 = reinterpret_cast<const f4 *>(cs);
+    const int64_t n = b * BLOCK + threadIdx.x;
+    // the neighbour table first (n < P: the padding rows are readable), in flight during the
+    // staging
+    uint32_t src[Q - 1];
+    load_nbr(nbr, P, n, src);
     if (w.fast == CW_ALL) {   // the same for the whole block
+        // slots threadIdx.x + i BLOCK: all loads in flight before the first LDS write (a slot
+        // past the runs, which nothing reads, gets run 0's first node)
+        constexpr int NS = SLOTS / BLOCK;
+        static_assert(SLOTS % BLOCK == 0, "whole slots per lane");
+        f4 ra[NS], rc[NS];
 #pragma unroll
-        for (int k = 0; k < CW_RUNS; ++k) {
-            const float4 *g = cs4 + (int64_t)w.lo[k];
-            for (int i = threadIdx.x; i < w.len[k]; i += BLOCK) {
-                lds[w.off[k] + i] = g[i];
-                lds[SLOTS + w.off[k] + i] = g[P + i];
-            }
+        for (int i = 0; i < NS; ++i) {
+            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * BLOCK);
+            const int64_t g = id >= 0 ? id : w.lo[0];
+            ra[i] = cs4[g];
+            rc[i] = cs4[P + g];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            lds[threadIdx.x + i * BLOCK] = ra[i];
+            lds[SLOTS + threadIdx.x + i * BLOCK] = rc[i];
         }
         __syncthreads();
     }
-    const int64_t n = b * BLOCK + threadIdx.x;
     if (n >= N) return;
-    uint32_t src[Q - 1];
-    load_nbr(nbr, P, n, src);
     auto lds_state = [&](int32_t slot, float (&v)[8]) {
-        const float4 a = lds[slot], c = lds[SLOTS + slot];
+        const f4 a = lds[slot], c = lds[SLOTS + slot];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
         v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
     };
