@@ -61,6 +61,7 @@ struct ColourStore<float> {
 // run, at slot id + d[j].  Such a block has every bit of `fast` set; a solid upstream site reads
 // the node's own slot.
 constexpr int CW_RUNS = 4;
+constexpr int CW_BLOCK = 512;   // nodes per block of the staged kernels
 constexpr uint32_t CW_ALL = (1u << (Q - 1)) - 1;
 struct ColourWin {
     int32_t lo[CW_RUNS];
@@ -557,7 +558,7 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_fx(NbrDev nbr, const f
 // with a read outside its runs (across the periodic wrap of a plane) gathers all its states
 // from global memory instead, as cg_colour_kernel_fx does.
 template <int SLOTS>
-__global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const float *__restrict__ cs,
+__global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win(NbrDev nbr, const float *__restrict__ cs,
                                                               const ColourWin *__restrict__ wins,
                                                               int32_t *__restrict__ col, float *__restrict__ psi,
                                                               int64_t N, int64_t P, int32_t order)
@@ -569,28 +570,28 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_win(NbrDev nbr, const 
     const int64_t b = tile_of(order, blockIdx.x, gridDim.x);
     const ColourWin w = wins[b];
     const f4 *__restrict__ cs4 = reinterpret_cast<const f4 *>(cs);
-    const int64_t n = b * BLOCK + threadIdx.x;
-    // the neighbour table first (n < P: the padding rows are readable), in flight during the
-    // staging
+    const int64_t n = b * CW_BLOCK + threadIdx.x;
+    // the neighbour table first, in flight during the staging (the table has P >= N rows, P a
+    // multiple of 256, so a 512-node block can reach past its end: lanes past N load nothing)
     uint32_t src[Q - 1];
-    load_nbr(nbr, P, n, src);
+    if (n < N) load_nbr(nbr, P, n, src);
     if (w.fast == CW_ALL) {   // the same for the whole block
-        // slots threadIdx.x + i BLOCK: all loads in flight before the first LDS write (a slot
+        // slots threadIdx.x + i CW_BLOCK: all loads in flight before the first LDS write (a slot
         // past the runs, which nothing reads, gets run 0's first node)
-        constexpr int NS = SLOTS / BLOCK;
-        static_assert(SLOTS % BLOCK == 0, "whole slots per lane");
+        constexpr int NS = SLOTS / CW_BLOCK;
+        static_assert(SLOTS % CW_BLOCK == 0, "whole slots per lane");
         f4 ra[NS], rc[NS];
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * BLOCK);
+            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * CW_BLOCK);
             const int64_t g = id >= 0 ? id : w.lo[0];
             ra[i] = cs4[g];
             rc[i] = cs4[P + g];
         }
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            lds[threadIdx.x + i * BLOCK] = ra[i];
-            lds[SLOTS + threadIdx.x + i * BLOCK] = rc[i];
+            lds[threadIdx.x + i * CW_BLOCK] = ra[i];
+            lds[SLOTS + threadIdx.x + i * CW_BLOCK] = rc[i];
         }
         __syncthreads();
     }
@@ -840,9 +841,9 @@ static double build_colour_windows(const SparseGeometry &g, int64_t grid, int sl
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : nfast)
     for (int64_t b = 0; b < grid; ++b) {
         std::vector<uint32_t> ids;
-        ids.reserve((size_t)BLOCK * Q);
-        const int64_t n1 = std::min<int64_t>(g.N, (b + 1) * BLOCK);
-        for (int64_t n = b * BLOCK; n < n1; ++n) {
+        ids.reserve((size_t)CW_BLOCK * Q);
+        const int64_t n1 = std::min<int64_t>(g.N, (b + 1) * CW_BLOCK);
+        for (int64_t n = b * CW_BLOCK; n < n1; ++n) {
             ids.push_back((uint32_t)n);
             for (int q = 1; q < Q; ++q) {
                 const uint32_t v = g.nbr[(size_t)((q - 1) * g.P + n)];
@@ -881,15 +882,15 @@ static double build_colour_windows(const SparseGeometry &g, int64_t grid, int sl
                 if (id - (uint32_t)w.lo[r] < (uint32_t)w.len[r]) return r;
             return -1;
         };
-        int own = run_of((uint32_t)(b * BLOCK));
-        for (int64_t n = b * BLOCK; n < n1 && own >= 0; ++n)
+        int own = run_of((uint32_t)(b * CW_BLOCK));
+        for (int64_t n = b * CW_BLOCK; n < n1 && own >= 0; ++n)
             if (run_of((uint32_t)n) != own) own = -1;
         if (own >= 0) {
             w.own = w.off[own] - w.lo[own];
             for (int j = 0; j < Q - 1; ++j) {
                 int kj = -1;
                 bool one = true;
-                for (int64_t n = b * BLOCK; n < n1 && one; ++n) {
+                for (int64_t n = b * CW_BLOCK; n < n1 && one; ++n) {
                     const uint32_t v = g.nbr[(size_t)(j * g.P + n)];
                     if (v & BB_FLAG) continue;
                     const int r = run_of(v);
@@ -954,6 +955,7 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
     LBM_HIP_TRY(hipEventCreate(&ev1_));
     const int64_t P = geo_.P;
     grid_ = (geo_.N + BLOCK - 1) / BLOCK;
+    cw_grid_ = (geo_.N + CW_BLOCK - 1) / CW_BLOCK;
     for (void *&b : pops_) LBM_HIP_TRY(hipMalloc(&b, (size_t)Q * P * st));
     LBM_HIP_TRY(hipMalloc(&d_supinv_, (size_t)P * sizeof(int32_t)));
     LBM_HIP_TRY(hipMalloc(&d_col_, (size_t)2 * P * ct_bytes_));
@@ -979,13 +981,13 @@ int CgSolver::create(const lbm_cg_params &p, const int32_t *solid)
         std::vector<ColourWin> wins;
         int best = 0, last = 0;
         double best_fast = -1.0;
-        for (int slots : {1024, 1536, 2048}) {
-            const double fast = build_colour_windows(geo_, grid_, slots, wins);
+        for (int slots : {1024, 1536, 2560}) {
+            const double fast = build_colour_windows(geo_, cw_grid_, slots, wins);
             last = slots;
             if (fast > best_fast) best = slots, best_fast = fast;
             if (fast >= 0.85) break;
         }
-        if (last != best) build_colour_windows(geo_, grid_, best, wins);
+        if (last != best) build_colour_windows(geo_, cw_grid_, best, wins);
         cwin_slots_ = best;
         cwin_fast_ = best_fast;
         LBM_HIP_TRY(hipMalloc(&d_cwin_, wins.size() * sizeof(ColourWin)));
@@ -1085,26 +1087,26 @@ static void cg_launch_pull_nb(const CgArgs<ST> &a, bool store_u, hipStream_t s)
 }
 
 template <typename ST, int SLOTS>
-static void cg_launch_pull_staged(const CgArgs<ST> &a, bool store_u, int64_t grid, hipStream_t s)
+static void cg_launch_pull_staged(const CgArgs<ST> &a, bool store_u, int64_t cw_grid, hipStream_t s)
 {
-    const dim3 g((unsigned)grid), b(BLOCK);
-    store_u ? cg_collide_kernel<ST, K_PULL, true, BLOCK, SLOTS><<<g, b, 0, s>>>(a)
-            : cg_collide_kernel<ST, K_PULL, false, BLOCK, SLOTS><<<g, b, 0, s>>>(a);
+    const dim3 g((unsigned)cw_grid), b(CW_BLOCK);
+    store_u ? cg_collide_kernel<ST, K_PULL, true, CW_BLOCK, SLOTS><<<g, b, 0, s>>>(a)
+            : cg_collide_kernel<ST, K_PULL, false, CW_BLOCK, SLOTS><<<g, b, 0, s>>>(a);
 }
 
 template <typename ST>
-static hipError_t cg_launch_collide(CgArgs<ST> a, int mode, bool store_u, int64_t grid, const lbm_launch &ls,
-                                    const void *wins, int slots, hipStream_t s)
+static hipError_t cg_launch_collide(CgArgs<ST> a, int mode, bool store_u, int64_t grid, int64_t cw_grid,
+                                    const lbm_launch &ls, const void *wins, int slots, hipStream_t s)
 {
     dim3 g((unsigned)grid), b(BLOCK);
     a.order = ls.order;
     a.wins = (const ColourWin *)wins;
     if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 1024)
-        cg_launch_pull_staged<ST, 1024>(a, store_u, grid, s);
+        cg_launch_pull_staged<ST, 1024>(a, store_u, cw_grid, s);
     else if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 1536)
-        cg_launch_pull_staged<ST, 1536>(a, store_u, grid, s);
-    else if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 2048)
-        cg_launch_pull_staged<ST, 2048>(a, store_u, grid, s);
+        cg_launch_pull_staged<ST, 1536>(a, store_u, cw_grid, s);
+    else if (mode == K_PULL && ls.variant == LBM_VARIANT_STAGED && slots == 2560)
+        cg_launch_pull_staged<ST, 2560>(a, store_u, cw_grid, s);
     else if (mode == K_INIT)
         store_u ? cg_collide_kernel<ST, K_INIT, true><<<g, b, 0, s>>>(a) : cg_collide_kernel<ST, K_INIT, false><<<g, b, 0, s>>>(a);
     else if (mode == K_DIRECT)
@@ -1125,10 +1127,10 @@ hipError_t CgSolver::launch_collide(int mode, bool store_u, const lbm_launch &ls
     if (prm_.precision == LBM_FP64)
         return cg_launch_collide(make_cg_args<double>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                                       d_rho_, N, P),
-                                 mode, store_u, grid_, ls, d_cwin_, cwin_slots_, stream_);
+                                 mode, store_u, grid_, cw_grid_, ls, d_cwin_, cwin_slots_, stream_);
     return cg_launch_collide(make_cg_args<float>(prm_, visc_, u_i_, src, dst, nbr_.view(), d_col_, d_psi_, d_cs_, d_u_,
                                                  d_rho_, N, P),
-                             mode, store_u, grid_, ls, d_cwin_, cwin_slots_, stream_);
+                             mode, store_u, grid_, cw_grid_, ls, d_cwin_, cwin_slots_, stream_);
 }
 
 // the colour-streaming launch: gathers, or (FP32) the LDS-staged variant
@@ -1145,12 +1147,13 @@ hipError_t CgSolver::launch_colour(const lbm_launch &ls)
     const float *cs = (const float *)d_cs_;
     int32_t *col = (int32_t *)d_col_;
     float *psi = (float *)d_psi_;
+    const dim3 gw((unsigned)cw_grid_), bw(CW_BLOCK);
     if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1024)
-        cg_colour_kernel_win<1024><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+        cg_colour_kernel_win<1024><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
     else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1536)
-        cg_colour_kernel_win<1536><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
-    else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 2048)
-        cg_colour_kernel_win<2048><<<g, b, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+        cg_colour_kernel_win<1536><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+    else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 2560)
+        cg_colour_kernel_win<2560><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
     else
         cg_colour_kernel_fx<<<g, b, 0, stream_>>>(nbr_.view(), cs, col, psi, N, P, ls.order);
     return hipGetLastError();
@@ -1209,7 +1212,7 @@ int CgSolver::autotune()
     lbm_launch best;
     const std::vector<int> staged{LBM_VARIANT_GATHER, LBM_VARIANT_STAGED};
     auto cc = tune_candidates(launch_collide_, {128, 256, 512}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {0});
-    for (const lbm_launch &l : tune_candidates(launch_collide_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
+    for (const lbm_launch &l : tune_candidates(launch_collide_, {CW_BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
         if (l.variant == LBM_VARIANT_STAGED) cc.push_back(l);
     LBM_HIP_TRY(tune_launch(stream_, cc,
                             [&](const lbm_launch &ls) {
@@ -1218,9 +1221,10 @@ int CgSolver::autotune()
                             },
                             best));
     launch_collide_ = best;
-    std::vector<int> variants{LBM_VARIANT_GATHER};
-    if (prm_.precision == LBM_FP32) variants.push_back(LBM_VARIANT_STAGED);
-    const auto cv = tune_candidates(launch_colour_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, variants);
+    auto cv = tune_candidates(launch_colour_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {LBM_VARIANT_GATHER});
+    if (prm_.precision == LBM_FP32)
+        for (const lbm_launch &l : tune_candidates(launch_colour_, {CW_BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
+            if (l.variant == LBM_VARIANT_STAGED) cv.push_back(l);
     LBM_HIP_TRY(tune_launch(stream_, cv,
                             [&](const lbm_launch &ls) {
                                 hipError_t e = launch_collide(K_PULL, false, launch_collide_, a, b);
@@ -1249,13 +1253,12 @@ int CgSolver::set_launch(int kernel, const lbm_launch *in)
     l.tuned = 0;
     l.time_us = 0.0;
     if (kernel == LBM_KERNEL_CG_COLLIDE) {
-        if (!(launch_valid(l, {128, 256, 512}, {0}) || launch_valid(l, {BLOCK}, {1})))
-            return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0; or block 256, variant 1");
+        if (!(launch_valid(l, {128, 256, 512}, {0}) || launch_valid(l, {CW_BLOCK}, {1})))
+            return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0; or block 512, variant 1");
         launch_collide_ = l;
     } else if (kernel == LBM_KERNEL_CG_COLOUR) {
-        const std::vector<int> v = prm_.precision == LBM_FP32 ? std::vector<int>{0, 1} : std::vector<int>{0};
-        if (!launch_valid(l, {BLOCK}, v))
-            return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0 (or 1 in FP32)");
+        if (!(launch_valid(l, {BLOCK}, {0}) || (prm_.precision == LBM_FP32 && launch_valid(l, {CW_BLOCK}, {1}))))
+            return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0; or (FP32) block 512, variant 1");
         launch_colour_ = l;
     } else {
         return fail(LBM_ERR_ARG, "a colour-gradient handle launches LBM_KERNEL_CG_COLLIDE and LBM_KERNEL_CG_COLOUR");

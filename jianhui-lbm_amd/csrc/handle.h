@@ -334,6 +334,7 @@ private:
     void *d_cs_ = nullptr;    // 8 x P (AoS per node): rho_r, rho_b, u, n of the last collision
     void *d_cwin_ = nullptr;  // FP32: per block, the runs of upstream colour states staged in LDS
     int cwin_slots_ = 0;      // LDS states per block of the staged colour kernel (0: gathers)
+    int64_t cw_grid_ = 0;     // blocks of the staged kernels (512 nodes each)
     double cwin_fast_ = 0;    // fraction of the staged blocks that read without a search
     lbm_launch launch_collide_{BLOCK_DEFAULT, 1, 0, 0, 0.0};   // runs of 8 tiles
     lbm_launch launch_colour_{BLOCK_DEFAULT, 2, 0, 0, 0.0};    // gathers, one slab per XCD

@@ -57,8 +57,9 @@ def test_colour_gradient_settings_give_identical_states(pkg, precision):
     cases += [(256, o, cv, v) for cv, v, o in itertools.product(("gather", "staged"), variants, ORDERS)]
     for block, order, collide_variant, variant in cases:
         s = pkg.ColorGradient(solid, 0.1, 0.05, 0.3, 0.01, (1e-5, 0, 0), precision=precision)
-        s.set_launch("cg_collide", block, order, collide_variant)
-        s.set_launch("cg_colour", 256, order, variant)
+        # the staged kernels run 512-node blocks
+        s.set_launch("cg_collide", 512 if collide_variant == "staged" else block, order, collide_variant)
+        s.set_launch("cg_colour", 512 if variant == "staged" else 256, order, variant)
         s.init(psi)
         s.step(20)
         st = s.state()
@@ -112,11 +113,11 @@ def test_invalid_settings_are_refused(pkg):
     s.close()
     c = pkg.ColorGradient(solid, 0.1, 0.1, 0.0, 0.01, (1e-5, 0, 0), precision="fp64")
     with pytest.raises(_capi.LBMError):
-        c.set_launch("cg_colour", 256, "slab", "staged")   # the staged variant is FP32 only
+        c.set_launch("cg_colour", 512, "slab", "staged")   # the staged variant is FP32 only
     with pytest.raises(_capi.LBMError):
         c.set_launch("cg_colour", 512, "slab", "gather")
     with pytest.raises(_capi.LBMError):
-        c.set_launch("cg_collide", 512, "slab", "staged")   # staging is per 256-node block
+        c.set_launch("cg_collide", 256, "slab", "staged")   # staging is per 512-node block
     c.close()
 
 
