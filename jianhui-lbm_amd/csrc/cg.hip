@@ -604,12 +604,16 @@ __global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win(NbrDev nbr, con
     ColourSums acc;
     if (w.fast == CW_ALL) {
         const int32_t own = (int32_t)n + w.own;
-        auto fetch = [&](int j, uint32_t v, float (&out)[8]) {
-            const int32_t up = (int32_t)v + w.d[j];
-            lds_state((v & BB_FLAG) ? own : up, out);
-        };
         float own_v[8];
         lds_state(own, own_v);
+        // the reads select an address, not a value: hidden from the optimiser, `own` is not
+        // matched with the read above, which would turn the select into a branch per lane group
+        int32_t own_sel = own;
+        asm volatile("" : "+v"(own_sel));
+        auto fetch = [&](int j, uint32_t v, float (&out)[8]) {
+            const int32_t up = (int32_t)v + w.d[j];
+            lds_state((v & BB_FLAG) ? own_sel : up, out);
+        };
         float first[CW_GROUP][8];
 #pragma unroll
         for (int j = 0; j < CW_GROUP; ++j) fetch(j, src[j], first[j]);
