@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "collide.h"
+#include "colour_win.h"
 #include "common.h"
 #include "handle.h"
 #include "pull.h"
@@ -53,35 +54,6 @@ struct ColourStore<float> {
     static __host__ __device__ __forceinline__ double real(int32_t v) { return (double)v * (1.0 / ONE); }
     static __host__ __device__ __forceinline__ int32_t put(double v) { return (int32_t)rint(v * ONE); }
 };
-
-// Per block of the LDS-staged kernels: up to CW_RUNS runs of consecutive node ids
-// [lo, lo + len) copied to LDS slots [off, off + len); unused runs have len 0.  Most blocks need
-// no search for a read: all their nodes lie in one run, at slot n + own, and for each column j of
-// the neighbour table (direction q = j + 1) all the upstream nodes that are not solid lie in one
-// run, at slot id + d[j].  Such a block has every bit of `fast` set; a solid upstream site reads
-// the node's own slot.
-constexpr int CW_RUNS = 4;
-constexpr int CW_BLOCK = 512;   // nodes per block of the staged kernels
-constexpr uint32_t CW_ALL = (1u << (Q - 1)) - 1;
-struct ColourWin {
-    int32_t lo[CW_RUNS];
-    int32_t len[CW_RUNS];
-    int32_t off[CW_RUNS];
-    uint32_t fast;
-    int32_t own;
-    int32_t d[Q - 1];
-};
-static_assert(sizeof(ColourWin) == 128, "one 128-byte record per block");
-
-// node id held by LDS slot s (the runs are packed from slot 0), or -1 past the last run
-__device__ __forceinline__ int32_t cw_slot_node(const ColourWin &w, int32_t s)
-{
-    int32_t id = -1;
-#pragma unroll
-    for (int k = 0; k < CW_RUNS; ++k)
-        if ((uint32_t)(s - w.off[k]) < (uint32_t)w.len[k]) id = w.lo[k] + (s - w.off[k]);
-    return id;
-}
 
 template <typename ST>
 struct CgArgs {
@@ -274,16 +246,12 @@ __global__ void __launch_bounds__(NB) cg_collide_kernel(CgArgs<ST> a)
     ColourWin w{};
     if (SLOTS > 0) w = a.wins[b];
     if (SLOTS > 0 && w.fast == CW_ALL) {   // the same for the whole block
-        // slots threadIdx.x + i NB: all loads in flight before the first LDS write (a slot past
-        // the runs, which nothing reads, gets run 0's first node)
+        // slots threadIdx.x + i NB: all loads in flight before the first LDS write
         constexpr int NS = SLOTS > 0 ? SLOTS / NB : 1;
         static_assert(SLOTS % NB == 0, "whole slots per lane");
         C v[NS];
 #pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * NB);
-            v[i] = a.psi[id >= 0 ? id : w.lo[0]];
-        }
+        for (int i = 0; i < NS; ++i) v[i] = a.psi[cw_slot_node(w, (int32_t)threadIdx.x + i * NB)];
 #pragma unroll
         for (int i = 0; i < NS; ++i) lpsi[threadIdx.x + i * NB] = v[i];
         __syncthreads();
@@ -576,15 +544,13 @@ __global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win(NbrDev nbr, con
     uint32_t src[Q - 1];
     if (n < N) load_nbr(nbr, P, n, src);
     if (w.fast == CW_ALL) {   // the same for the whole block
-        // slots threadIdx.x + i CW_BLOCK: all loads in flight before the first LDS write (a slot
-        // past the runs, which nothing reads, gets run 0's first node)
+        // slots threadIdx.x + i CW_BLOCK: all loads in flight before the first LDS write
         constexpr int NS = SLOTS / CW_BLOCK;
         static_assert(SLOTS % CW_BLOCK == 0, "whole slots per lane");
         f4 ra[NS], rc[NS];
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-            const int32_t id = cw_slot_node(w, (int32_t)threadIdx.x + i * CW_BLOCK);
-            const int64_t g = id >= 0 ? id : w.lo[0];
+            const int64_t g = cw_slot_node(w, (int32_t)threadIdx.x + i * CW_BLOCK);
             ra[i] = cs4[g];
             rc[i] = cs4[P + g];
         }
@@ -831,86 +797,6 @@ int cg_kmode(StateMode m) { return m == STATE_INIT ? K_INIT : (m == STATE_DIRECT
 
 }  // namespace
 
-// The runs of node ids each block of the staged kernels copies to LDS: the block's own nodes and
-// their upstream nodes, sorted, split where consecutive ids are more than RUN_GAP apart (ids in
-// such a gap are copied too: a gap of 128 joins the far end of a z-line that a node at z = 0
-// reads across the periodic wrap), and the runs with the most reads taken first while they fit
-// in `slots` states and CW_RUNS runs.  Then the slot offsets of the blocks that read every state
-// from these runs (ColourWin::fast).  Returns the fraction of such blocks.
-static double build_colour_windows(const SparseGeometry &g, int64_t grid, int slots, std::vector<ColourWin> &out)
-{
-    constexpr uint32_t RUN_GAP = 128;
-    out.assign((size_t)grid, ColourWin{});
-    int64_t nfast = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nfast)
-    for (int64_t b = 0; b < grid; ++b) {
-        std::vector<uint32_t> ids;
-        ids.reserve((size_t)CW_BLOCK * Q);
-        const int64_t n1 = std::min<int64_t>(g.N, (b + 1) * CW_BLOCK);
-        for (int64_t n = b * CW_BLOCK; n < n1; ++n) {
-            ids.push_back((uint32_t)n);
-            for (int q = 1; q < Q; ++q) {
-                const uint32_t v = g.nbr[(size_t)((q - 1) * g.P + n)];
-                ids.push_back((v & BB_FLAG) ? (uint32_t)n : v);
-            }
-        }
-        std::sort(ids.begin(), ids.end());
-        struct Run {
-            uint32_t lo, hi;
-            int64_t reads;
-        };
-        std::vector<Run> runs;
-        for (size_t i = 0; i < ids.size(); ++i) {
-            if (runs.empty() || ids[i] > runs.back().hi - 1 + RUN_GAP)
-                runs.push_back(Run{ids[i], ids[i] + 1, 0});
-            runs.back().hi = ids[i] + 1;
-            ++runs.back().reads;
-        }
-        std::sort(runs.begin(), runs.end(), [](const Run &x, const Run &y) { return x.reads > y.reads; });
-        ColourWin w{};
-        int used = 0, k = 0;
-        for (const Run &r : runs) {
-            const int len = (int)(r.hi - r.lo);
-            if (k < CW_RUNS && used + len <= slots) {
-                w.lo[k] = (int32_t)r.lo;
-                w.len[k] = len;
-                w.off[k] = used;
-                used += len;
-                ++k;
-            }
-        }
-        // one run for all the block's nodes, and per table column one run for all the upstream
-        // nodes that are not solid (a column of solid sites only takes run 0, unused)
-        auto run_of = [&w](uint32_t id) {
-            for (int r = 0; r < CW_RUNS; ++r)
-                if (id - (uint32_t)w.lo[r] < (uint32_t)w.len[r]) return r;
-            return -1;
-        };
-        int own = run_of((uint32_t)(b * CW_BLOCK));
-        for (int64_t n = b * CW_BLOCK; n < n1 && own >= 0; ++n)
-            if (run_of((uint32_t)n) != own) own = -1;
-        if (own >= 0) {
-            w.own = w.off[own] - w.lo[own];
-            for (int j = 0; j < Q - 1; ++j) {
-                int kj = -1;
-                bool one = true;
-                for (int64_t n = b * CW_BLOCK; n < n1 && one; ++n) {
-                    const uint32_t v = g.nbr[(size_t)(j * g.P + n)];
-                    if (v & BB_FLAG) continue;
-                    const int r = run_of(v);
-                    one = r >= 0 && (kj < 0 || r == kj);
-                    kj = r;
-                }
-                if (!one) continue;
-                w.fast |= 1u << j;
-                w.d[j] = kj < 0 ? 0 : w.off[kj] - w.lo[kj];
-            }
-        }
-        nfast += w.fast == CW_ALL;
-        out[(size_t)b] = w;
-    }
-    return grid ? (double)nfast / (double)grid : 1.0;
-}
 
 CgSolver::~CgSolver() { release(); }
 

@@ -6,9 +6,13 @@
 // into device tables: the sparse numbering and pull neighbour lists (build_sparse_geometry),
 // their packed device form (pack_neighbours, which decodes every entry back and compares), the
 // wrap-round connectivity test, the reference's rank split, the face-rule resolution
-// (build_boundaries), and the INPUT / geometry / phase / checkpoint / VTK readers and writers.
+// (build_boundaries), the colour-gradient kernels' LDS staging tables (build_colour_windows:
+// every slot a kernel can read holds the node it expects), and the INPUT / geometry / phase /
+// checkpoint / VTK readers and writers.
 // No device is touched.  Exits non-zero on a wrong result; the sanitizers abort on a memory or
 // undefined-behaviour error.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -16,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "colour_win.h"
 #include "handle.h"
 #include "host_io.h"
 #include "lattice.h"
@@ -42,6 +47,50 @@ std::vector<int32_t> random_solid(int nx, int ny, int nz, double porosity, std::
     return s;
 }
 
+// The staged colour-gradient kernels' tables: runs packed from slot 0 within the slots and the
+// node range; every slot maps to a node; in a block that reads without a search, each node's own
+// slot and, for every column whose upstream site is not solid, the slot id + d[j] hold exactly
+// that node.
+void check_colour_windows(const SparseGeometry &g, int nx, int ny, int nz)
+{
+    const int64_t grid = (g.N + CW_BLOCK - 1) / CW_BLOCK;
+    for (int slots : {1024, 1536, 2560}) {
+        std::vector<ColourWin> wins;
+        const double fast = build_colour_windows(g, grid, slots, wins);
+        bool ok = wins.size() == (size_t)grid;
+        int64_t nfast = 0;
+        for (int64_t b = 0; b < grid && ok; ++b) {
+            const ColourWin &w = wins[(size_t)b];
+            int32_t used = 0;
+            for (int k = 0; k < CW_RUNS; ++k) {
+                ok = ok && w.off[k] == used && w.len[k] >= 0 &&
+                     (w.len[k] == 0 || (w.lo[k] >= 0 && (int64_t)w.lo[k] + w.len[k] <= g.N));
+                used += w.len[k];
+            }
+            ok = ok && used <= slots;
+            for (int32_t s = 0; s < slots && ok; ++s) {
+                const int32_t id = cw_slot_node(w, s);
+                ok = id >= 0 && id < g.N;
+            }
+            if (!ok || w.fast != CW_ALL) continue;
+            ++nfast;
+            const int64_t n1 = std::min<int64_t>(g.N, (b + 1) * CW_BLOCK);
+            for (int64_t n = b * CW_BLOCK; n < n1 && ok; ++n) {
+                const int32_t own = (int32_t)n + w.own;
+                ok = own >= 0 && own < used && cw_slot_node(w, own) == (int32_t)n;
+                for (int j = 0; j < Q - 1 && ok; ++j) {
+                    const uint32_t v = g.nbr[(size_t)j * g.P + n];
+                    if (v & BB_FLAG) continue;
+                    const int32_t s = (int32_t)v + w.d[j];
+                    ok = s >= 0 && s < used && cw_slot_node(w, s) == (int32_t)v;
+                }
+            }
+        }
+        expect(ok, "colour staging tables", nx, ny, nz);
+        expect(ok && std::fabs((double)nfast / (double)grid - fast) < 1e-12, "colour staging fast fraction", nx, ny, nz);
+    }
+}
+
 void check_geometry(int nx, int ny, int nz, const std::vector<int32_t> &solid)
 {
     SparseGeometry g;
@@ -66,6 +115,7 @@ void check_geometry(int nx, int ny, int nz, const std::vector<int32_t> &solid)
     PackedNbr packed;
     expect(pack_neighbours(g, packed, err) == LBM_OK, "packed neighbour table decodes back", nx, ny, nz);
     for (int a = 0; a < 3; ++a) (void)pores_wrap_along(g, a);
+    check_colour_windows(g, nx, ny, nz);
     for (int k = 1; k <= std::min(nx, 5); ++k) {
         std::vector<int64_t> ends;
         reference_rank_ends(g, k, ends);
