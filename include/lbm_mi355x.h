@@ -339,7 +339,7 @@ double lbm_srv_clock(const lbm_handle *h);
 #define LBM_ORDER_SLAB 2          /* each XCD sweeps one contiguous slab of tiles */
 #define LBM_VARIANT_GATHER 0      /* colour-gradient kernels: 18 neighbour gathers per node through the caches */
 #define LBM_VARIANT_STAGED 1      /* the block's neighbourhood staged in LDS first (block 512; the colour
-                                     streaming in FP32 only, the collide's phase field in both) */
+                                     states for the colour streaming, the phase field for the collide) */
 typedef struct lbm_launch {
     int32_t block;     /* threads (nodes) per workgroup: 128, 256 or 512 */
     int32_t order;     /* LBM_ORDER_* */

@@ -314,58 +314,73 @@ __device__ __forceinline__ void colour_pull_term(const C (&v)[8], bool bb, C &gr
     gb -= mn * en;
 }
 
-// The colour sums are formed in FP64 whatever the storage type: each node's rho_r is split
-// over 19 directions and re-summed at 19 receivers, and in FP32 the rounding of
-// 1 - 1.5 u.u + ... next to 1 is biased enough to drift the total colour mass (about 3e-6 of
-// it per 200 steps); in FP64 the sums are exact to ~1e-16 and only the unbiased rounding of
-// the stored totals remains.  The gathers are not staged through LDS: windows of colour states
-// around the block's node range and its two x-neighbour planes catch 99.6 % of the reads on a
-// 192^3 pack, but measured slower (148 us against 119 us for this form), the occupancy lost to
-// the windows costing more than the L2 traffic they save.
-template <typename C>
-__global__ void __launch_bounds__(BLOCK) cg_colour_kernel(NbrDev nbr, const C *__restrict__ cs,
-                                                          C *__restrict__ col, C *__restrict__ psi, int64_t N, int64_t P,
-                                                          int32_t order)
+// The FP64 build's colour sums of one node, direction by direction: begin() with the node's own
+// colour state (the rest population stays on the node: e_0 = 0, no recolouring term), add() the
+// term arriving along q from an upstream state (bb: BB_FLAG when the upstream site is solid and v
+// is the node's own state, whose opposite term comes back), then store().  The gather and the
+// staged kernel add the same terms in the same order, so they give the same bits.
+struct ColourSums64 {
+    double rr, rb;
+    __device__ __forceinline__ void begin(const double (&own)[8])
+    {
+        const double uu = own[2] * own[2] + own[3] * own[3] + own[4] * own[4];
+        const double w0 = weight(0) * (1.0 - 1.5 * uu);
+        rr = w0 * own[0];
+        rb = w0 * own[1];
+    }
+    template <int q>
+    __device__ __forceinline__ void add(uint32_t bb, const double (&v)[8])
+    {
+        double gr, gb;
+        colour_pull_term<q>(v, bb != 0, gr, gb);
+        rr += gr;
+        rb += gb;
+    }
+    __device__ __forceinline__ void store(int64_t n, int64_t P, double *__restrict__ col, double *__restrict__ psi) const
+    {
+        col[n] = rr;
+        col[P + n] = rb;
+        psi[n] = (rr - rb) / (rr + rb);
+    }
+};
+
+// FP64 gathers of the 18 upstream colour states: all 18 source indices first, then the states in
+// groups of three directions with no branch between them, so that each group's gathers are in
+// flight together (measured best: 351 us at 3, 402 us at 6).  A solid upstream site sends back
+// this node's own opposite term: same load address n, sign -1.
+__device__ __forceinline__ void colour_gather64(const double *__restrict__ cs, const uint32_t (&src)[Q - 1], int64_t n,
+                                                int64_t P, ColourSums64 &acc)
 {
-    const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-    if (n >= N) return;
-    C own_s[8];
-    load_cs(cs, n, P, own_s);
-    double own[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) own[k] = (double)own_s[k];
-    // rest population stays on the node (e_0 = 0: no recolouring term)
-    const double uu = own[2] * own[2] + own[3] * own[3] + own[4] * own[4];
-    const double w0 = weight(0) * (1.0 - 1.5 * uu);
-    double rr = w0 * own[0], rb = w0 * own[1];
-    // All 18 source indices first, then the colour states in groups of G directions with no
-    // branch between them, so that each group's gathers are in flight together.  A solid
-    // upstream site sends back this node's own opposite term: same load address n, sign -1.
-    uint32_t src[Q - 1];
-    load_nbr(nbr, P, n, src);
-    constexpr int G = sizeof(C) == 4 ? 18 : 3;   // measured best per type (FP64: 351 us at 3, 402 us at 6)
+    constexpr int G = 3;
     static_for<0, (Q - 1) / G>([&](auto g) {
         constexpr int q0 = 1 + decltype(g)::value * G;
-        C up_s[G][8];
+        double up[G][8];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             const uint32_t v = src[q0 + j - 1];
-            load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, up_s[j]);
+            load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, up[j]);
         }
         static_for<0, G>([&](auto jj) {
             constexpr int j = decltype(jj)::value;
-            double up[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) up[k] = (double)up_s[j][k];
-            double gr, gb;
-            colour_pull_term<q0 + j>(up, (src[q0 + j - 1] & BB_FLAG) != 0, gr, gb);
-            rr += gr;
-            rb += gb;
+            acc.add<q0 + j>(src[q0 + j - 1] & BB_FLAG, up[j]);
         });
     });
-    col[n] = (C)rr;
-    col[P + n] = (C)rb;
-    psi[n] = (C)((rr - rb) / (rr + rb));
+}
+
+__global__ void __launch_bounds__(BLOCK) cg_colour_kernel64(NbrDev nbr, const double *__restrict__ cs,
+                                                            double *__restrict__ col, double *__restrict__ psi,
+                                                            int64_t N, int64_t P, int32_t order)
+{
+    const int64_t n = tile_of(order, blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n >= N) return;
+    double own[8];
+    load_cs(cs, n, P, own);
+    uint32_t src[Q - 1];
+    load_nbr(nbr, P, n, src);
+    ColourSums64 acc;
+    acc.begin(own);
+    colour_gather64(cs, src, n, P, acc);
+    acc.store(n, P, col, psi);
 }
 
 // The FP32 build's colour streaming.  Each node's stored colour R (fixed point, units of 2^-29)
@@ -470,30 +485,31 @@ struct ColourStep<Q> {
 #define CW_GROUP 3
 #endif
 static_assert((Q - 1) % CW_GROUP == 0, "groups cover the 18 directions");
-template <int q, int i = 0>
+// (T: the state's value type; Acc: ColourSums or ColourSums64)
+template <int q, typename T, typename Acc, int i = 0>
 struct ColourAdd {
-    static __device__ __forceinline__ void run(ColourSums &acc, const uint32_t (&src)[Q - 1], const float (&v)[CW_GROUP][8])
+    static __device__ __forceinline__ void run(Acc &acc, const uint32_t (&src)[Q - 1], const T (&v)[CW_GROUP][8])
     {
-        acc.add<q + i>(src[q + i - 1] & BB_FLAG, v[i]);
-        if constexpr (i + 1 < CW_GROUP) ColourAdd<q, i + 1>::run(acc, src, v);
+        acc.template add<q + i>(src[q + i - 1] & BB_FLAG, v[i]);
+        if constexpr (i + 1 < CW_GROUP) ColourAdd<q, T, Acc, i + 1>::run(acc, src, v);
     }
 };
-template <int q>
+template <int q, typename T, typename Acc>
 struct ColourGroups {
     template <typename F>
-    static __device__ __forceinline__ void run(ColourSums &acc, const uint32_t (&src)[Q - 1], const float (&cur)[CW_GROUP][8],
+    static __device__ __forceinline__ void run(Acc &acc, const uint32_t (&src)[Q - 1], const T (&cur)[CW_GROUP][8],
                                                F &&fetch)
     {
         constexpr int qn = q + CW_GROUP;
-        float nxt[CW_GROUP][8];
+        T nxt[CW_GROUP][8];
         if constexpr (qn < Q) {
 #pragma unroll
             for (int j = 0; j < CW_GROUP; ++j) fetch(qn + j - 1, src[qn + j - 1], nxt[j]);
         }
         __builtin_amdgcn_sched_barrier(0);
-        ColourAdd<q>::run(acc, src, cur);
+        ColourAdd<q, T, Acc>::run(acc, src, cur);
         __builtin_amdgcn_sched_barrier(0);
-        if constexpr (qn < Q) ColourGroups<qn>::run(acc, src, nxt, fetch);
+        if constexpr (qn < Q) ColourGroups<qn, T, Acc>::run(acc, src, nxt, fetch);
     }
 };
 
@@ -584,7 +600,7 @@ __global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win(NbrDev nbr, con
 #pragma unroll
         for (int j = 0; j < CW_GROUP; ++j) fetch(j, src[j], first[j]);
         acc.begin(own_v);
-        ColourGroups<1>::run(acc, src, first, fetch);
+        ColourGroups<1, float, ColourSums>::run(acc, src, first, fetch);
         acc.store(n, P, col, psi);
         return;
     }
@@ -594,6 +610,74 @@ __global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win(NbrDev nbr, con
     acc.begin(own);
     ColourStep<1>::run(acc, src,
                        [&](int, uint32_t v, float (&out)[8]) { load_cs(cs, (v & BB_FLAG) ? n : (int64_t)v, P, out); });
+    acc.store(n, P, col, psi);
+}
+
+// The FP64 colour streaming with the same staging: a 64-byte state per LDS slot, in four rows of
+// 16-byte quarters (so a wavefront's read of a quarter is 1 KB contiguous); 2560 slots fill the
+// CU's 160 KB, one block per CU.
+template <int SLOTS>
+__global__ void __launch_bounds__(CW_BLOCK) cg_colour_kernel_win64(NbrDev nbr, const double *__restrict__ cs,
+                                                                 const ColourWin *__restrict__ wins,
+                                                                 double *__restrict__ col, double *__restrict__ psi,
+                                                                 int64_t N, int64_t P, int32_t order)
+{
+    using d2 = double __attribute__((ext_vector_type(2)));
+    __shared__ d2 lds[4 * SLOTS];
+    const int64_t b = tile_of(order, blockIdx.x, gridDim.x);
+    const ColourWin w = wins[b];
+    const d2 *__restrict__ cs2 = reinterpret_cast<const d2 *>(cs);   // node m: cs2[4 m .. 4 m + 3]
+    const int64_t n = b * CW_BLOCK + threadIdx.x;
+    uint32_t src[Q - 1];
+    if (n < N) load_nbr(nbr, P, n, src);   // lanes past N load nothing (see cg_colour_kernel_win)
+    if (w.fast == CW_ALL) {   // the same for the whole block
+        constexpr int NS = SLOTS / CW_BLOCK;
+        static_assert(SLOTS % CW_BLOCK == 0, "whole slots per lane");
+        d2 r[NS][4];
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int64_t g = cw_slot_node(w, (int32_t)threadIdx.x + i * CW_BLOCK);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r[i][c] = cs2[4 * g + c];
+        }
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) lds[c * SLOTS + threadIdx.x + i * CW_BLOCK] = r[i][c];
+        __syncthreads();
+    }
+    if (n >= N) return;
+    auto lds_state = [&](int32_t slot, double (&v)[8]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const d2 x = lds[c * SLOTS + slot];
+            v[2 * c] = x.x;
+            v[2 * c + 1] = x.y;
+        }
+    };
+    ColourSums64 acc;
+    if (w.fast == CW_ALL) {
+        const int32_t own = (int32_t)n + w.own;
+        double own_v[8];
+        lds_state(own, own_v);
+        int32_t own_sel = own;   // kept apart from `own`, as in cg_colour_kernel_win
+        asm volatile("" : "+v"(own_sel));
+        auto fetch = [&](int j, uint32_t v, double (&out)[8]) {
+            const int32_t up = (int32_t)v + w.d[j];
+            lds_state((v & BB_FLAG) ? own_sel : up, out);
+        };
+        double first[CW_GROUP][8];
+#pragma unroll
+        for (int j = 0; j < CW_GROUP; ++j) fetch(j, src[j], first[j]);
+        acc.begin(own_v);
+        ColourGroups<1, double, ColourSums64>::run(acc, src, first, fetch);
+        acc.store(n, P, col, psi);
+        return;
+    }
+    double own[8];
+    load_cs(cs, n, P, own);
+    acc.begin(own);
+    colour_gather64(cs, src, n, P, acc);
     acc.store(n, P, col, psi);
 }
 
@@ -1028,16 +1112,24 @@ hipError_t CgSolver::launch_colour(const lbm_launch &ls)
 {
     const int64_t N = geo_.N, P = geo_.P;
     const dim3 g((unsigned)grid_), b(BLOCK);
+    const dim3 gw((unsigned)cw_grid_), bw(CW_BLOCK);
+    const ColourWin *w = (const ColourWin *)d_cwin_;
     if (prm_.precision == LBM_FP64) {
-        cg_colour_kernel<double><<<g, b, 0, stream_>>>(nbr_.view(), (const double *)d_cs_, (double *)d_col_,
-                                                        (double *)d_psi_, N, P, ls.order);
+        const double *cs = (const double *)d_cs_;
+        double *col = (double *)d_col_, *psi = (double *)d_psi_;
+        if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1024)
+            cg_colour_kernel_win64<1024><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+        else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1536)
+            cg_colour_kernel_win64<1536><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+        else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 2560)
+            cg_colour_kernel_win64<2560><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
+        else
+            cg_colour_kernel64<<<g, b, 0, stream_>>>(nbr_.view(), cs, col, psi, N, P, ls.order);
         return hipGetLastError();
     }
-    const ColourWin *w = (const ColourWin *)d_cwin_;
     const float *cs = (const float *)d_cs_;
     int32_t *col = (int32_t *)d_col_;
     float *psi = (float *)d_psi_;
-    const dim3 gw((unsigned)cw_grid_), bw(CW_BLOCK);
     if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1024)
         cg_colour_kernel_win<1024><<<gw, bw, 0, stream_>>>(nbr_.view(), cs, w, col, psi, N, P, ls.order);
     else if (ls.variant == LBM_VARIANT_STAGED && cwin_slots_ == 1536)
@@ -1112,9 +1204,8 @@ int CgSolver::autotune()
                             best));
     launch_collide_ = best;
     auto cv = tune_candidates(launch_colour_, {BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, {LBM_VARIANT_GATHER});
-    if (prm_.precision == LBM_FP32)
-        for (const lbm_launch &l : tune_candidates(launch_colour_, {CW_BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
-            if (l.variant == LBM_VARIANT_STAGED) cv.push_back(l);
+    for (const lbm_launch &l : tune_candidates(launch_colour_, {CW_BLOCK}, {ORDER_HW, ORDER_RUNS8, ORDER_SLAB}, staged))
+        if (l.variant == LBM_VARIANT_STAGED) cv.push_back(l);
     LBM_HIP_TRY(tune_launch(stream_, cv,
                             [&](const lbm_launch &ls) {
                                 hipError_t e = launch_collide(K_PULL, false, launch_collide_, a, b);
@@ -1147,8 +1238,8 @@ int CgSolver::set_launch(int kernel, const lbm_launch *in)
             return fail(LBM_ERR_ARG, "collide launch: block 128, 256 or 512, order 0-2, variant 0; or block 512, variant 1");
         launch_collide_ = l;
     } else if (kernel == LBM_KERNEL_CG_COLOUR) {
-        if (!(launch_valid(l, {BLOCK}, {0}) || (prm_.precision == LBM_FP32 && launch_valid(l, {CW_BLOCK}, {1}))))
-            return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0; or (FP32) block 512, variant 1");
+        if (!(launch_valid(l, {BLOCK}, {0}) || launch_valid(l, {CW_BLOCK}, {1})))
+            return fail(LBM_ERR_ARG, "colour launch: block 256, order 0-2, variant 0; or block 512, variant 1");
         launch_colour_ = l;
     } else {
         return fail(LBM_ERR_ARG, "a colour-gradient handle launches LBM_KERNEL_CG_COLLIDE and LBM_KERNEL_CG_COLOUR");

@@ -51,7 +51,7 @@ def test_colour_gradient_settings_give_identical_states(pkg, precision):
     _log(f"[gpu] launch settings, colour gradient {precision}")
     solid = pkg.geometry.sphere_pack((48, 40, 36), 4.0, 0.5, seed=3)
     psi = np.where(np.random.default_rng(4).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
-    variants = ("gather", "staged") if precision == "fp32" else ("gather",)
+    variants = ("gather", "staged")
     ref = None
     cases = [(b, o, "gather", "gather") for b, o in itertools.product(BLOCKS, ORDERS)]
     cases += [(256, o, cv, v) for cv, v, o in itertools.product(("gather", "staged"), variants, ORDERS)]
@@ -113,7 +113,7 @@ def test_invalid_settings_are_refused(pkg):
     s.close()
     c = pkg.ColorGradient(solid, 0.1, 0.1, 0.0, 0.01, (1e-5, 0, 0), precision="fp64")
     with pytest.raises(_capi.LBMError):
-        c.set_launch("cg_colour", 512, "slab", "staged")   # the staged variant is FP32 only
+        c.set_launch("cg_colour", 256, "slab", "staged")   # staging is per 512-node block
     with pytest.raises(_capi.LBMError):
         c.set_launch("cg_colour", 512, "slab", "gather")
     with pytest.raises(_capi.LBMError):
