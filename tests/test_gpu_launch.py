@@ -71,6 +71,28 @@ def test_colour_gradient_settings_give_identical_states(pkg, precision):
         s.close()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("shape", [(128, 12, 12), (64, 24, 24)])
+def test_staged_kernels_on_small_planes(pkg, precision, shape):
+    """Boxes of few nodes per x-plane, whose 512-node blocks stage 1024 (128x12x12) and 1536
+    (64x24x24) states rather than 2560: the staged kernels give the gathers' bits."""
+    _log(f"[gpu] staged colour gradient, {precision}, {shape}")
+    solid = pkg.geometry.sphere_pack(shape, 4.0, 0.5, seed=3)
+    psi = np.where(np.random.default_rng(4).uniform(size=solid.shape) < 0.5, 1.0, -1.0)
+    states = []
+    for variant in ("gather", "staged"):
+        s = pkg.ColorGradient(solid, 0.1, 0.05, 0.3, 0.01, (1e-5, 0, 0), precision=precision)
+        block = 512 if variant == "staged" else 256
+        s.set_launch("cg_collide", block, "runs8", variant)
+        s.set_launch("cg_colour", block, "slab", variant)
+        s.init(psi)
+        s.step(20)
+        states.append(s.state())
+        s.close()
+    for a, b in zip(*states):
+        assert np.array_equal(a, b)
+
+
 def test_solute_settings_give_identical_states(pkg):
     _log("[gpu] launch settings, solute")
     solid = _c1_solid()
