@@ -74,7 +74,7 @@ struct CgArgs {
     int32_t bfa;
     float scale, inv_scale;
     int32_t order;                            // tile order of the collide launch (TileOrder)
-    const ColourWin *__restrict__ wins;       // the staged variant's runs per 256-node block
+    const ColourWin *__restrict__ wins;       // the staged variant's runs per CW_BLOCK-node block
 };
 
 // colour state of a node: its stored colour densities (raw) and six values of the compute type.
@@ -232,7 +232,7 @@ __device__ __forceinline__ void cg_collide_node(const CgArgs<ST> &a, int64_t n, 
     store_cs(a.cs, n, a.P, rr_raw, rb_raw, st);
 }
 
-// SLOTS > 0: the staged variant (NB = 256).  The phase field of the block's neighbourhood -- the
+// SLOTS > 0: the staged variant (NB = CW_BLOCK).  The phase field of the block's neighbourhood -- the
 // runs of node ids of the colour kernel's staging (ColourWin) -- is copied to LDS first, and the
 // colour gradient reads its 18 neighbour values from there; a block with a read outside its runs
 // reads them from global memory.
@@ -532,10 +532,10 @@ __global__ void __launch_bounds__(BLOCK) cg_colour_kernel_fx(NbrDev nbr, const f
 }
 
 // The same with the colour states of the block's upstream neighbourhood staged in LDS first.
-// A block's 256 nodes lie on two or three z-lines of one x-plane; their upstream nodes are the
-// nodes of a few neighbouring lines in the planes x-1, x and x+1, i.e. a few runs of consecutive
-// node ids.  The host lists, per block, up to CW_RUNS such runs (ColourWin) that fit in SLOTS
-// states; the block copies them with coalesced loads, then each of its 18 x 256 reads comes
+// A block's CW_BLOCK = 512 nodes lie on five to seven z-lines of one x-plane; their upstream
+// nodes are the nodes of a few neighbouring lines in the planes x-1, x and x+1, i.e. a few runs
+// of consecutive node ids.  The host lists, per block, up to CW_RUNS such runs (ColourWin) that
+// fit in SLOTS states; the block copies them with coalesced loads, then each of its 18 x 512 reads comes
 // from LDS at slot id + d[j] -- one add per read, no search, no branch -- and the reads run a
 // group of directions ahead of the sums.  Each state is copied by the few blocks whose
 // neighbourhoods hold it, instead of being gathered by each of its 18 downstream nodes.  A block
